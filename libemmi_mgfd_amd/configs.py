@@ -1,0 +1,203 @@
+"""Synthetic survey configurations for the multigrid Maxwell solve.
+
+Each configuration is a plain description of the reference's inputs: an input
+resistivity model on a uniform grid (the arrays that ``frho11/frho22/frho33``
+point to, float32 ``[nz][ny][nx]``), a source list, a receiver list, a
+source-receiver table, the frequencies and the multigrid size.  The files the
+reference reads (``src/emf_init_free.c:81-107``, ``src/acq_init_free.c:68-262``)
+can be written with :func:`write_reference_inputs`; the same description feeds
+the MI355X host program and the benchmarks.
+
+Cell-centred classification is used throughout: a cell belongs to a layer
+when its centre lies in it.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+RHO_WATER = 0.3125
+
+
+@dataclasses.dataclass
+class Survey:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ox: float
+    oy: float
+    oz: float
+    rho11: np.ndarray  # float32 [nz][ny][nx]
+    rho22: np.ndarray
+    rho33: np.ndarray
+    sources: List[Tuple[float, float, float, float, float]]    # x, y, z, azimuth deg, dip deg
+    receivers: List[Tuple[float, float, float, float, float]]
+    table: List[Tuple[int, int]]                               # 1-based (isrc, irec)
+    freqs: List[float]
+    n1: int
+    n2: int
+    n3: int
+    chsrc: Sequence[str] = ("Ex",)
+    chrec: Sequence[str] = ("Ex", "Ey", "Hx", "Hy")
+    d1min: float = 100.0
+    d2min: float = 100.0
+    d3min: float = 20.0
+    nb: int = 10
+    lextend: float = 50e3
+    rho_air: float = 1e8
+    bathy: np.ndarray | None = None  # float32 [ny][nx], depth of seafloor
+    extra: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def nshots(self) -> int:
+        return len(self.sources)
+
+    def getpar_args(self, files: dict) -> List[str]:
+        """key=value arguments in the reference's getpar convention."""
+        a = [
+            "freqs=" + ",".join(_fmt(f) for f in self.freqs),
+            "chsrc=" + ",".join(self.chsrc),
+            "chrec=" + ",".join(self.chrec),
+            f"nx={self.nx}", f"ny={self.ny}", f"nz={self.nz}",
+            f"dx={_fmt(self.dx)}", f"dy={_fmt(self.dy)}", f"dz={_fmt(self.dz)}",
+            f"ox={_fmt(self.ox)}", f"oy={_fmt(self.oy)}", f"oz={_fmt(self.oz)}",
+            f"n1={self.n1}", f"n2={self.n2}", f"n3={self.n3}",
+            f"d1min={_fmt(self.d1min)}", f"d2min={_fmt(self.d2min)}", f"d3min={_fmt(self.d3min)}",
+            f"nb={self.nb}", f"lextend={_fmt(self.lextend)}", f"rho_air={_fmt(self.rho_air)}",
+            f"frho11={files['frho11']}", f"frho22={files['frho22']}", f"frho33={files['frho33']}",
+            f"fsrc={files['fsrc']}", f"frec={files['frec']}", f"fsrcrec={files['fsrcrec']}",
+        ]
+        if "fbathy" in files:
+            a.append(f"fbathy={files['fbathy']}")
+        return a
+
+
+def _fmt(v: float) -> str:
+    r = repr(float(v))
+    return r[:-2] if r.endswith(".0") else r
+
+
+def write_reference_inputs(s: Survey, d: str) -> dict:
+    """Write the model and acquisition files in the reference's formats."""
+    os.makedirs(d, exist_ok=True)
+    files = {}
+    for key, arr in (("frho11", s.rho11), ("frho22", s.rho22), ("frho33", s.rho33)):
+        p = os.path.join(d, key + ".bin")
+        np.ascontiguousarray(arr, dtype=np.float32).tofile(p)
+        files[key] = p
+    p = os.path.join(d, "sources.txt")
+    with open(p, "w") as fp:
+        fp.write(" x y z azimuth dip isrc\n")
+        for i, (x, y, z, az, dip) in enumerate(s.sources):
+            fp.write(f"{x!r} {y!r} {z!r} {az!r} {dip!r} {i + 1}\n")
+    files["fsrc"] = p
+    p = os.path.join(d, "receivers.txt")
+    with open(p, "w") as fp:
+        fp.write(" x y z azimuth dip irec\n")
+        for i, (x, y, z, az, dip) in enumerate(s.receivers):
+            fp.write(f"{x!r} {y!r} {z!r} {az!r} {dip!r} {i + 1}\n")
+    files["frec"] = p
+    p = os.path.join(d, "src_rec_table.txt")
+    with open(p, "w") as fp:
+        fp.write(" isrc irec\n")
+        for isrc, irec in s.table:
+            fp.write(f"{isrc} {irec}\n")
+    files["fsrcrec"] = p
+    if s.bathy is not None:
+        p = os.path.join(d, "bathy.bin")
+        np.ascontiguousarray(s.bathy, dtype=np.float32).tofile(p)
+        files["fbathy"] = p
+    return files
+
+
+def _centres(n, d, o):
+    return o + (np.arange(n) + 0.5) * d
+
+
+def _layered(nx, ny, nz, dx, dy, dz, ox, oy, oz, seafloor, layers, rhov_sed=1.0, blocks=()):
+    """Water above ``seafloor(x, y)``, sediment below, plus resistive layers
+    ``(ztop, zbot, rho)`` and boxes ``(xc, yc, zc, wx, wy, wz, rho)``."""
+    xc = _centres(nx, dx, ox)
+    yc = _centres(ny, dy, oy)
+    zc = _centres(nz, dz, oz)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    sf = seafloor(X, Y)
+    rh = np.where(Z < sf, RHO_WATER, 1.0)
+    rv = np.where(Z < sf, RHO_WATER, rhov_sed)
+    for ztop, zbot, rho in layers:
+        m = (Z >= ztop) & (Z < zbot) & (Z >= sf)
+        rh = np.where(m, rho, rh)
+        rv = np.where(m, rho, rv)
+    for xcen, ycen, zcen, wx, wy, wz, rho in blocks:
+        m = ((np.abs(X - xcen) < 0.5 * wx) & (np.abs(Y - ycen) < 0.5 * wy)
+             & (np.abs(Z - zcen) < 0.5 * wz) & (Z >= sf))
+        rh = np.where(m, rho, rh)
+        rv = np.where(m, rho, rv)
+    bathy = seafloor(X[0], Y[0]).astype(np.float32)
+    return rh.astype(np.float32), rv.astype(np.float32), bathy
+
+
+def config1(n=64) -> Survey:
+    """Plain FP64 forward modelling, 1 shot at 0.25 Hz, MG grid 64^3."""
+    nx = ny = 101
+    nz = 51
+    dx = dy = 100.0
+    dz = 40.0
+    ox = oy = -5050.0
+    rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0,
+                             lambda x, y: np.full_like(x, 1000.0),
+                             [(1480.0, 1600.0, 100.0)])
+    recs = [(float(x), 0.0, 1000.0, 0.0, 0.0) for x in np.arange(-5000.0, 5000.0 + 1, 100.0)]
+    return Survey("config1", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh.copy(), rv,
+                  [(0.0, 0.0, 950.0, 0.0, 0.0)], recs, [(1, i + 1) for i in range(len(recs))],
+                  [0.25], n, n, n, bathy=bathy)
+
+
+def _seafloor2(x, y):
+    return 1000.0 + 100.0 * np.sin(2 * np.pi * x / 4000.0) * np.cos(2 * np.pi * y / 6000.0)
+
+
+def config2(n=128) -> Survey:
+    """Anisotropic sediment, sinusoidal seafloor, one 100 ohm-m block; 3 freqs at MG 128^3."""
+    nx = ny = 201
+    nz = 101
+    dx = dy = 50.0
+    dz = 20.0
+    ox = oy = -5025.0
+    rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0, _seafloor2,
+                             [(1480.0, 1600.0, 100.0)], rhov_sed=2.0,
+                             blocks=[(1000.0, 0.0, 1550.0, 4000.0, 4000.0, 100.0, 100.0)])
+    zs = float(_seafloor2(np.float64(0.0), np.float64(0.0))) - 50.0
+    xr = np.arange(-5000.0, 5000.0 + 1, 50.0)
+    recs = [(float(x), 0.0, float(_seafloor2(x, 0.0)), 0.0, 0.0) for x in xr]
+    return Survey("config2", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh.copy(), rv,
+                  [(0.0, 0.0, zs, 0.0, 0.0)], recs, [(1, i + 1) for i in range(len(recs))],
+                  [0.25, 0.75, 1.25], n, n, n, bathy=bathy)
+
+
+def tiny(n1=12, n2=10, n3=16, nrec=5) -> Survey:
+    """A small anisotropic survey for kernel-level known-answer tests; the
+    multigrid sizes are deliberately unequal so a transposed index fails."""
+    nx, ny, nz = 24, 20, 10
+    dx = dy = 200.0
+    dz = 100.0
+    ox, oy = -2400.0, -2000.0
+    rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0,
+                             lambda x, y: 500.0 + 40.0 * np.sin(2 * np.pi * x / 3000.0),
+                             [(700.0, 800.0, 50.0)], rhov_sed=1.7)
+    recs = [(float(x), 100.0, 520.0, 10.0 * i, 3.0 * i) for i, x in enumerate(np.linspace(-1800, 1800, nrec))]
+    return Survey("tiny", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh * np.float32(1.1), rv,
+                  [(100.0, -50.0, 450.0, 30.0, 5.0)], recs, [(1, i + 1) for i in range(len(recs))],
+                  [0.5, 1.5], n1, n2, n3, d1min=200.0, d2min=200.0, d3min=60.0, nb=3,
+                  lextend=8000.0, bathy=bathy)
+
+
+CONFIGS = {"config1": config1, "config2": config2, "tiny": tiny}
