@@ -1,0 +1,203 @@
+/* emmi_mgfd.h -- C-ABI of the MI355X multigrid Maxwell solve.
+ *
+ * Two layers:
+ *  1. the reference's own entry points for this path, same names, argument
+ *     meaning and ownership (the callers declare them locally at
+ *     src/do_modelling.c:14-26 and src/fg_fwi.c:19-41 of libEMMI_MGFD), so a
+ *     host program built against libEMMI_MGFD links this library instead;
+ *  2. an extended device-resident API (emmi_*) with explicit handles, status
+ *     codes, initial guesses and batches of right-hand sides.
+ *
+ * Layouts are the reference's: field vectors are complex double
+ * [3][n3+1][n2+1][n1+1] with i fastest; cell arrays [n3][n2][n1]; input-grid
+ * arrays [nz][ny][nx].
+ */
+#ifndef EMMI_MGFD_H
+#define EMMI_MGFD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+typedef struct { double re, im; } emmi_c128;   /* layout of double _Complex */
+typedef struct { float re, im; } emmi_c64;     /* layout of float _Complex */
+#else
+typedef double _Complex emmi_c128;
+typedef float _Complex emmi_c64;
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* carriers, layout-identical to include/emf.h:4-48, include/acq.h:4-19 and  */
+/* include/fwi.h:4-28 of libEMMI_MGFD (3D/4D arrays are pointer tables over  */
+/* one contiguous block, &a[0][0][0] being its base)                          */
+/* ------------------------------------------------------------------------- */
+#ifndef EMMI_NO_CARRIERS
+typedef struct {
+  int mode, verb, reciprocity;
+  int nfreq;
+  float *freqs, *omegas;
+  float rho_air, rho_water, lextend;
+  int nchsrc, nchrec;
+  char **chsrc, **chrec;
+  float x1min, x1max, x2min, x2max, x3min, x3max;
+  float *xx, *yy, *zz;
+  int n1, n2, n3;
+  float d1min, d2min, d3min;
+  float *x1, *x2, *x3;
+  float *x1s, *x2s, *x3s;
+  float *d1, *d2, *d3;
+  float *d1s, *d2s, *d3s;
+  float ***sigma11, ***sigma22, ***sigma33, ***invmur;
+  emmi_c64 ***dcal_fd, ***dobs_fd, ***dres_fd;
+  emmi_c64 ****E1, ****E2, ****E3;
+  emmi_c64 ****H1, ****H2, ****H3;
+  float ***mur, ***rho11, ***rho22, ***rho33;
+  float **bathy;
+  float rhomin, rhomax, depthmax;
+  float ox, oy, oz;
+  float dx, dy, dz;
+  int nx, ny, nz, nb;
+  int addnoise;
+  float amp_perc, delta_phi, delta_theta;
+  float noisefloorE, noisefloorH;
+  float ***delta_emf;
+  float ***rmse;
+  float dsmute, drmute;
+} emf_t;
+
+typedef struct {
+  int nsrc, nrec, nsubsrc, nsubrec;
+  float lensrc, lenrec;
+  float *src_x1, *src_x2, *src_x3, *src_azimuth, *src_dip;
+  float *rec_x1, *rec_x2, *rec_x3, *rec_azimuth, *rec_dip;
+  float x1min, x1max, x2min, x2max, x3min, x3max;
+  int nsrc_total, nrec_total;
+  int *shot_idx;
+} acq_t;
+
+typedef struct {
+  int npar, n, preco, r1, r2, r3, repeat;
+  int *idxpar;
+  float *minpar, *maxpar;
+  int iter, niter;
+  float *xref;
+  float gamma1, gamma2;
+  int firstgrad;
+  float alpha;
+  float fcost_dat, fcost_mod, fcost;
+  float *fcost_list;
+  float ****grad, ****phess, ***Hv;
+  float mse;
+} fwi_t;
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* 1. drop-in entry points (reference semantics)                              */
+/* ------------------------------------------------------------------------- */
+
+/* replaces src/find_index.c:9-29: k with x[k] <= val < x[k+1] by bisection */
+int find_index(int n, float *x, float val);
+/* replaces src/create_nugrid.c:13-36: geometric grid, returns the ratio */
+float create_nugrid(int n, float len, float dx, float *x);
+/* replaces src/homogenization.c:15-111: overlap-weighted volume average */
+void homogenization(float ***in, float ***out, int nx, int ny, int nz, float *xx, float *yy, float *zz,
+                    int n1, int n2, int n3, float *x1, float *x2, float *x3);
+/* replaces src/regrid_init_free.c:37-232 / :235-262: stretched MG grid around
+ * the source and homogenised medium (bit-exact) */
+void regrid_init(acq_t *acq, emf_t *emf);
+void regrid_free(emf_t *emf);
+
+/* replaces src/gmg.c:1704-1733: reads cycleopt, ncycle, v1, v2, tol,
+ * isemicoarsen (and the extensions precision=, coarse_sweeps=) through
+ * getpar; keeps emf (borrowed, must outlive gmg_apply) */
+void gmg_init(emf_t *emf, int ifreq);
+/* replaces src/gmg.c:1894-1932: solves from a zero guess on the GPU; x <- E,
+ * b <- H; n must be 3(n1+1)(n2+1)(n3+1).  Host pointers. */
+void gmg_apply(int n, emmi_c128 *b, emmi_c128 *x);
+/* replaces src/gmg.c:1887-1890 */
+void gmg_free(void);
+
+/* replace src/inject_extract.c:26-120, :124-314, :317-474, :477-635,
+ * :638-785 (host vectors, bit-exact) */
+void inject_source(acq_t *acq, emf_t *emf, emmi_c128 *b, int ifreq);
+void inject_adj_source(acq_t *acq, emf_t *emf, emmi_c128 *x, emmi_c128 *b, int ifreq);
+void extract_emf_data(acq_t *acq, emf_t *emf, emmi_c128 *x, emmi_c128 *b, int ifreq);
+void extract_electric_field(acq_t *acq, emf_t *emf, emmi_c128 *x, int ifreq);
+void extract_magnetic_field(acq_t *acq, emf_t *emf, emmi_c128 *b, int ifreq);
+/* replaces src/build_gradient.c:13-111 */
+void build_gradient(acq_t *acq, emf_t *emf, fwi_t *fwi,
+                    emmi_c64 ****E1f, emmi_c64 ****E2f, emmi_c64 ****E3f,
+                    emmi_c64 ****E1a, emmi_c64 ****E2a, emmi_c64 ****E3a);
+
+/* ------------------------------------------------------------------------- */
+/* 2. extended API                                                            */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int cycleopt;       /* 1 V, 2 F (default), 3 W */
+  int ncycle;         /* default 30 */
+  int v1, v2;         /* default 1, 1 */
+  int isemicoarsen;   /* default 1 */
+  double tol;         /* relative residual ||f - A u|| / ||f||, default 1e-6 */
+  int precision;      /* 0 complex FP64; 1 complex FP32 cycles in FP64 refinement */
+  int coarse_sweeps;  /* line sweeps per coarsest-level smoothing, default 2 */
+  int verb;
+} emmi_params_t;
+
+typedef struct {
+  int cycles;         /* completed cycles */
+  int status;         /* EMMI_CONVERGED, EMMI_NOT_CONVERGED, EMMI_NONFINITE */
+  double rnorm0;      /* ||f|| */
+  double rnorm;       /* last ||f - A u|| */
+} emmi_stats_t;
+
+enum { EMMI_CONVERGED = 0, EMMI_NOT_CONVERGED = 1, EMMI_NONFINITE = 2, EMMI_ERROR = -1 };
+
+typedef struct emmi_solver emmi_solver_t;
+
+void emmi_default_params(emmi_params_t *p);
+const char *emmi_last_error(void);
+
+/* A solver owns the level hierarchy of one (grid, medium, frequency).
+ * x1..x3: float node coordinates (n+1 each, host); sig11..sig33: float
+ * conductivities [n3][n2][n1], host pointers or device pointers when
+ * sig_on_device != 0.  Returns NULL on error (see emmi_last_error). */
+emmi_solver_t *emmi_solver_create(int n1, int n2, int n3, const float *x1, const float *x2, const float *x3,
+                                  const float *sig11, const float *sig22, const float *sig33, int sig_on_device,
+                                  float omega, const emmi_params_t *p);
+/* Device vectors (complex double, 3(n1+1)(n2+1)(n3+1)): b source term
+ * i omega mu0 J / vol as inject_source writes it, x <- E, h <- H (h may be b),
+ * x0 initial guess or NULL.  Returns the status code. */
+int emmi_solver_solve(emmi_solver_t *s, const void *b, void *x, void *h, const void *x0, emmi_stats_t *st);
+/* residual norms of the last solve, one per cycle; returns the count */
+int emmi_solver_history(emmi_solver_t *s, double *out, int maxn);
+void emmi_solver_destroy(emmi_solver_t *s);
+int emmi_solver_levels(emmi_solver_t *s, int *npatterns);
+void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
+
+/* Single-kernel checks on host vectors (complex double), used by the parity
+ * tests: op 0 residual r = f - A u (in1 = u, in2 = f); op 1 one line
+ * Gauss-Seidel sweep along `axis` (4 colours, reversed when iter is odd) of
+ * u = in1 with f = in2; op 2 restriction of in1 from level lev to lev+1;
+ * op 3 prolongation: out = in2 + P in1 (in1 on level lev+1, in2 on lev);
+ * op 4 H = curl in1 / (i omega mu0) (level 0); op 5 coarsest-level smoothing
+ * of u = in1 with f = in2.  `precision` selects the FP64 or FP32 kernels (the
+ * solver must have been created with that precision). */
+int emmi_solver_check_op(emmi_solver_t *s, int op, int pattern, int lev, int axis, int iter,
+                         const emmi_c128 *in1, const emmi_c128 *in2, emmi_c128 *out);
+
+/* device memory helpers for hosts without a GPU runtime of their own */
+void *emmi_device_alloc(size_t bytes);
+void emmi_device_free(void *p);
+int emmi_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int emmi_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int emmi_device_sync(void);
+/* kernel timing: accumulated milliseconds and launch counts per kernel
+ * class (0 residual, 1 line sweep, 2 restriction, 3 prolongation, 4 other)
+ * measured with HIP events on the solver stream when enabled */
+void emmi_timing_enable(int on);
+int emmi_timing_read(double *ms, long *count, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,72 @@
+"""Build the in-tree shared library libemmi_mgfd.so for gfx950 with hipcc.
+
+Device code is compiled for gfx950 only; the host-side bit-exact path is
+compiled without floating-point contraction.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libemmi_mgfd.so")
+
+HIP_SOURCES = ["mg_kernels.hip", "solver.hip", "batch.hip", "path_kernels.hip"]
+HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp", "batch_api.cpp"]
+
+ARCH = "gfx950"
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+          "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-lambda-capture"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(args):
+    src, obj, flags = args
+    cmd = [hipcc()] + flags + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        p = os.path.join(CSRC, s)
+        if os.path.exists(p):
+            jobs.append((p, os.path.join(BUILD, s + ".o"), COMMON + DEVICE + ["-x", "hip"]))
+    for s in HOST_SOURCES:
+        p = os.path.join(CSRC, s)
+        if os.path.exists(p):
+            jobs.append((p, os.path.join(BUILD, s + ".o"), COMMON + ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"]))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(_compile, jobs))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

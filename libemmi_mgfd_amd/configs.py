@@ -183,7 +183,7 @@ def config2(n=128) -> Survey:
                   [0.25, 0.75, 1.25], n, n, n, bathy=bathy)
 
 
-def tiny(n1=12, n2=10, n3=16, nrec=5) -> Survey:
+def tiny(n1=12, n2=10, n3=16, nrec=5, rho_air=1e8) -> Survey:
     """A small anisotropic survey for kernel-level known-answer tests; the
     multigrid sizes are deliberately unequal so a transposed index fails."""
     nx, ny, nz = 24, 20, 10
@@ -197,7 +197,7 @@ def tiny(n1=12, n2=10, n3=16, nrec=5) -> Survey:
     return Survey("tiny", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh * np.float32(1.1), rv,
                   [(100.0, -50.0, 450.0, 30.0, 5.0)], recs, [(1, i + 1) for i in range(len(recs))],
                   [0.5, 1.5], n1, n2, n3, d1min=200.0, d2min=200.0, d3min=60.0, nb=3,
-                  lextend=8000.0, bathy=bathy)
+                  lextend=8000.0, rho_air=rho_air, bathy=bathy)
 
 
 CONFIGS = {"config1": config1, "config2": config2, "tiny": tiny}

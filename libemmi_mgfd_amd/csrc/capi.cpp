@@ -1,0 +1,323 @@
+// capi.cpp -- C-ABI of the solver: the reference's gmg_init/gmg_apply/gmg_free
+// and the extended emmi_* API.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emmi_mgfd.h"
+#include "solver.h"
+#include "timing.h"
+
+using namespace emmi;
+
+static thread_local std::string g_err;
+
+static void set_err(const char* m) { g_err = m ? m : ""; }
+
+// getpar hooks: the reference's host program provides getparint and
+// getpardouble (src/cstd.c); without one the defaults apply
+extern "C" {
+__attribute__((weak)) int getparint(char* name, int* p);
+__attribute__((weak)) int getpardouble(char* name, double* p);
+}
+
+static int par_int(const char* name, int def) {
+  int v = def;
+  if (getparint && getparint(const_cast<char*>(name), &v)) return v;
+  return def;
+}
+static double par_double(const char* name, double def) {
+  double v = def;
+  if (getpardouble && getpardouble(const_cast<char*>(name), &v)) return v;
+  return def;
+}
+
+static SolveParams to_params(const emmi_params_t* p) {
+  SolveParams q;
+  if (p) {
+    q.cycleopt = p->cycleopt;
+    q.ncycle = p->ncycle;
+    q.v1 = p->v1;
+    q.v2 = p->v2;
+    q.isemicoarsen = p->isemicoarsen;
+    q.tol = p->tol;
+    q.precision = p->precision;
+    q.coarse_sweeps = p->coarse_sweeps;
+    q.verb = p->verb;
+  }
+  return q;
+}
+
+static void validate(const SolveParams& q) {
+  if (q.cycleopt < 1 || q.cycleopt > 3) throw std::runtime_error("cycleopt must be 1, 2 or 3");
+  if (q.ncycle < 0 || q.v1 < 0 || q.v2 < 0) throw std::runtime_error("negative cycle or smoothing count");
+  if (q.precision != 0 && q.precision != 1) throw std::runtime_error("precision must be 0 or 1");
+  if (q.coarse_sweeps < 1) throw std::runtime_error("coarse_sweeps must be >= 1");
+}
+
+struct emmi_solver {
+  Solver* s = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<double> hist;
+  int precision = 0;
+};
+
+extern "C" {
+
+void emmi_default_params(emmi_params_t* p) {
+  SolveParams q;
+  p->cycleopt = q.cycleopt;
+  p->ncycle = q.ncycle;
+  p->v1 = q.v1;
+  p->v2 = q.v2;
+  p->isemicoarsen = q.isemicoarsen;
+  p->tol = q.tol;
+  p->precision = q.precision;
+  p->coarse_sweeps = q.coarse_sweeps;
+  p->verb = q.verb;
+}
+
+const char* emmi_last_error(void) { return g_err.c_str(); }
+
+emmi_solver_t* emmi_solver_create(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3,
+                                  const float* sig11, const float* sig22, const float* sig33, int sig_on_device,
+                                  float omega, const emmi_params_t* p) {
+  emmi_solver* h = new emmi_solver;
+  try {
+    SolveParams q = to_params(p);
+    validate(q);
+    if (n1 < 2 || n2 < 2 || n3 < 2) throw std::runtime_error("n1, n2, n3 must be >= 2");
+    if (!x1 || !x2 || !x3 || !sig11 || !sig22 || !sig33) throw std::runtime_error("null grid or medium");
+    check_hip(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    h->precision = q.precision;
+    h->s = new Solver(n1, n2, n3, x1, x2, x3, sig11, sig22, sig33, sig_on_device, omega, q, h->stream);
+    return h;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return nullptr;
+  }
+}
+
+int emmi_solver_solve(emmi_solver_t* h, const void* b, void* x, void* hout, const void* x0, emmi_stats_t* st) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    SolveStats ss;
+    h->s->solve(b, x, hout, x0, &ss);
+    h->hist = ss.history;
+    if (st) {
+      st->cycles = ss.cycles;
+      st->status = ss.status;
+      st->rnorm0 = ss.rnorm0;
+      st->rnorm = ss.rnorm;
+    }
+    return ss.status;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
+int emmi_solver_history(emmi_solver_t* h, double* out, int maxn) {
+  if (!h) return 0;
+  const int n = (int)h->hist.size();
+  for (int i = 0; i < n && i < maxn; i++) out[i] = h->hist[i];
+  return n;
+}
+
+void emmi_solver_destroy(emmi_solver_t* h) {
+  if (!h) return;
+  delete h->s;
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int emmi_solver_levels(emmi_solver_t* h, int* npat) {
+  if (!h) return 0;
+  if (npat) *npat = h->s->npatterns();
+  return h->s->lmax();
+}
+
+void emmi_solver_dims(emmi_solver_t* h, int pattern, int lev, int* n123) {
+  const int* d = h->s->dims(pattern, lev);
+  n123[0] = d[0]; n123[1] = d[1]; n123[2] = d[2];
+}
+
+void* emmi_device_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) { set_err("hipMalloc failed"); return nullptr; }
+  return p;
+}
+void emmi_device_free(void* p) { if (p) hipFree(p); }
+int emmi_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+int emmi_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+int emmi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+
+void emmi_timing_enable(int on) { timing::enable(on != 0); }
+int emmi_timing_read(double* ms, long* count, int n) { return timing::read(ms, count, n); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// single-kernel checks
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct DevVec {
+  void* p = nullptr;
+  long n = 0;
+  explicit DevVec(long n_) : n(n_) { check_hip(hipMalloc(&p, sizeof(typename cvec<T>::type) * (n ? n : 1)), "hipMalloc"); }
+  ~DevVec() { if (p) hipFree(p); }
+  void up(const emmi_c128* h) {
+    std::vector<typename cvec<T>::type> t(n);
+    for (long i = 0; i < n; i++) { t[i].x = (T)h[i].re; t[i].y = (T)h[i].im; }
+    check_hip(hipMemcpy(p, t.data(), sizeof(t[0]) * n, hipMemcpyHostToDevice), "h2d");
+  }
+  void down(emmi_c128* h) {
+    std::vector<typename cvec<T>::type> t(n);
+    check_hip(hipMemcpy(t.data(), p, sizeof(t[0]) * n, hipMemcpyDeviceToHost), "d2h");
+    for (long i = 0; i < n; i++) { h[i].re = t[i].x; h[i].im = t[i].y; }
+  }
+};
+
+template <typename T>
+int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, int iter, const emmi_c128* in1,
+             const emmi_c128* in2, emmi_c128* out) {
+  hipStream_t s = S.stream();
+  const DevLevel<T>& L = H.pat[pat][lev];
+  const Level<T> V = L.view(S.omu());
+  const long nv = 3 * L.nc;
+  double* scratch = S.scratch();
+  if (op == 0 || op == 1 || op == 5) {
+    DevVec<T> u(nv), f(nv), r(nv);
+    u.up(in1);
+    f.up(in2);
+    if (op == 0) {
+      launch_residual<T>(V, u.p, f.p, r.p, nullptr, s);
+      check_hip(hipStreamSynchronize(s), "sync");
+      r.down(out);
+    } else {
+      const int naxis = op == 5 ? 1 : 1;
+      (void)naxis;
+      for (int k = 0; k < 4; k++) {
+        const int colour = (iter % 2 == 1) ? 3 - k : k;
+        launch_line_gs<T>(V, axis, colour, u.p, f.p, scratch, s);
+      }
+      check_hip(hipStreamSynchronize(s), "sync");
+      u.down(out);
+    }
+    return 0;
+  }
+  if (op == 2 || op == 3) {
+    if (lev + 1 >= (int)H.pat[pat].size()) throw std::runtime_error("no coarser level");
+    const DevLevel<T>& C = H.pat[pat][lev + 1];
+    const Level<T> VC = C.view(S.omu());
+    DevVec<T> fine(nv), coarse(3 * C.nc);
+    if (op == 2) {
+      fine.up(in1);
+      launch_restrict<T>(V, VC, C.transfer(), fine.p, coarse.p, s);
+      check_hip(hipStreamSynchronize(s), "sync");
+      coarse.down(out);
+    } else {
+      coarse.up(in1);
+      fine.up(in2);
+      launch_prolong<T>(V, VC, C.transfer(), coarse.p, fine.p, s);
+      check_hip(hipStreamSynchronize(s), "sync");
+      fine.down(out);
+    }
+    return 0;
+  }
+  if (op == 4) {
+    DevVec<T> u(nv), h(nv);
+    u.up(in1);
+    launch_curl_h<T>(V, u.p, h.p, s);
+    check_hip(hipStreamSynchronize(s), "sync");
+    h.down(out);
+    return 0;
+  }
+  throw std::runtime_error("unknown check op");
+}
+
+}  // namespace
+
+extern "C" int emmi_solver_check_op(emmi_solver_t* h, int op, int pattern, int lev, int axis, int iter,
+                                    const emmi_c128* in1, const emmi_c128* in2, emmi_c128* out) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    if (pattern < 0 || pattern >= h->s->npatterns() || lev < 0 || lev >= h->s->lmax()) throw std::runtime_error("bad level");
+    if (axis < 0 || axis > 2) throw std::runtime_error("bad axis");
+    if (h->precision == 0) return check_op<double>(*h->s, h->s->h64(), op, pattern, lev, axis, iter, in1, in2, out);
+    return check_op<float>(*h->s, h->s->h32(), op, pattern, lev, axis, iter, in1, in2, out);
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reference entry points gmg_init / gmg_apply / gmg_free (src/gmg.c:1704-1932)
+// ---------------------------------------------------------------------------
+namespace {
+struct LegacyState {
+  emf_t* emf = nullptr;
+  float omega = 0;
+  emmi_params_t p;
+  bool init = false;
+};
+LegacyState g_legacy;
+
+[[noreturn]] void fatal(const char* m) {
+  fprintf(stderr, "emmi_mgfd: %s\n", m);
+  exit(EXIT_FAILURE);
+}
+}  // namespace
+
+extern "C" void gmg_init(emf_t* emf, int ifreq) {
+  g_legacy.emf = emf;
+  g_legacy.omega = emf->omegas[ifreq];
+  emmi_default_params(&g_legacy.p);
+  g_legacy.p.cycleopt = par_int("cycleopt", 2);
+  g_legacy.p.ncycle = par_int("ncycle", 30);
+  g_legacy.p.v1 = par_int("v1", 1);
+  g_legacy.p.v2 = par_int("v2", 1);
+  g_legacy.p.tol = par_double("tol", 1e-6);
+  g_legacy.p.isemicoarsen = par_int("isemicoarsen", 1);
+  g_legacy.p.precision = par_int("precision", 0);
+  g_legacy.p.coarse_sweeps = par_int("coarse_sweeps", 2);
+  g_legacy.p.verb = emf->verb;
+  g_legacy.init = true;
+}
+
+extern "C" void gmg_apply(int n, emmi_c128* b, emmi_c128* x) {
+  if (!g_legacy.init) fatal("gmg_apply called before gmg_init");
+  emf_t* e = g_legacy.emf;
+  const long nv = 3L * (e->n1 + 1) * (e->n2 + 1) * (e->n3 + 1);
+  if (n != nv) fatal("gmg_apply: n != 3(n1+1)(n2+1)(n3+1)");
+  emmi_solver_t* s = emmi_solver_create(e->n1, e->n2, e->n3, e->x1, e->x2, e->x3, &e->sigma11[0][0][0],
+                                        &e->sigma22[0][0][0], &e->sigma33[0][0][0], 0, g_legacy.omega, &g_legacy.p);
+  if (!s) fatal(emmi_last_error());
+  void *db = emmi_device_alloc(sizeof(emmi_c128) * nv), *dx = emmi_device_alloc(sizeof(emmi_c128) * nv);
+  if (!db || !dx) fatal("device allocation failed");
+  emmi_memcpy_h2d(db, b, sizeof(emmi_c128) * nv);
+  emmi_stats_t st;
+  const int rc = emmi_solver_solve(s, db, dx, db, nullptr, &st);
+  if (rc == EMMI_ERROR) fatal(emmi_last_error());
+  emmi_memcpy_d2h(x, dx, sizeof(emmi_c128) * nv);
+  emmi_memcpy_d2h(b, db, sizeof(emmi_c128) * nv);
+  emmi_device_free(db);
+  emmi_device_free(dx);
+  emmi_solver_destroy(s);
+}
+
+extern "C" void gmg_free(void) { g_legacy.init = false; }

@@ -1,0 +1,44 @@
+// mg_kernels.h -- launchers of the gfx950 multigrid kernels (mg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include "emmi_dev.h"
+
+namespace emmi {
+
+template <typename T>
+struct Transfer {
+  int sc[3];
+  const T* wr[3];
+};
+
+template <typename T>
+void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s);
+int residual_blocks(long nc);
+void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
+template <typename T>
+void launch_line_gs(const Level<T>& L, int axis, int colour, void* u, const void* f, double* scratch, hipStream_t s);
+template <typename T>
+size_t line_scratch_reals(const int* n);
+template <typename T>
+void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s);
+template <typename T>
+void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s);
+template <typename T>
+void launch_zero(void* v, long n, hipStream_t s);
+template <typename T>
+void launch_scale_rhs(const Level<T>& L, const T* D0, const T* D1, const T* D2, const void* b, void* f, hipStream_t s);
+template <typename T>
+void launch_curl_h(const Level<T>& L, const void* u, void* h, hipStream_t s);
+void launch_demote(const void* hi, void* lo, long n, hipStream_t s);
+void launch_promote_add(void* hi, const void* lo, long n, hipStream_t s);
+void launch_sigma_vol0(int n1, int n2, int n3, const double* W0, const double* W1, const double* W2, const float* sig,
+                       double* out, hipStream_t s);
+void launch_coarsen_cells(int fn1, int fn2, int n1, int n2, int n3, const int* sc, const double* fine, double* out,
+                          hipStream_t s);
+template <typename T>
+void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* s22, const double* s33, T* sbar,
+                       hipStream_t s);
+
+}  // namespace emmi

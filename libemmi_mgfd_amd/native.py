@@ -1,0 +1,219 @@
+"""ctypes binding of libemmi_mgfd.so (the HIP library of this package).
+
+The product path is the native library; nothing here computes on the CPU.
+Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemmi_mgfd.so")
+_LIB = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+c128p = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("cycleopt", C.c_int), ("ncycle", C.c_int), ("v1", C.c_int), ("v2", C.c_int),
+                ("isemicoarsen", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
+                ("coarse_sweeps", C.c_int), ("verb", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("status", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double)]
+
+
+STATUS = {0: "converged", 1: "not converged", 2: "non-finite residual", -1: "error"}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m libemmi_mgfd_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.emmi_default_params.argtypes = [C.POINTER(Params)]
+        L.emmi_last_error.restype = C.c_char_p
+        L.emmi_solver_create.restype = C.c_void_p
+        L.emmi_solver_create.argtypes = [C.c_int] * 3 + [C.c_void_p] * 6 + [C.c_int, C.c_float, C.POINTER(Params)]
+        L.emmi_solver_solve.restype = C.c_int
+        L.emmi_solver_solve.argtypes = [C.c_void_p] * 5 + [C.POINTER(Stats)]
+        L.emmi_solver_history.restype = C.c_int
+        L.emmi_solver_history.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.emmi_solver_destroy.argtypes = [C.c_void_p]
+        L.emmi_solver_levels.restype = C.c_int
+        L.emmi_solver_levels.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.emmi_solver_dims.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.emmi_solver_check_op.restype = C.c_int
+        L.emmi_solver_check_op.argtypes = [C.c_void_p] + [C.c_int] * 5 + [C.c_void_p] * 3
+        L.emmi_device_alloc.restype = C.c_void_p
+        L.emmi_device_alloc.argtypes = [C.c_size_t]
+        L.emmi_device_free.argtypes = [C.c_void_p]
+        L.emmi_memcpy_h2d.restype = C.c_int
+        L.emmi_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.emmi_memcpy_d2h.restype = C.c_int
+        L.emmi_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.emmi_device_sync.restype = C.c_int
+        L.emmi_timing_enable.argtypes = [C.c_int]
+        L.emmi_timing_read.restype = C.c_int
+        L.emmi_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return lib().emmi_last_error().decode()
+
+
+def params(**kw) -> Params:
+    p = Params()
+    lib().emmi_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class DeviceBuffer:
+    """A block of device memory owned by this object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().emmi_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise NativeError(last_error())
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        if lib().emmi_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes):
+            raise NativeError("host to device copy failed")
+        return self
+
+    def download(self, dtype, shape=None) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        if lib().emmi_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes):
+            raise NativeError("device to host copy failed")
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().emmi_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Solver:
+    """A multigrid solver for one (MG grid, medium, frequency) on the GPU."""
+
+    def __init__(self, n, x1, x2, x3, sig11, sig22, sig33, omega, sig_on_device=False, **kw):
+        self.n = tuple(int(v) for v in n)
+        self.p = params(**kw)
+        self._keep = [np.ascontiguousarray(a, np.float32) for a in (x1, x2, x3)]
+        if sig_on_device:
+            ptrs = [int(s) for s in (sig11, sig22, sig33)]
+        else:
+            sig = [np.ascontiguousarray(a, np.float32) for a in (sig11, sig22, sig33)]
+            self._keep += sig
+            ptrs = [a.ctypes.data for a in sig]
+        self.h = lib().emmi_solver_create(self.n[0], self.n[1], self.n[2],
+                                          self._keep[0].ctypes.data, self._keep[1].ctypes.data,
+                                          self._keep[2].ctypes.data, ptrs[0], ptrs[1], ptrs[2],
+                                          1 if sig_on_device else 0, np.float32(omega), C.byref(self.p))
+        if not self.h:
+            raise NativeError(last_error())
+
+    @property
+    def nvec(self) -> int:
+        n1, n2, n3 = self.n
+        return 3 * (n1 + 1) * (n2 + 1) * (n3 + 1)
+
+    def solve_device(self, b_ptr, x_ptr, h_ptr, x0_ptr=None) -> Stats:
+        st = Stats()
+        rc = lib().emmi_solver_solve(self.h, b_ptr, x_ptr, h_ptr, x0_ptr, C.byref(st))
+        if rc < 0:
+            raise NativeError(last_error())
+        return st
+
+    def solve(self, b: np.ndarray, x0: np.ndarray | None = None):
+        """Host convenience: returns (E, H, stats)."""
+        nb = 16 * self.nvec
+        db = DeviceBuffer(nb).upload(np.ascontiguousarray(b, np.complex128).reshape(-1))
+        dx = DeviceBuffer(nb)
+        d0 = DeviceBuffer(nb).upload(np.ascontiguousarray(x0, np.complex128).reshape(-1)) if x0 is not None else None
+        st = self.solve_device(db.ptr, dx.ptr, db.ptr, d0.ptr if d0 else None)
+        return dx.download(np.complex128), db.download(np.complex128), st
+
+    def history(self) -> np.ndarray:
+        buf = (C.c_double * 4096)()
+        n = lib().emmi_solver_history(self.h, buf, 4096)
+        return np.array(buf[:min(n, 4096)])
+
+    def levels(self):
+        npat = C.c_int()
+        lmax = lib().emmi_solver_levels(self.h, C.byref(npat))
+        return lmax, npat.value
+
+    def dims(self, pattern, lev):
+        d = (C.c_int * 3)()
+        lib().emmi_solver_dims(self.h, pattern, lev, d)
+        return tuple(d)
+
+    def check_op(self, op, pattern, lev, axis, it, in1, in2=None, out_len=None):
+        """Run one kernel on host vectors (see emmi_solver_check_op)."""
+        def nv(d):
+            return 3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1)
+        lmax, npat = self.levels()
+        if not (0 <= pattern < npat and 0 <= lev < lmax):
+            raise ValueError("no such level")
+        nf = nv(self.dims(pattern, lev))
+        nc = nv(self.dims(pattern, lev + 1)) if op in (2, 3) and lev + 1 < lmax else None
+        in1 = np.ascontiguousarray(in1, np.complex128).reshape(-1)
+        in2 = None if in2 is None else np.ascontiguousarray(in2, np.complex128).reshape(-1)
+        want = {0: (nf, nf, nf), 1: (nf, nf, nf), 2: (nf, None, nc), 3: (nc, nf, nf), 4: (nf, None, nf), 5: (nf, nf, nf)}[op]
+        if want[0] is None or in1.size != want[0] or (want[1] is not None and (in2 is None or in2.size != want[1])):
+            raise ValueError("vector sizes do not match the level")
+        out = np.zeros(want[2], np.complex128)
+        rc = lib().emmi_solver_check_op(self.h, op, pattern, lev, axis, it, in1.ctypes.data,
+                                        None if in2 is None else in2.ctypes.data, out.ctypes.data)
+        if rc != 0:
+            raise NativeError(last_error())
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().emmi_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def timing_enable(on=True):
+    lib().emmi_timing_enable(1 if on else 0)
+
+
+def timing_read():
+    ms = (C.c_double * 8)()
+    cnt = (C.c_long * 8)()
+    n = lib().emmi_timing_read(ms, cnt, 8)
+    names = ["residual", "line_sweep", "restriction", "prolongation", "other"]
+    return {names[i]: (ms[i], cnt[i]) for i in range(min(n, len(names)))}
