@@ -1,0 +1,208 @@
+#!/usr/bin/env python
+"""Throughput of the multigrid Maxwell solve on one MI355X.
+
+Metric: converged solves per second at 128^3 to relative residual 1e-6,
+plain FP64 versus FP32 cycles in FP64 refinement.
+
+Workload (one step): the headline survey (201x201x101 input model, 50/50/20 m
+cells, anisotropic sediment, sinusoidal seafloor, one resistive block;
+multigrid 128^3; one x-dipole shot; 201 inline receivers; 0.25, 0.75 and
+1.25 Hz).  A step runs, for each frequency, the source injection into the
+device field vector, the solve to ||f - Au|| < 1e-6 ||f|| and the receiver
+extraction; the model, the regridded medium and the solver hierarchy are
+resident in HBM before the timed region.  Data are synthetic.
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one
+GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s at 128^3 to rel-res 1e-6: plain FP64 vs mixed+batched+learned x0"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--config", default="config2")
+    p.add_argument("--variants", default="plain,mixed")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-cycles", type=int, default=2, help="F-cycles of the bounded CPU sample")
+    p.add_argument("--verbose", action="store_true")
+    return p.parse_args()
+
+
+class ShotRun:
+    """One shot on the GPU: device-resident medium, solver and vectors."""
+
+    def __init__(self, survey, precision, shot=0):
+        from libemmi_mgfd_amd import carriers, native
+        self.native = native
+        self.sc = carriers.ShotCarriers(survey, shot)
+        L = native.lib()
+        L.regrid_init(self.sc.acq, self.sc.emf)
+        e = self.sc.emf
+        self.n = (e.n1, e.n2, e.n3)
+        ncell = e.n1 * e.n2 * e.n3
+        g = self.sc.grid()
+        self.dsig = []
+        for k in ("sigma11", "sigma22", "sigma33"):
+            d = native.DeviceBuffer(4 * ncell).upload(np.ascontiguousarray(g[k], np.float32))
+            self.dsig.append(d)
+        x1, x2, x3 = self.sc.x_arrays()
+        t0 = time.perf_counter()
+        self.solver = native.Solver(self.n, x1, x2, x3, *[d.ptr for d in self.dsig], self.sc.omegas[0],
+                                    sig_on_device=True, precision=precision, tol=1e-6, ncycle=60)
+        native.lib().emmi_device_sync()
+        self.setup_s = time.perf_counter() - t0
+        nb = 16 * self.sc.nvec
+        self.b = native.DeviceBuffer(nb)
+        self.x = native.DeviceBuffer(nb)
+
+    def solve_all(self):
+        """Inject, solve and extract every frequency; returns the stats."""
+        L = self.native.lib()
+        stats = []
+        for f in range(self.sc.emf.nfreq):
+            self.solver.set_omega(self.sc.omegas[f])
+            if L.emmi_inject_source_dev(self.sc.acq, self.sc.emf, f, self.b.ptr, None):
+                raise RuntimeError(L.emmi_path_last_error().decode())
+            st = self.solver.solve_device(self.b.ptr, self.x.ptr, self.b.ptr)
+            if L.emmi_extract_emf_data_dev(self.sc.acq, self.sc.emf, f, self.x.ptr, self.b.ptr, None):
+                raise RuntimeError(L.emmi_path_last_error().decode())
+            stats.append((st.status, st.cycles, st.rnorm / st.rnorm0))
+        return stats
+
+
+def time_variant(survey, precision, steps, warmup, timing=False):
+    from libemmi_mgfd_amd import native
+    run = ShotRun(survey, precision)
+    for _ in range(warmup):
+        run.solve_all()
+    native.lib().emmi_device_sync()
+    if timing:
+        native.timing_enable(True)
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        stats += run.solve_all()
+    native.lib().emmi_device_sync()
+    dt = time.perf_counter() - t0
+    kt = native.timing_read() if timing else None
+    if timing:
+        native.timing_enable(False)
+    nsolve = steps * len(survey.freqs)
+    return {"solves_per_s": nsolve / dt, "ms_per_step": 1e3 * dt / steps, "setup_s": run.setup_s,
+            "cycles": [c for _, c, _ in stats[:len(survey.freqs)]],
+            "status": sorted(set(s for s, _, _ in stats)), "rel_res": max(r for _, _, r in stats),
+            "kernels": kt, "dcal": run.sc.dcal.copy()}
+
+
+def cpu_baseline(survey, cycles):
+    """The reference (compiled from its own sources into oracle/_ref) on one
+    host core: a bounded sample of `cycles` F-cycles of the first frequency,
+    scaled to a converged solve with the reference's own cycle count."""
+    runner = os.path.join(ROOT, "oracle", "_ref", "ref_runner")
+    if not os.path.exists(runner):
+        return None
+    from libemmi_mgfd_amd import configs
+    s1 = configs.CONFIGS[survey.name]()
+    s1.freqs = s1.freqs[:1]
+    with tempfile.TemporaryDirectory() as tmp:
+        files = configs.write_reference_inputs(s1, os.path.join(tmp, "in"))
+        args = [runner, "task=model", f"outdir={tmp}/out", f"ncycle={cycles}", "verb=0"] + s1.getpar_args(files)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    m = re.search(r"solve_end ifreq=0 seconds=(\S+)", r.stdout)
+    if not m:
+        return None
+    t = float(m.group(1))
+    ref_cycles = None
+    gold = os.path.join(ROOT, "tests", "golden", f"{survey.name}.npz")
+    if os.path.exists(gold):
+        h = np.load(gold)["hist"][0]
+        ref_cycles = int(np.sum(np.isfinite(h))) - 1
+    per_cycle = t / cycles
+    nc = ref_cycles if ref_cycles else 15
+    return {"value": 1.0 / (per_cycle * nc), "unit": "solves/s", "cores": 1, "kind": "reference",
+            "sample": f"{cycles} F-cycles of the {s1.freqs[0]} Hz solve at {s1.n1}x{s1.n2}x{s1.n3} "
+                      f"({t:.1f} s, single-threaded) scaled to the reference's {nc} cycles to rel-res 1e-6"
+                      + ("" if ref_cycles else " (cycle count assumed)")}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    from libemmi_mgfd_amd import configs, native
+    native.lib()
+    survey = configs.CONFIGS[a.config]()
+    res = {}
+    for v in a.variants.split(","):
+        prec = {"plain": 0, "mixed": 1}[v]
+        res[v] = time_variant(survey, prec, a.steps, a.warmup, timing=True)
+        if a.verbose:
+            print(v, {k: res[v][k] for k in ("solves_per_s", "ms_per_step", "cycles", "status", "rel_res")},
+                  file=sys.stderr)
+    best = "mixed" if "mixed" in res else list(res)[0]
+    r = res[best]
+    # dominant kernel class of the reported variant
+    kt = r["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms, cnt, by = kt[dom]
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": r["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "plain" else "f64 arithmetic, c64-stored cycles",
+        "data": "synthetic",
+        "config": {"workload": f"{survey.name}: {survey.nx}x{survey.ny}x{survey.nz} input model, MG "
+                               f"{survey.n1}x{survey.n2}x{survey.n3}, 1 shot x {len(survey.freqs)} freqs "
+                               f"{survey.freqs} Hz, {len(survey.receivers)} receivers, rel-res 1e-6",
+                   "solves_per_step": len(survey.freqs), "variant": best},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_launch_us": 1e3 * ms / max(cnt, 1), "launches": cnt},
+        "variants": {k: {"solves_per_s": v["solves_per_s"], "ms_per_step": v["ms_per_step"],
+                         "cycles": v["cycles"], "status": v["status"], "max_rel_res": v["rel_res"],
+                         "setup_s": v["setup_s"],
+                         "kernels_ms": {c: round(t[0], 3) for c, t in v["kernels"].items()}}
+                     for k, v in res.items()},
+    }
+    if "plain" in res and "mixed" in res:
+        out["speedup_vs_plain"] = res["mixed"]["solves_per_s"] / res["plain"]["solves_per_s"]
+        d0, d1 = res["plain"]["dcal"], res["mixed"]["dcal"]
+        amp = np.abs(d0)
+        m = amp > 1e-15
+        out["mixed_vs_plain_receiver_rel_err"] = float(np.max(np.abs(d1 - d0)[m] / amp[m])) if m.any() else None
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(survey, a.cpu_cycles)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -168,6 +168,9 @@ emmi_solver_t *emmi_solver_create(int n1, int n2, int n3, const float *x1, const
  * i omega mu0 J / vol as inject_source writes it, x <- E, h <- H (h may be b),
  * x0 initial guess or NULL.  Returns the status code. */
 int emmi_solver_solve(emmi_solver_t *s, const void *b, void *x, void *h, const void *x0, emmi_stats_t *st);
+/* switch the frequency of a solver (the level hierarchy depends on the grid
+ * and medium only, so the frequencies of one shot share it) */
+int emmi_solver_set_omega(emmi_solver_t *s, float omega);
 /* residual norms of the last solve, one per cycle; returns the count */
 int emmi_solver_history(emmi_solver_t *s, double *out, int maxn);
 void emmi_solver_destroy(emmi_solver_t *s);
@@ -185,17 +188,30 @@ void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
 int emmi_solver_check_op(emmi_solver_t *s, int op, int pattern, int lev, int axis, int iter,
                          const emmi_c128 *in1, const emmi_c128 *in2, emmi_c128 *out);
 
+/* Device-resident steps around the solve (stream may be NULL).  The source
+ * term is computed with the arithmetic of inject_source and scattered into
+ * the device vector b_dev; receiver data are gathered from the device E/H
+ * vectors and written to emf->dcal_fd[.][ifreq][.] with the arithmetic of
+ * extract_emf_data.  Both are bit-identical to the host entry points.
+ * Return 0, or -1 with emmi_path_last_error() set. */
+int emmi_inject_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *b_dev, void *stream);
+int emmi_extract_emf_data_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_dev, const void *h_dev,
+                              void *stream);
+const char *emmi_path_last_error(void);
+
 /* device memory helpers for hosts without a GPU runtime of their own */
 void *emmi_device_alloc(size_t bytes);
 void emmi_device_free(void *p);
 int emmi_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int emmi_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int emmi_device_sync(void);
-/* kernel timing: accumulated milliseconds and launch counts per kernel
- * class (0 residual, 1 line sweep, 2 restriction, 3 prolongation, 4 other)
- * measured with HIP events on the solver stream when enabled */
+/* kernel timing: accumulated milliseconds, launch counts and algorithmic
+ * bytes (each input read once, each output written once per launch) per
+ * kernel class (0 residual, 1 line sweep, 2 restriction, 3 prolongation,
+ * 4 other), measured with HIP events on the stream of each launch while
+ * enabled; enabling clears the counters */
 void emmi_timing_enable(int on);
-int emmi_timing_read(double *ms, long *count, int n);
+int emmi_timing_read(double *ms, long *count, double *bytes, int n);
 
 #ifdef __cplusplus
 }

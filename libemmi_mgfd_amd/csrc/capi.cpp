@@ -124,6 +124,12 @@ int emmi_solver_solve(emmi_solver_t* h, const void* b, void* x, void* hout, cons
   }
 }
 
+int emmi_solver_set_omega(emmi_solver_t* h, float omega) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  h->s->set_omega(omega);
+  return 0;
+}
+
 int emmi_solver_history(emmi_solver_t* h, double* out, int maxn) {
   if (!h) return 0;
   const int n = (int)h->hist.size();
@@ -164,7 +170,7 @@ int emmi_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 int emmi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
 
 void emmi_timing_enable(int on) { timing::enable(on != 0); }
-int emmi_timing_read(double* ms, long* count, int n) { return timing::read(ms, count, n); }
+int emmi_timing_read(double* ms, long* count, double* bytes, int n) { return timing::read(ms, count, bytes, n); }
 
 }  // extern "C"
 

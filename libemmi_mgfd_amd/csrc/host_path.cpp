@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <vector>
 
 #include "emmi_mgfd.h"
 #include "host_path.h"
@@ -349,34 +351,47 @@ Tri locate(const emf_t* emf, int pat, float px, float py, float pz) {
 using emmi::Tri;
 
 namespace {
+// destinations and sources of the trilinear stencils: a dense host vector, or
+// a sparse list (device-resident vectors are reached through such lists)
+struct DenseSink {
+  emmi_c128* v;
+  void add(long i, C a) { addto(v, i, a); }
+};
+struct DenseSource {
+  const emmi_c128* v;
+  C get(long i) const { return ld(v, i); }
+};
+
 // v[corner] += val * f1 * f2 * f3 in the reference's order
-void scatter8(emmi_c128* v, long comp_off, const Tri& t, int n1, int n2, C val) {
+template <typename Sink>
+void scatter8(Sink& v, long comp_off, const Tri& t, int n1, int n2, C val) {
   const long s1 = n1 + 1, s2 = (long)(n1 + 1) * (n2 + 1);
   const long o = comp_off + t.i + s1 * t.j + s2 * t.k;
   const double w1 = t.w1, w2 = t.w2, w3 = t.w3;
-  addto(v, o, cmulr(cmulr(cmulr(val, 1. - w1), 1. - w2), 1. - w3));
-  addto(v, o + 1, cmulr(cmulr(cmulr(val, w1), 1. - w2), 1. - w3));
-  addto(v, o + s1, cmulr(cmulr(cmulr(val, 1. - w1), w2), 1. - w3));
-  addto(v, o + 1 + s1, cmulr(cmulr(cmulr(val, w1), w2), 1. - w3));
-  addto(v, o + s2, cmulr(cmulr(cmulr(val, 1. - w1), 1. - w2), w3));
-  addto(v, o + 1 + s2, cmulr(cmulr(cmulr(val, w1), 1. - w2), w3));
-  addto(v, o + s1 + s2, cmulr(cmulr(cmulr(val, 1. - w1), w2), w3));
-  addto(v, o + 1 + s1 + s2, cmulr(cmulr(cmulr(val, w1), w2), w3));
+  v.add(o, cmulr(cmulr(cmulr(val, 1. - w1), 1. - w2), 1. - w3));
+  v.add(o + 1, cmulr(cmulr(cmulr(val, w1), 1. - w2), 1. - w3));
+  v.add(o + s1, cmulr(cmulr(cmulr(val, 1. - w1), w2), 1. - w3));
+  v.add(o + 1 + s1, cmulr(cmulr(cmulr(val, w1), w2), 1. - w3));
+  v.add(o + s2, cmulr(cmulr(cmulr(val, 1. - w1), 1. - w2), w3));
+  v.add(o + 1 + s2, cmulr(cmulr(cmulr(val, w1), 1. - w2), w3));
+  v.add(o + s1 + s2, cmulr(cmulr(cmulr(val, 1. - w1), w2), w3));
+  v.add(o + 1 + s1 + s2, cmulr(cmulr(cmulr(val, w1), w2), w3));
 }
 
-C gather8(const emmi_c128* v, long comp_off, int i, int j, int k, float w1f, float w2f, float w3f, int n1, int n2) {
+template <typename Source>
+C gather8(const Source& v, long comp_off, int i, int j, int k, float w1f, float w2f, float w3f, int n1, int n2) {
   const long s1 = n1 + 1, s2 = (long)(n1 + 1) * (n2 + 1);
   const long o = comp_off + i + s1 * j + s2 * k;
   const double w1 = w1f, w2 = w2f, w3 = w3f;
   C s = {0, 0};
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o), 1. - w1), 1. - w2), 1. - w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + 1), w1), 1. - w2), 1. - w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + s1), 1. - w1), w2), 1. - w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + 1 + s1), w1), w2), 1. - w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + s2), 1. - w1), 1. - w2), w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + 1 + s2), w1), 1. - w2), w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + s1 + s2), 1. - w1), w2), w3));
-  s = cadd(s, cmulr(cmulr(cmulr(ld(v, o + 1 + s1 + s2), w1), w2), w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o), 1. - w1), 1. - w2), 1. - w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + 1), w1), 1. - w2), 1. - w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + s1), 1. - w1), w2), 1. - w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + 1 + s1), w1), w2), 1. - w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + s2), 1. - w1), 1. - w2), w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + 1 + s2), w1), 1. - w2), w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + s1 + s2), 1. - w1), w2), w3));
+  s = cadd(s, cmulr(cmulr(cmulr(v.get(o + 1 + s1 + s2), w1), w2), w3));
   return s;
 }
 
@@ -406,10 +421,11 @@ long n123_of(const emf_t* e) { return (long)(e->n1 + 1) * (e->n2 + 1) * (e->n3 +
 
 }  // namespace
 
-// src/inject_extract.c:26-120
-extern "C" void inject_source(acq_t* acq, emf_t* emf, emmi_c128* b, int ifreq) {
+namespace emmi {
+// source term of src/inject_extract.c:26-120 into any sink
+template <typename Sink>
+void source_terms(acq_t* acq, emf_t* emf, int ifreq, Sink& sink) {
   const long n123 = n123_of(emf);
-  memset(b, 0, sizeof(emmi_c128) * 3 * n123);
   C* amp = (C*)xmalloc(sizeof(C) * (emf->nchsrc + 1));
   for (int i = 0; i < emf->nchsrc; i++) amp[i] = C{1, 0};
   const C iwm = i_omega_mu0(emf->omegas[ifreq]);
@@ -419,10 +435,38 @@ extern "C" void inject_source(acq_t* acq, emf_t* emf, emmi_c128* b, int ifreq) {
     for (int comp = 0; comp < 3; comp++) {
       const Tri t = emmi::locate(emf, comp, acq->src_x1[is], acq->src_x2[is], acq->src_x3[is]);
       const C sv = cdivr(cmul(iwm, F[comp]), (double)t.vol);
-      scatter8(b, comp * n123, t, emf->n1, emf->n2, sv);
+      scatter8(sink, comp * n123, t, emf->n1, emf->n2, sv);
     }
   }
   free(amp);
+}
+
+struct SparseSink {
+  std::map<long, C> m;
+  void add(long i, C a) {
+    auto it = m.find(i);
+    if (it == m.end()) m.emplace(i, C{0.0 + a.re, 0.0 + a.im});
+    else it->second = cadd(it->second, a);
+  }
+};
+
+void source_entries(acq_t* acq, emf_t* emf, int ifreq, std::vector<long>& idx, std::vector<emmi_c128>& val) {
+  SparseSink sk;
+  source_terms(acq, emf, ifreq, sk);
+  idx.clear();
+  val.clear();
+  for (auto& kv : sk.m) {
+    idx.push_back(kv.first);
+    val.push_back(emmi_c128{kv.second.re, kv.second.im});
+  }
+}
+}  // namespace emmi
+
+// src/inject_extract.c:26-120
+extern "C" void inject_source(acq_t* acq, emf_t* emf, emmi_c128* b, int ifreq) {
+  memset(b, 0, sizeof(emmi_c128) * 3 * n123_of(emf));
+  DenseSink sk{b};
+  emmi::source_terms(acq, emf, ifreq, sk);
 }
 
 // src/inject_extract.c:124-314
@@ -442,12 +486,14 @@ extern "C" void inject_adj_source(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c12
     moment(emf->nchrec, emf->chrec, 0, amp, acq->rec_azimuth[ir], acq->rec_dip[ir], F);
     for (int comp = 0; comp < 3; comp++) {
       const Tri t = emmi::locate(emf, comp, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
-      scatter8(b, comp * n123, t, n1, n2, cdivr(cmul(iwm, F[comp]), (double)t.vol));
+      DenseSink sb{b};
+      scatter8(sb, comp * n123, t, n1, n2, cdivr(cmul(iwm, F[comp]), (double)t.vol));
     }
     moment(emf->nchrec, emf->chrec, 3, amp, acq->rec_azimuth[ir], acq->rec_dip[ir], F);
     for (int comp = 0; comp < 3; comp++) {
       const Tri t = emmi::locate(emf, 3 + comp, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
-      scatter8(x, comp * n123, t, n1, n2, cdivr(F[comp], (double)t.vol));
+      DenseSink sx{x};
+      scatter8(sx, comp * n123, t, n1, n2, cdivr(F[comp], (double)t.vol));
     }
   }
   free(amp);
@@ -480,8 +526,11 @@ emmi_c64 rotate(int which, float phi, float theta, C Fx, C Fy, C Fz) {
 }
 }  // namespace
 
-// src/inject_extract.c:317-474
-extern "C" void extract_emf_data(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128* b, int ifreq) {
+namespace emmi {
+// receiver data of src/inject_extract.c:317-474 from any source of E (x) and
+// H (b) values
+template <typename SrcE, typename SrcH>
+void receiver_data(acq_t* acq, emf_t* emf, int ifreq, const SrcE& x, const SrcH& b) {
   const long n123 = n123_of(emf);
   const int n1 = emf->n1, n2 = emf->n2;
   const float* s33 = &emf->sigma33[0][0][0];
@@ -500,7 +549,7 @@ extern "C" void extract_emf_data(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128
       const double w1 = t.w1, w2 = t.w2, w3 = t.w3;
       const int i = t.i, j = t.j, k = t.k;
 #define S33(ii, jj, kk) ((double)s33[(ii) + (long)n1 * ((jj) + (long)n2 * (kk))])
-#define XZ(ii, jj, kk) ld(x, 2 * n123 + (ii) + s1 * (jj) + s2 * (kk))
+#define XZ(ii, jj, kk) x.get(2 * n123 + (ii) + s1 * (jj) + s2 * (kk))
       Fz = cadd(Fz, cmulr(cmulr(cmulr(rmulc(S33(i, j, k), XZ(i, j, k)), 1. - w1), 1. - w2), 1. - w3));
       Fz = cadd(Fz, cmulr(cmulr(cmulr(rmulc(S33(i + 1, j, k), XZ(i + 1, j, k)), w1), 1. - w2), 1. - w3));
       Fz = cadd(Fz, cmulr(cmulr(cmulr(rmulc(S33(i, j + 1, k), XZ(i, j + 1, k)), 1. - w1), w2), 1. - w3));
@@ -528,6 +577,50 @@ extern "C" void extract_emf_data(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128
       if (code >= 3) emf->dcal_fd[ic][ifreq][ir] = rotate(code - 3, phi, theta, Hx, Hy, Hz);
     }
   }
+}
+
+// records the entries a receiver extraction reads
+struct RecordingSource {
+  std::vector<long>* idx;
+  C get(long i) const { idx->push_back(i); return C{0, 0}; }
+};
+// serves previously gathered entries in the order they were recorded
+struct ReplaySource {
+  const std::vector<emmi_c128>* val;
+  mutable size_t pos = 0;
+  C get(long) const { const emmi_c128& v = (*val)[pos++]; return C{v.re, v.im}; }
+};
+
+void receiver_indices(acq_t* acq, emf_t* emf, std::vector<long>& ix, std::vector<long>& ib) {
+  ix.clear();
+  ib.clear();
+  emmi_c64*** keep = emf->dcal_fd;
+  // run against a scratch output table so the caller's data is untouched
+  const int nch = emf->nchrec, nf = emf->nfreq, nr = acq->nrec;
+  std::vector<emmi_c64> buf((size_t)nch * nf * (nr > 0 ? nr : 1));
+  std::vector<emmi_c64*> p1((size_t)nch * nf);
+  std::vector<emmi_c64**> p2(nch);
+  for (int c = 0; c < nch; c++) {
+    p2[c] = &p1[(size_t)c * nf];
+    for (int f = 0; f < nf; f++) p1[(size_t)c * nf + f] = &buf[((size_t)c * nf + f) * nr];
+  }
+  emf->dcal_fd = p2.data();
+  RecordingSource rx{&ix}, rb{&ib};
+  receiver_data(acq, emf, 0, rx, rb);
+  emf->dcal_fd = keep;
+}
+
+void receiver_data_from(acq_t* acq, emf_t* emf, int ifreq, const std::vector<emmi_c128>& vx,
+                        const std::vector<emmi_c128>& vb) {
+  ReplaySource sx{&vx}, sb{&vb};
+  receiver_data(acq, emf, ifreq, sx, sb);
+}
+}  // namespace emmi
+
+// src/inject_extract.c:317-474
+extern "C" void extract_emf_data(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128* b, int ifreq) {
+  DenseSource sx{x}, sb{b};
+  emmi::receiver_data(acq, emf, ifreq, sx, sb);
 }
 
 namespace emmi {
@@ -572,9 +665,10 @@ extern "C" void extract_electric_field(acq_t* acq, emf_t* emf, emmi_c128* x, int
     for (int j = 0; j < T.ny; j++)
       for (int i = 0; i < T.nx; i++) {
         const long o = i + (long)T.nx * (j + (long)T.ny * k);
-        C v = gather8(x, 0, T.i1s[i], T.i2[j], T.i3[k], T.w1s[i], T.w2[j], T.w3[k], n1, n2);
+        const DenseSource sx{x};
+        C v = gather8(sx, 0, T.i1s[i], T.i2[j], T.i3[k], T.w1s[i], T.w2[j], T.w3[k], n1, n2);
         E1[o] = emmi_c64{(float)v.re, (float)v.im};
-        v = gather8(x, n123, T.i1[i], T.i2s[j], T.i3[k], T.w1[i], T.w2s[j], T.w3[k], n1, n2);
+        v = gather8(sx, n123, T.i1[i], T.i2s[j], T.i3[k], T.w1[i], T.w2s[j], T.w3[k], n1, n2);
         E2[o] = emmi_c64{(float)v.re, (float)v.im};
       }
 #define S33(a, b, c) s33[(a) + (long)n1 * ((b) + (long)n2 * (c))]
@@ -624,11 +718,12 @@ extern "C" void extract_magnetic_field(acq_t* acq, emf_t* emf, emmi_c128* b, int
     for (int j = 0; j < T.ny; j++)
       for (int i = 0; i < T.nx; i++) {
         const long o = i + (long)T.nx * (j + (long)T.ny * k);
-        C v = gather8(b, 0, T.i1[i], T.i2s[j], T.i3s[k], T.w1[i], T.w2s[j], T.w3s[k], n1, n2);
+        const DenseSource sb{b};
+        C v = gather8(sb, 0, T.i1[i], T.i2s[j], T.i3s[k], T.w1[i], T.w2s[j], T.w3s[k], n1, n2);
         H1[o] = emmi_c64{(float)v.re, (float)v.im};
-        v = gather8(b, n123, T.i1s[i], T.i2[j], T.i3s[k], T.w1s[i], T.w2[j], T.w3s[k], n1, n2);
+        v = gather8(sb, n123, T.i1s[i], T.i2[j], T.i3s[k], T.w1s[i], T.w2[j], T.w3s[k], n1, n2);
         H2[o] = emmi_c64{(float)v.re, (float)v.im};
-        v = gather8(b, 2 * n123, T.i1s[i], T.i2s[j], T.i3[k], T.w1s[i], T.w2s[j], T.w3[k], n1, n2);
+        v = gather8(sb, 2 * n123, T.i1s[i], T.i2s[j], T.i3[k], T.w1s[i], T.w2s[j], T.w3[k], n1, n2);
         H3[o] = emmi_c64{(float)v.re, (float)v.im};
       }
 }

@@ -26,3 +26,14 @@ struct CentreTables {
 };
 
 }  // namespace emmi
+
+namespace emmi {
+// the nonzero entries of the source term of src/inject_extract.c:26-120,
+// accumulated in the reference's order (bit-identical to inject_source)
+void source_entries(acq_t* acq, emf_t* emf, int ifreq, std::vector<long>& idx, std::vector<emmi_c128>& val);
+// the E (ix) and H (ib) vector entries receiver extraction reads, in reading order
+void receiver_indices(acq_t* acq, emf_t* emf, std::vector<long>& ix, std::vector<long>& ib);
+// extract_emf_data from previously gathered entries (same order as receiver_indices)
+void receiver_data_from(acq_t* acq, emf_t* emf, int ifreq, const std::vector<emmi_c128>& vx,
+                        const std::vector<emmi_c128>& vb);
+}  // namespace emmi

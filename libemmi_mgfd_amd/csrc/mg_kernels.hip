@@ -567,7 +567,8 @@ static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / b
 template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s) {
   using V = typename cvec<T>::type;
-  timing::Scope ts(timing::RESIDUAL, s);
+  const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
+  timing::Scope ts(timing::RESIDUAL, s, 3.0 * L.nc * (3 * sz + sr));
   hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, (const V*)u, (const V*)f, (V*)r, partial);
 }
 
@@ -594,7 +595,9 @@ void launch_line_gs(const Level<T>& L, int a, int colour, void* u, const void* f
   const int nl = nlo * nhi;
   if (nl <= 0 || L.n[a] < 2) return;
   const dim3 g(nblk(nl, 64)), b(64);
-  timing::Scope ts(timing::LINE, s);
+  // rows of the launched lines: u, f and sbar read, u written
+  const double rows = (double)nl * (5.0 * L.n[a] - 4.0);
+  timing::Scope ts(timing::LINE, s, rows * (3.0 * sizeof(typename cvec<T>::type) + sizeof(T)));
   if (a == 0) hipLaunchKernelGGL((k_line_gs<T, 0>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
   else if (a == 1) hipLaunchKernelGGL((k_line_gs<T, 1>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
   else hipLaunchKernelGGL((k_line_gs<T, 2>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
@@ -615,14 +618,14 @@ size_t line_scratch_reals(const int* n) {
 template <typename T>
 void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s) {
   using V = typename cvec<T>::type;
-  timing::Scope ts(timing::RESTRICT, s);
+  timing::Scope ts(timing::RESTRICT, s, 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)rf, (V*)fc);
 }
 
 template <typename T>
 void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s) {
   using V = typename cvec<T>::type;
-  timing::Scope ts(timing::PROLONG, s);
+  timing::Scope ts(timing::PROLONG, s, 3.0 * (2.0 * F.nc + C.nc) * sizeof(typename cvec<T>::type));
   hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(F.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)uc, (V*)uf);
 }
 

@@ -77,6 +77,8 @@ class Solver {
   int lmax() const { return lmax_; }
   int npatterns() const { return npat_; }
   double omu() const { return omu_; }
+  // the hierarchy depends on the grid and medium only: switch frequency
+  void set_omega(float omega) { omu_ = (double)omega * (4. * 3.14159265358979323846264 * 1e-7); }
   const int* dims(int pat, int lev) const { return dims_[pat][lev].data(); }
 
   // kernel-level access for the element-wise checks (FP64 hierarchy)

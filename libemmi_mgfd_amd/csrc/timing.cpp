@@ -10,6 +10,7 @@ namespace {
 struct Pending {
   int cls;
   hipEvent_t a, b;
+  double bytes;
 };
 bool g_on = false;
 std::mutex g_mu;
@@ -18,6 +19,7 @@ std::vector<hipEvent_t> g_pool;
 hipEvent_t g_open[NCLASS];
 double g_ms[NCLASS];
 long g_count[NCLASS];
+double g_bytes[NCLASS];
 
 hipEvent_t get_event() {
   if (!g_pool.empty()) {
@@ -34,7 +36,7 @@ hipEvent_t get_event() {
 void enable(bool on) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_on = on;
-  for (int i = 0; i < NCLASS; i++) { g_ms[i] = 0; g_count[i] = 0; }
+  for (int i = 0; i < NCLASS; i++) { g_ms[i] = 0; g_count[i] = 0; g_bytes[i] = 0; }
 }
 
 bool enabled() { return g_on; }
@@ -46,15 +48,15 @@ void begin(int cls, hipStream_t s) {
   hipEventRecord(g_open[cls], s);
 }
 
-void end(int cls, hipStream_t s) {
+void end(int cls, hipStream_t s, double bytes) {
   if (!g_on) return;
   std::lock_guard<std::mutex> lk(g_mu);
   hipEvent_t e = get_event();
   hipEventRecord(e, s);
-  g_pending.push_back({cls, g_open[cls], e});
+  g_pending.push_back({cls, g_open[cls], e, bytes});
 }
 
-int read(double* ms, long* count, int n) {
+int read(double* ms, long* count, double* bytes, int n) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto& p : g_pending) {
     hipEventSynchronize(p.b);
@@ -62,6 +64,7 @@ int read(double* ms, long* count, int n) {
     hipEventElapsedTime(&t, p.a, p.b);
     g_ms[p.cls] += t;
     g_count[p.cls] += 1;
+    g_bytes[p.cls] += p.bytes;
     g_pool.push_back(p.a);
     g_pool.push_back(p.b);
   }
@@ -69,6 +72,7 @@ int read(double* ms, long* count, int n) {
   for (int i = 0; i < n && i < NCLASS; i++) {
     ms[i] = g_ms[i];
     count[i] = g_count[i];
+    if (bytes) bytes[i] = g_bytes[i];
   }
   return NCLASS;
 }

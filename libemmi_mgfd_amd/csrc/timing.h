@@ -13,15 +13,17 @@ void enable(bool on);
 bool enabled();
 // bracket one launch; no-ops when disabled
 void begin(int cls, hipStream_t s);
-void end(int cls, hipStream_t s);
-// synchronises the recorded events, returns accumulated ms and counts
-int read(double* ms, long* count, int n);
+void end(int cls, hipStream_t s, double bytes);
+// synchronises the recorded events, returns accumulated ms, launch counts and
+// algorithmic bytes per class
+int read(double* ms, long* count, double* bytes, int n);
 
 struct Scope {
   int cls;
   hipStream_t s;
-  Scope(int c, hipStream_t st) : cls(c), s(st) { begin(cls, s); }
-  ~Scope() { end(cls, s); }
+  double bytes;
+  Scope(int c, hipStream_t st, double b = 0) : cls(c), s(st), bytes(b) { begin(cls, s); }
+  ~Scope() { end(cls, s, bytes); }
 };
 
 }  // namespace timing

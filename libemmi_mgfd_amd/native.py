@@ -47,6 +47,29 @@ def lib() -> C.CDLL:
         L.emmi_solver_create.argtypes = [C.c_int] * 3 + [C.c_void_p] * 6 + [C.c_int, C.c_float, C.POINTER(Params)]
         L.emmi_solver_solve.restype = C.c_int
         L.emmi_solver_solve.argtypes = [C.c_void_p] * 5 + [C.POINTER(Stats)]
+        L.emmi_solver_set_omega.restype = C.c_int
+        L.emmi_solver_set_omega.argtypes = [C.c_void_p, C.c_float]
+        from . import carriers as K
+        PE, PA, PF = C.POINTER(K.Emf), C.POINTER(K.Acq), C.POINTER(K.Fwi)
+        L.regrid_init.argtypes = [PA, PE]
+        L.regrid_free.argtypes = [PE]
+        L.inject_source.argtypes = [PA, PE, C.c_void_p, C.c_int]
+        L.inject_adj_source.argtypes = [PA, PE, C.c_void_p, C.c_void_p, C.c_int]
+        L.extract_emf_data.argtypes = [PA, PE, C.c_void_p, C.c_void_p, C.c_int]
+        L.extract_electric_field.argtypes = [PA, PE, C.c_void_p, C.c_int]
+        L.extract_magnetic_field.argtypes = [PA, PE, C.c_void_p, C.c_int]
+        L.build_gradient.argtypes = [PA, PE, PF] + [K.CPPPP] * 6
+        L.gmg_init.argtypes = [PE, C.c_int]
+        L.emmi_inject_source_dev.restype = C.c_int
+        L.emmi_inject_source_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p]
+        L.emmi_extract_emf_data_dev.restype = C.c_int
+        L.emmi_extract_emf_data_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.emmi_path_last_error.restype = C.c_char_p
+        L.gmg_apply.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.find_index.restype = C.c_int
+        L.find_index.argtypes = [C.c_int, C.c_void_p, C.c_float]
+        L.create_nugrid.restype = C.c_float
+        L.create_nugrid.argtypes = [C.c_int, C.c_float, C.c_float, C.c_void_p]
         L.emmi_solver_history.restype = C.c_int
         L.emmi_solver_history.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
         L.emmi_solver_destroy.argtypes = [C.c_void_p]
@@ -65,7 +88,7 @@ def lib() -> C.CDLL:
         L.emmi_device_sync.restype = C.c_int
         L.emmi_timing_enable.argtypes = [C.c_int]
         L.emmi_timing_read.restype = C.c_int
-        L.emmi_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
+        L.emmi_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double), C.c_int]
         _LIB = L
     return _LIB
 
@@ -159,6 +182,9 @@ class Solver:
         st = self.solve_device(db.ptr, dx.ptr, db.ptr, d0.ptr if d0 else None)
         return dx.download(np.complex128), db.download(np.complex128), st
 
+    def set_omega(self, omega):
+        lib().emmi_solver_set_omega(self.h, np.float32(omega))
+
     def history(self) -> np.ndarray:
         buf = (C.c_double * 4096)()
         n = lib().emmi_solver_history(self.h, buf, 4096)
@@ -211,9 +237,13 @@ def timing_enable(on=True):
     lib().emmi_timing_enable(1 if on else 0)
 
 
+TIMING_CLASSES = ["residual", "line_sweep", "restriction", "prolongation", "other"]
+
+
 def timing_read():
+    """{class: (total ms, launches, algorithmic bytes)} since timing_enable."""
     ms = (C.c_double * 8)()
     cnt = (C.c_long * 8)()
-    n = lib().emmi_timing_read(ms, cnt, 8)
-    names = ["residual", "line_sweep", "restriction", "prolongation", "other"]
-    return {names[i]: (ms[i], cnt[i]) for i in range(min(n, len(names)))}
+    by = (C.c_double * 8)()
+    n = lib().emmi_timing_read(ms, cnt, by, 8)
+    return {TIMING_CLASSES[i]: (ms[i], cnt[i], by[i]) for i in range(min(n, len(TIMING_CLASSES)))}

@@ -22,6 +22,7 @@
 #include <math.h>
 #include <complex.h>
 #include <sys/stat.h>
+#include <time.h>
 #include "cstd.h"
 #include "emf.h"
 #include "acq.h"
@@ -269,9 +270,14 @@ static void task_model(acq_t *acq, emf_t *emf)
     inject_source(acq, emf, b, ifreq);
     if (dumpfields) { snprintf(name, sizeof name, "b_f%d", ifreq); dump(name, b, n*sizeof(double _Complex)); }
     memset(x, 0, n*sizeof(double _Complex));
-    printf("solve_begin ifreq=%d\n", ifreq);
-    gmg_apply(n, b, x);
-    printf("solve_end ifreq=%d\n", ifreq);
+    {
+      struct timespec t0, t1;
+      printf("solve_begin ifreq=%d\n", ifreq);
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+      gmg_apply(n, b, x);
+      clock_gettime(CLOCK_MONOTONIC, &t1);
+      printf("solve_end ifreq=%d seconds=%.6f\n", ifreq, (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec));
+    }
     if (dumpfields) {
       snprintf(name, sizeof name, "E_f%d", ifreq); dump(name, x, n*sizeof(double _Complex));
       snprintf(name, sizeof name, "H_f%d", ifreq); dump(name, b, n*sizeof(double _Complex));

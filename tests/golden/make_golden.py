@@ -13,6 +13,8 @@ synthetic surveys of libemmi_mgfd_amd.configs and keeps only small outputs:
   tiny_grad.npz    one mode-2 gradient (forward + adjoint) on the same survey
   config1.npz      grid arrays, medium hashes, receiver data and residual
                    histories of the 64^3 survey (default and tol=1e-10)
+  config2.npz      (with --config2) the 128^3 three-frequency survey: grid,
+                   medium hashes, receiver data and residual histories
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -167,15 +169,35 @@ def config1(tmp):
     np.savez_compressed(os.path.join(HERE, "config1.npz"), **out)
 
 
+def config2(tmp):
+    """Headline survey: receiver data and residual histories (default tol)."""
+    s = configs.config2()
+    d, hist, _ = run(s, "model", os.path.join(tmp, "c2"))
+    out = {"n": np.array([s.n1, s.n2, s.n3])}
+    g = grid_dict(d, s)
+    for k in GRID_KEYS:
+        out[k] = g[k]
+    for k in ("sigma11", "sigma22", "sigma33"):
+        out[k + "_sha256"] = np.array(sha(g[k]))
+    out["dcal"] = ld(d, "dcal", np.complex64)
+    out["hist"] = pad_hist(hist)
+    np.savez_compressed(os.path.join(HERE, "config2.npz"), **out)
+
+
 def main():
     if not os.path.exists(RUNNER):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     with tempfile.TemporaryDirectory() as tmp:
+        if "--only-config2" in sys.argv:
+            config2(tmp)
+            return
         tiny_units(tmp)
         model = tiny_model(tmp)
         tiny_grad(tmp, model)
         if "--skip-config1" not in sys.argv:
             config1(tmp)
+        if "--config2" in sys.argv:
+            config2(tmp)
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
