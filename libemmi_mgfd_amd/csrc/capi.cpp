@@ -204,7 +204,6 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
   const DevLevel<T>& L = H.pat[pat][lev];
   const Level<T> V = L.view(S.omu());
   const long nv = 3 * L.nc;
-  double* scratch = S.scratch();
   if (op == 0 || op == 1 || op == 5) {
     DevVec<T> u(nv), f(nv), r(nv);
     u.up(in1);
@@ -214,12 +213,7 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
       check_hip(hipStreamSynchronize(s), "sync");
       r.down(out);
     } else {
-      const int naxis = op == 5 ? 1 : 1;
-      (void)naxis;
-      for (int k = 0; k < 4; k++) {
-        const int colour = (iter % 2 == 1) ? 3 - k : k;
-        launch_line_gs<T>(V, axis, colour, u.p, f.p, scratch, s);
-      }
+      S.line_sweep_on<T>(pat, lev, axis, iter, u.p, f.p);
       check_hip(hipStreamSynchronize(s), "sync");
       u.down(out);
     }

@@ -1,0 +1,380 @@
+// line_kernels.hip -- exact line block Gauss-Seidel, factored once per level.
+//
+// Line (a; pb, pc) holds the vertex patches of the nodes (p, pb, pc),
+// p = 1..N-1, N = n_a (src/gmg.c:340-1298): the along-line edges E_a(e),
+// e = 0..N-1, and per node four perpendicular edges, slots
+//   0: E_b(p, pb-1, pc)  1: E_b(p, pb, pc)  2: E_c(p, pb, pc-1)  3: E_c(p, pb, pc)
+// with (a, b, c) cyclic.  Each line system is solved exactly for the
+// correction of the current residual.  An along-line edge couples only to
+// the perpendicular edges of its two end nodes, through the constant vector
+// v = (Hc, -Hc, Hb, -Hb):
+//     d_e dE_a(e) + v.da_e - v.da_{e+1} = rho_e,
+// so it is eliminated first; the node unknowns then form a complex
+// symmetric 4x4 block-tridiagonal system
+//     D_p = K_p - v v^T (1/d_{p-1} + 1/d_p),   B_p = diag(l_{p-1}) + v v^T / d_{p-1}
+//     W_1 = D_1,  W_p = D_p - B_p W_{p-1}^{-1} B_p
+// whose factors depend on the grid, the medium and the frequency only.
+// k_line_factor stores, per node, W_p^{-1} (symmetric, 10 entries) and
+// 1/d_p once; k_line_solve then runs, per sweep,
+//     y_p = g_p - B_p z_{p-1},  z_p = W_p^{-1} y_p          (forward)
+//     da_p = z_p - W_p^{-1} B_{p+1} da_{p+1},
+//     dE_a(p) = rho_p/d_p + v.(da_{p+1} - da_p) / d_p        (backward)
+// with g_p = sigma_p + v (rho_{p-1}/d_{p-1} - rho_p/d_p) from the residual.
+// Lines of one colour (pb%2, pc%2) are not coupled, so a launch sweeps one
+// colour; every kernel computes in FP64 whatever the storage precision.
+#include "line_kernels.h"
+#include "stencil.h"
+#include "timing.h"
+
+namespace emmi {
+
+constexpr int FREC = 22;  // doubles per node in the factor records
+constexpr int SREC = 10;  // doubles per node in the substitution scratch
+
+__device__ __forceinline__ int symidx(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a * 4 - (a * (a - 1)) / 2 + (b - a);
+}
+
+__device__ __forceinline__ void inv4(cxd (&M)[4][4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const cxd piv = cinv(M[k][k]);
+    M[k][k] = cxd(1.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) M[k][j] = M[k][j] * piv;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (i == k) continue;
+      const cxd fct = M[i][k];
+      M[i][k] = cxd(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) M[i][j] -= fct * M[k][j];
+    }
+  }
+}
+
+// constants of one line
+template <typename T, int A>
+struct LineGeo {
+  static constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  int N, pb, pc;
+  long sa, base, oa, ob0, ob1, oc0, oc1;
+  double Hb, Hc, Wb0, Wb1, Wc0, Wc1, sIb, sIc, dre_a;
+  double v[4];
+
+  __device__ __forceinline__ LineGeo(const Level<T>& L, int pb_, int pc_) {
+    pb = pb_;
+    pc = pc_;
+    N = L.n[A];
+    sa = L.es[A];
+    base = (long)pb * L.es[B] + (long)pc * L.es[C];
+    oa = A * L.nc + base;
+    ob0 = B * L.nc + base - L.es[B];
+    ob1 = B * L.nc + base;
+    oc0 = C * L.nc + base - L.es[C];
+    oc1 = C * L.nc + base;
+    Hb = L.H[B][pb];
+    Hc = L.H[C][pc];
+    Wb0 = L.W[B][pb - 1];
+    Wb1 = L.W[B][pb];
+    Wc0 = L.W[C][pc - 1];
+    Wc1 = L.W[C][pc];
+    sIb = (double)L.IW[B][pb - 1] + (double)L.IW[B][pb];
+    sIc = (double)L.IW[C][pc - 1] + (double)L.IW[C][pc];
+    dre_a = Hc * sIb + Hb * sIc;
+    v[0] = Hc; v[1] = -Hc; v[2] = Hb; v[3] = -Hb;
+  }
+  // diagonal of the along-line edge e
+  __device__ __forceinline__ cxd d_edge(const Level<T>& L, int e) const {
+    return cxd((double)L.W[A][e] * dre_a, -L.omu * (double)L.sbar[oa + e * sa]);
+  }
+  // along-line coupling of node p to node p+1, per slot
+  __device__ __forceinline__ void lcoef(const Level<T>& L, int p, double (&l)[4]) const {
+    const double iWa = L.IW[A][p];
+    l[0] = -Wb0 * Hc * iWa; l[1] = -Wb1 * Hc * iWa; l[2] = -Wc0 * Hb * iWa; l[3] = -Wc1 * Hb * iWa;
+  }
+};
+
+template <typename T, int A>
+__device__ __forceinline__ bool line_of(const LineJob<T>& J, int tid, int& pb, int& pc) {
+  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  if (tid >= J.nlo * J.nhi) return false;
+  const int sb = J.cb ? 1 : 2, sc = J.cc ? 1 : 2;
+  // the perpendicular axis with the smaller stride varies fastest
+  if (B < C) { pb = sb + 2 * (tid % J.nlo); pc = sc + 2 * (tid / J.nlo); }
+  else { pc = sc + 2 * (tid % J.nlo); pb = sb + 2 * (tid / J.nlo); }
+  return true;
+}
+
+template <typename T, int A>
+__global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb, pc;
+  if (!line_of<T, A>(J, tid, pb, pc)) return;
+  const LineGeo<T, A> G(L, pb, pc);
+  const long stride = (long)J.nlo * J.nhi;
+  double* rec = J.fac + tid;
+  cxd rd_prev = cinv(G.d_edge(L, 0));
+  rec[20 * stride] = rd_prev.re;
+  rec[21 * stride] = rd_prev.im;
+  cxd Wi[4][4];
+  for (int p = 1; p <= G.N - 1; p++) {
+    const cxd rd = cinv(G.d_edge(L, p));
+    const double Ha = L.H[A][p];
+    const double iWa0 = L.IW[A][p - 1], iWa1 = L.IW[A][p];
+    const double Gb = Ha * G.sIc + G.Hc * (iWa1 + iWa0);
+    const double Gc = G.Hb * (iWa1 + iWa0) + Ha * G.sIb;
+    const long off = p * G.sa;
+    cxd D[4][4];
+    D[0][0] = cxd(G.Wb0 * Gb, -L.omu * (double)L.sbar[G.ob0 + off]);
+    D[1][1] = cxd(G.Wb1 * Gb, -L.omu * (double)L.sbar[G.ob1 + off]);
+    D[2][2] = cxd(G.Wc0 * Gc, -L.omu * (double)L.sbar[G.oc0 + off]);
+    D[3][3] = cxd(G.Wc1 * Gc, -L.omu * (double)L.sbar[G.oc1 + off]);
+    D[0][1] = D[1][0] = cxd(0, 0);
+    D[2][3] = D[3][2] = cxd(0, 0);
+    D[0][2] = D[2][0] = cxd(-Ha, 0);
+    D[0][3] = D[3][0] = cxd(Ha, 0);
+    D[1][2] = D[2][1] = cxd(Ha, 0);
+    D[1][3] = D[3][1] = cxd(-Ha, 0);
+    {
+      const cxd s = rd_prev + rd;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) D[i][j] -= s * (G.v[i] * G.v[j]);
+    }
+    if (p >= 2) {
+      double l[4];
+      G.lcoef(L, p - 1, l);
+      // P = W_{p-1}^{-1} B_p ; D -= B_p P
+      cxd wv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) wv[i] = Wi[i][0] * G.v[0] + Wi[i][1] * G.v[1] + Wi[i][2] * G.v[2] + Wi[i][3] * G.v[3];
+      cxd Pm[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) Pm[i][j] = Wi[i][j] * l[j] + (wv[i] * rd_prev) * G.v[j];
+      cxd vP[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) vP[j] = Pm[0][j] * G.v[0] + Pm[1][j] * G.v[1] + Pm[2][j] * G.v[2] + Pm[3][j] * G.v[3];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) D[i][j] -= Pm[i][j] * l[i] + (vP[j] * rd_prev) * G.v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) Wi[i][j] = D[i][j];
+    inv4(Wi);
+    double* r = rec + (long)p * FREC * stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = i; j < 4; j++) {
+        const int sidx = symidx(i, j);
+        const cxd w = (i == j) ? Wi[i][j] : (Wi[i][j] + Wi[j][i]) * 0.5;
+        r[(2 * sidx) * stride] = w.re;
+        r[(2 * sidx + 1) * stride] = w.im;
+      }
+    r[20 * stride] = rd.re;
+    r[21 * stride] = rd.im;
+    rd_prev = rd;
+  }
+}
+
+// z = W^{-1} y with the packed symmetric inverse of node record r
+__device__ __forceinline__ void symv(const double* r, long stride, const cxd (&y)[4], cxd (&z)[4]) {
+  cxd w[10];
+#pragma unroll
+  for (int s = 0; s < 10; s++) w[s] = cxd(r[(2 * s) * stride], r[(2 * s + 1) * stride]);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    cxd acc(0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc += w[symidx(i, j)] * y[j];
+    z[i] = acc;
+  }
+}
+
+template <typename T, int A>
+__global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
+  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int pb, pc;
+  if (!line_of<T, A>(J, tid, pb, pc)) return;
+  const LineGeo<T, A> G(L, pb, pc);
+  const long stride = (long)J.nlo * J.nhi;
+  const double* fac = J.fac + tid;
+  double* scr = J.scr + tid;
+  typename cvec<T>::type* __restrict__ u = J.u;
+  const typename cvec<T>::type* __restrict__ f = J.f;
+  const int N = G.N;
+
+  auto res_a = [&](int e) -> cxd {
+    int q[3];
+    q[A] = e; q[B] = pb; q[C] = pc;
+    return ld<T>(f, G.oa + e * G.sa) - apply_row<T, A>(L, u, q[A], q[B], q[C], G.base + e * G.sa);
+  };
+
+  // ---- forward substitution ----
+  cxd rd_prev(fac[20 * stride], fac[21 * stride]);
+  cxd q_prev = res_a(0) * rd_prev;
+  scr[8 * stride] = q_prev.re;
+  scr[9 * stride] = q_prev.im;
+  cxd z[4];
+  for (int p = 1; p <= N - 1; p++) {
+    const double* r = fac + (long)p * FREC * stride;
+    const cxd rd(r[20 * stride], r[21 * stride]);
+    const cxd q = res_a(p) * rd;
+    cxd g[4];
+    {
+      const long off = p * G.sa;
+      int qq[3];
+      qq[A] = p; qq[B] = pb - 1; qq[C] = pc;
+      g[0] = ld<T>(f, G.ob0 + off) - apply_row<T, B>(L, u, qq[B], qq[C], qq[A], G.ob0 + off - B * L.nc);
+      qq[B] = pb;
+      g[1] = ld<T>(f, G.ob1 + off) - apply_row<T, B>(L, u, qq[B], qq[C], qq[A], G.ob1 + off - B * L.nc);
+      qq[C] = pc - 1;
+      g[2] = ld<T>(f, G.oc0 + off) - apply_row<T, C>(L, u, qq[C], qq[A], qq[B], G.oc0 + off - C * L.nc);
+      qq[C] = pc;
+      g[3] = ld<T>(f, G.oc1 + off) - apply_row<T, C>(L, u, qq[C], qq[A], qq[B], G.oc1 + off - C * L.nc);
+    }
+    const cxd dq = q_prev - q;
+#pragma unroll
+    for (int i = 0; i < 4; i++) g[i] += dq * G.v[i];
+    if (p >= 2) {
+      double l[4];
+      G.lcoef(L, p - 1, l);
+      const cxd vz = z[0] * G.v[0] + z[1] * G.v[1] + z[2] * G.v[2] + z[3] * G.v[3];
+#pragma unroll
+      for (int i = 0; i < 4; i++) g[i] -= z[i] * l[i] + (vz * rd_prev) * G.v[i];
+    }
+    symv(r, stride, g, z);
+    double* s = scr + (long)p * SREC * stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { s[(2 * i) * stride] = z[i].re; s[(2 * i + 1) * stride] = z[i].im; }
+    s[8 * stride] = q.re;
+    s[9 * stride] = q.im;
+    rd_prev = rd;
+    q_prev = q;
+  }
+
+  // ---- back substitution and update ----
+  cxd an[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) an[i] = cxd(0, 0);
+  for (int p = N - 1; p >= 1; p--) {
+    const double* r = fac + (long)p * FREC * stride;
+    const double* s = scr + (long)p * SREC * stride;
+    const cxd rd(r[20 * stride], r[21 * stride]);
+    const cxd q(s[8 * stride], s[9 * stride]);
+    cxd ap[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) ap[i] = cxd(s[(2 * i) * stride], s[(2 * i + 1) * stride]);
+    if (p < N - 1) {
+      double l[4];
+      G.lcoef(L, p, l);
+      const cxd va = an[0] * G.v[0] + an[1] * G.v[1] + an[2] * G.v[2] + an[3] * G.v[3];
+      cxd t[4], wt[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) t[i] = an[i] * l[i] + (va * rd) * G.v[i];
+      symv(r, stride, t, wt);
+#pragma unroll
+      for (int i = 0; i < 4; i++) ap[i] -= wt[i];
+    }
+    {
+      cxd dv(0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
+      const cxd de = q + dv * rd;
+      const long o = G.oa + p * G.sa;
+      st<T>(u, o, ld<T>(u, o) + de);
+    }
+    {
+      const long off = p * G.sa;
+      st<T>(u, G.ob0 + off, ld<T>(u, G.ob0 + off) + ap[0]);
+      st<T>(u, G.ob1 + off, ld<T>(u, G.ob1 + off) + ap[1]);
+      st<T>(u, G.oc0 + off, ld<T>(u, G.oc0 + off) + ap[2]);
+      st<T>(u, G.oc1 + off, ld<T>(u, G.oc1 + off) + ap[3]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) an[i] = ap[i];
+  }
+  {
+    const cxd q0(scr[8 * stride], scr[9 * stride]);
+    const cxd rd0(fac[20 * stride], fac[21 * stride]);
+    cxd dv(0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dv += an[i] * G.v[i];
+    st<T>(u, G.oa, ld<T>(u, G.oa) + (q0 + dv * rd0));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi) {
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  const int cb = colour & 1, cc = (colour >> 1) & 1;
+  auto cnt = [](int m, int par) { const int s = par ? 1 : 2; return m - 1 >= s ? (m - 1 - s) / 2 + 1 : 0; };
+  const int nbv = cnt(n[b], cb), ncv = cnt(n[c], cc);
+  if (b < c) { *nlo = nbv; *nhi = ncv; } else { *nlo = ncv; *nhi = nbv; }
+}
+
+size_t line_factor_doubles(const int* n, int a, int colour) {
+  int nlo, nhi;
+  line_colour_counts(n, a, colour, &nlo, &nhi);
+  return (size_t)nlo * nhi * (size_t)n[a] * FREC;
+}
+
+size_t line_scratch_doubles(const int* n, int a, int colour) {
+  int nlo, nhi;
+  line_colour_counts(n, a, colour, &nlo, &nhi);
+  return (size_t)nlo * nhi * (size_t)n[a] * SREC;
+}
+
+template <typename T>
+static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
+  int maxl = 0;
+  double bytes = 0;
+  for (int j = 0; j < P.njobs; j++) {
+    const int nl = P.job[j].nlo * P.job[j].nhi;
+    if (nl > maxl) maxl = nl;
+    // compulsory traffic of one colour: a quarter of a full sweep, which
+    // reads u and f and writes u once (3 Ne s) and reads the medium (Ne c)
+    const double ne = 3.0 * (double)P.job[j].L.nc;
+    bytes += 0.25 * ne * (3.0 * sizeof(typename cvec<T>::type) + sizeof(T));
+  }
+  if (maxl == 0 || P.njobs == 0) return;
+  const dim3 g((maxl + 63) / 64, P.njobs), b(64);
+  if (factor) {
+    timing::Scope ts(timing::OTHER, s, 0);
+    if (axis == 0) hipLaunchKernelGGL((k_line_factor<T, 0>), g, b, 0, s, P);
+    else if (axis == 1) hipLaunchKernelGGL((k_line_factor<T, 1>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((k_line_factor<T, 2>), g, b, 0, s, P);
+  } else {
+    timing::Scope ts(timing::LINE, s, bytes);
+    if (axis == 0) hipLaunchKernelGGL((k_line_solve<T, 0>), g, b, 0, s, P);
+    else if (axis == 1) hipLaunchKernelGGL((k_line_solve<T, 1>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((k_line_solve<T, 2>), g, b, 0, s, P);
+  }
+}
+
+template <typename T>
+void launch_line_factor(const LinePack<T>& P, int axis, hipStream_t s) { launch_pack<T>(P, axis, true, s); }
+template <typename T>
+void launch_line_solve(const LinePack<T>& P, int axis, hipStream_t s) { launch_pack<T>(P, axis, false, s); }
+
+template void launch_line_factor<double>(const LinePack<double>&, int, hipStream_t);
+template void launch_line_factor<float>(const LinePack<float>&, int, hipStream_t);
+template void launch_line_solve<double>(const LinePack<double>&, int, hipStream_t);
+template void launch_line_solve<float>(const LinePack<float>&, int, hipStream_t);
+
+}  // namespace emmi

@@ -18,10 +18,6 @@ void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, d
 int residual_blocks(long nc);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 template <typename T>
-void launch_line_gs(const Level<T>& L, int axis, int colour, void* u, const void* f, double* scratch, hipStream_t s);
-template <typename T>
-size_t line_scratch_reals(const int* n);
-template <typename T>
 void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s);
 template <typename T>
 void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s);

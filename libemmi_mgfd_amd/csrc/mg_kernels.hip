@@ -16,38 +16,10 @@
 // zeros and are never written.
 #include "emmi_dev.h"
 #include "mg_kernels.h"
+#include "stencil.h"
 #include "timing.h"
 
 namespace emmi {
-
-// ---------------------------------------------------------------------------
-// operator row
-// ---------------------------------------------------------------------------
-template <typename T, int A>
-__device__ __forceinline__ cxd apply_row(const Level<T>& L, const typename cvec<T>::type* __restrict__ u,
-                                           int pa, int pb, int pc, long o) {
-  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
-  const long sa = L.es[A], sb = L.es[B], sc = L.es[C];
-  const typename cvec<T>::type* ua = u + A * L.nc + o;
-  const typename cvec<T>::type* ub = u + B * L.nc + o;
-  const typename cvec<T>::type* uc = u + C * L.nc + o;
-  const double iWa = L.IW[A][pa], iWb = L.IW[B][pb], iWbm = L.IW[B][pb - 1];
-  const double iWc = L.IW[C][pc], iWcm = L.IW[C][pc - 1];
-  const cxd ea = ld<T>(ua, 0);
-  const cxd c1 = (ld<T>(ub, sa) - ld<T>(ub, 0)) * iWa - (ld<T>(ua, sb) - ea) * iWb;
-  const cxd c2 = (ld<T>(ub, sa - sb) - ld<T>(ub, -sb)) * iWa - (ea - ld<T>(ua, -sb)) * iWbm;
-  const cxd c3 = (ld<T>(ua, sc) - ea) * iWc - (ld<T>(uc, sa) - ld<T>(uc, 0)) * iWa;
-  const cxd c4 = (ea - ld<T>(ua, -sc)) * iWcm - (ld<T>(uc, sa - sc) - ld<T>(uc, -sc)) * iWa;
-  cxd r = ((c1 - c2) * (double)L.H[C][pc] - (c3 - c4) * (double)L.H[B][pb]) * (double)L.W[A][pa];
-  return r + mul_minus_i(ea, L.omu * (double)L.sbar[A * L.nc + o]);
-}
-
-template <int A>
-__device__ __forceinline__ bool interior(const int* n, int i, int j, int k) {
-  const int p[3] = {i, j, k};
-  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
-  return p[A] < n[A] && p[B] >= 1 && p[B] <= n[B] - 1 && p[C] >= 1 && p[C] <= n[C] - 1;
-}
 
 // block-wide sum of doubles into partial[blockIdx]; blockDim multiple of 64
 __device__ __forceinline__ void block_sum_store(double v, double* partial) {
@@ -104,264 +76,6 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
     double s = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
     *out = s;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// line block Gauss-Seidel
-//
-// Line (a; pb, pc): all edges of the vertex patches of the nodes (p, pb, pc),
-// p = 1..N-1, N = n_a.  Unknowns: E_a(e) for e = 0..N-1 and, per node, the
-// four perpendicular edges s = 0: E_b(p,pb-1,pc), 1: E_b(p,pb,pc),
-// 2: E_c(p,pb,pc-1), 3: E_c(p,pb,pc).  The line system is solved exactly for
-// the correction of the current residual.  The along-line edges couple only
-// to the perpendicular edges of their two end nodes through the constant
-// vector v = (Hc, -Hc, Hb, -Hb):
-//     d_e dE_a(e) + v.da_e - v.da_{e+1} = rho_e
-// so they are eliminated first; the node unknowns then form a complex
-// symmetric 4x4 block-tridiagonal system solved by block Thomas:
-//     D_p = K_p - v v^T (1/d_{p-1} + 1/d_p),  B_p = diag(l_{p-1}) + v v^T/d_{p-1}
-//     W_p = D_p - B_p W_{p-1}^{-1} B_p,       y_p = g_p - B_p W_{p-1}^{-1} y_{p-1}
-// Per node the scratch keeps W_p^{-1} (symmetric, 10 entries), y_p and the
-// along-edge pair (rho_p/d_p, 1/d_p).
-// ---------------------------------------------------------------------------
-constexpr int LREC = 32;  // reals per node record
-
-__device__ __forceinline__ int symidx(int i, int j) {
-  // packed upper triangle of a 4x4
-  const int a = i < j ? i : j, b = i < j ? j : i;
-  return a * 4 - (a * (a - 1)) / 2 + (b - a);
-}
-
-template <typename T>
-__device__ __forceinline__ void inv4(cxd (&M)[4][4]) {
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const cxd piv = cinv(M[k][k]);
-    M[k][k] = cxd(1.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < 4; j++) M[k][j] = M[k][j] * piv;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (i == k) continue;
-      const cxd fct = M[i][k];
-      M[i][k] = cxd(0.0, 0.0);
-#pragma unroll
-      for (int j = 0; j < 4; j++) M[i][j] -= fct * M[k][j];
-    }
-  }
-}
-
-template <typename T, int A>
-__global__ void __launch_bounds__(64) k_line_gs(Level<T> L, typename cvec<T>::type* __restrict__ u,
-                                                const typename cvec<T>::type* __restrict__ f,
-                                                double* __restrict__ scratch, int cb, int cc, int nlo, int nhi) {
-  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
-  constexpr int LO = B < C ? B : C;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nlines = nlo * nhi;
-  if (tid >= nlines) return;
-  // lines of this colour: pb = sb + 2*ib, pc = sc + 2*ic; the perpendicular
-  // axis with the smaller stride varies fastest across threads
-  const int sb = cb ? 1 : 2, sc = cc ? 1 : 2;
-  int pb, pc;
-  if (LO == B) { pb = sb + 2 * (tid % nlo); pc = sc + 2 * (tid / nlo); }
-  else { pc = sc + 2 * (tid % nlo); pb = sb + 2 * (tid / nlo); }
-  const int N = L.n[A];
-  const long sa = L.es[A];
-  const long base = (long)pb * L.es[B] + (long)pc * L.es[C];
-  // offsets of the four perpendicular edges of node p (add p*sa)
-  const long ob0 = B * L.nc + base - L.es[B];
-  const long ob1 = B * L.nc + base;
-  const long oc0 = C * L.nc + base - L.es[C];
-  const long oc1 = C * L.nc + base;
-  const long oa = A * L.nc + base;
-
-  const double Hb = L.H[B][pb], Hc = L.H[C][pc];
-  const double Wb0 = L.W[B][pb - 1], Wb1 = L.W[B][pb], Wc0 = L.W[C][pc - 1], Wc1 = L.W[C][pc];
-  const double sIb = (double)L.IW[B][pb - 1] + (double)L.IW[B][pb];
-  const double sIc = (double)L.IW[C][pc - 1] + (double)L.IW[C][pc];
-  const double v[4] = {Hc, -Hc, Hb, -Hb};
-  const double dre_a = Hc * sIb + Hb * sIc;   // d_e = W_a(e) * dre_a - i omu sbar
-  const long stride = nlines;            // lane-interleaved records
-  double* rec = scratch + tid;
-
-  auto edge_d = [&](int e) -> cxd {
-    return cxd((double)L.W[A][e] * dre_a, -L.omu * (double)L.sbar[oa + e * sa]);
-  };
-  auto res_a = [&](int e) -> cxd {
-    const long o = base + e * sa;
-    int q[3];
-    q[A] = e; q[B] = pb; q[C] = pc;
-    return ld<T>(f, oa + e * sa) - apply_row<T, A>(L, u, q[A], q[B], q[C], o);
-  };
-
-  // ---- forward elimination ----
-  cxd Wi[4][4];      // W_{p-1}^{-1}
-  cxd y[4];          // y_{p-1}
-  cxd rd_prev, q_prev;
-  {
-    const cxd d0 = edge_d(0);
-    rd_prev = cinv(d0);
-    q_prev = res_a(0) * rd_prev;
-    rec[(0 * LREC + 28) * stride] = q_prev.re; rec[(0 * LREC + 29) * stride] = q_prev.im;
-    rec[(0 * LREC + 30) * stride] = rd_prev.re; rec[(0 * LREC + 31) * stride] = rd_prev.im;
-  }
-  for (int p = 1; p <= N - 1; p++) {
-    const cxd dp = edge_d(p);
-    const cxd rdp = cinv(dp);
-    const cxd qp = res_a(p) * rdp;
-    const double Ha = L.H[A][p];
-    const double iWa0 = L.IW[A][p - 1], iWa1 = L.IW[A][p];
-    // node rows: diagonal k, residual sigma
-    const double Gb = Ha * sIc + Hc * (iWa1 + iWa0);
-    const double Gc = Hb * (iWa1 + iWa0) + Ha * sIb;
-    cxd D[4][4];
-    cxd g[4];
-    {
-      const long e0 = ob0 + p * sa, e1 = ob1 + p * sa, e2 = oc0 + p * sa, e3 = oc1 + p * sa;
-      D[0][0] = cxd(Wb0 * Gb, -L.omu * (double)L.sbar[e0]);
-      D[1][1] = cxd(Wb1 * Gb, -L.omu * (double)L.sbar[e1]);
-      D[2][2] = cxd(Wc0 * Gc, -L.omu * (double)L.sbar[e2]);
-      D[3][3] = cxd(Wc1 * Gc, -L.omu * (double)L.sbar[e3]);
-      int q[3];
-      q[A] = p; q[B] = pb - 1; q[C] = pc;
-      g[0] = ld<T>(f, e0) - apply_row<T, B>(L, u, q[B], q[C], q[A], e0 - B * L.nc);
-      q[B] = pb;
-      g[1] = ld<T>(f, e1) - apply_row<T, B>(L, u, q[B], q[C], q[A], e1 - B * L.nc);
-      q[C] = pc - 1;
-      g[2] = ld<T>(f, e2) - apply_row<T, C>(L, u, q[C], q[A], q[B], e2 - C * L.nc);
-      q[C] = pc;
-      g[3] = ld<T>(f, e3) - apply_row<T, C>(L, u, q[C], q[A], q[B], e3 - C * L.nc);
-    }
-    D[0][1] = D[1][0] = cxd(0, 0);
-    D[2][3] = D[3][2] = cxd(0, 0);
-    D[0][2] = D[2][0] = cxd(-Ha, 0);
-    D[0][3] = D[3][0] = cxd(Ha, 0);
-    D[1][2] = D[2][1] = cxd(Ha, 0);
-    D[1][3] = D[3][1] = cxd(-Ha, 0);
-    // eliminate the along-line edges p-1 and p
-    {
-      const cxd s = rd_prev + rdp;
-      const cxd dq = q_prev - qp;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) D[i][j] -= s * (v[i] * v[j]);
-        g[i] += dq * v[i];
-      }
-    }
-    if (p >= 2) {
-      // B_p = diag(l) + v v^T rd_prev, l = along-line coupling to node p-1
-      const double l[4] = {-Wb0 * Hc * iWa0, -Wb1 * Hc * iWa0, -Wc0 * Hb * iWa0, -Wc1 * Hb * iWa0};
-      // P = W_{p-1}^{-1} B_p ; t = W_{p-1}^{-1} y_{p-1}
-      cxd wv[4], t[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        wv[i] = Wi[i][0] * v[0] + Wi[i][1] * v[1] + Wi[i][2] * v[2] + Wi[i][3] * v[3];
-        t[i] = Wi[i][0] * y[0] + Wi[i][1] * y[1] + Wi[i][2] * y[2] + Wi[i][3] * y[3];
-      }
-      cxd P[4][4];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) P[i][j] = Wi[i][j] * l[j] + (wv[i] * rd_prev) * v[j];
-      // B_p P and B_p t
-      cxd vP[4], vt = cxd(0, 0);
-#pragma unroll
-      for (int j = 0; j < 4; j++) vP[j] = P[0][j] * v[0] + P[1][j] * v[1] + P[2][j] * v[2] + P[3][j] * v[3];
-#pragma unroll
-      for (int i = 0; i < 4; i++) vt += t[i] * v[i];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) D[i][j] -= P[i][j] * l[i] + (vP[j] * rd_prev) * v[i];
-        g[i] -= t[i] * l[i] + (vt * rd_prev) * v[i];
-      }
-    }
-    // W_p = D; store W_p^{-1}, y_p, and the along-edge pair of edge p
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      y[i] = g[i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) Wi[i][j] = D[i][j];
-    }
-    inv4<T>(Wi);
-    double* r = rec + (long)p * LREC * stride;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = i; j < 4; j++) {
-        const int s = symidx(i, j);
-        const cxd w = (i == j) ? Wi[i][j] : (Wi[i][j] + Wi[j][i]) * 0.5;
-        r[(2 * s) * stride] = w.re;
-        r[(2 * s + 1) * stride] = w.im;
-      }
-#pragma unroll
-    for (int i = 0; i < 4; i++) { r[(20 + 2 * i) * stride] = y[i].re; r[(21 + 2 * i) * stride] = y[i].im; }
-    r[28 * stride] = qp.re; r[29 * stride] = qp.im;
-    r[30 * stride] = rdp.re; r[31 * stride] = rdp.im;
-    rd_prev = rdp;
-    q_prev = qp;
-  }
-
-  // ---- back substitution and update ----
-  cxd an[4];   // da_{p+1}
-#pragma unroll
-  for (int i = 0; i < 4; i++) an[i] = cxd(0, 0);
-  for (int p = N - 1; p >= 1; p--) {
-    const double* r = rec + (long)p * LREC * stride;
-    cxd M[4][4], rhs[4], ap[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int s = symidx(i, j);
-        M[i][j] = cxd(r[(2 * s) * stride], r[(2 * s + 1) * stride]);
-      }
-#pragma unroll
-    for (int i = 0; i < 4; i++) rhs[i] = cxd(r[(20 + 2 * i) * stride], r[(21 + 2 * i) * stride]);
-    const cxd qp(r[28 * stride], r[29 * stride]);
-    const cxd rdp(r[30 * stride], r[31 * stride]);
-    if (p < N - 1) {
-      // rhs -= B_{p+1} da_{p+1},  B_{p+1} = diag(l) + v v^T rd_p
-      const double iWa = L.IW[A][p];
-      const double l[4] = {-Wb0 * Hc * iWa, -Wb1 * Hc * iWa, -Wc0 * Hb * iWa, -Wc1 * Hb * iWa};
-      const cxd va = an[0] * v[0] + an[1] * v[1] + an[2] * v[2] + an[3] * v[3];
-#pragma unroll
-      for (int i = 0; i < 4; i++) rhs[i] -= an[i] * l[i] + (va * rdp) * v[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) ap[i] = M[i][0] * rhs[0] + M[i][1] * rhs[1] + M[i][2] * rhs[2] + M[i][3] * rhs[3];
-    // along-line edge p between node p and p+1
-    {
-      cxd dv = cxd(0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * v[i];
-      const cxd de = qp + dv * rdp;
-      const long o = oa + p * sa;
-      st<T>(u, o, ld<T>(u, o) + de);
-    }
-    {
-      const long e0 = ob0 + p * sa, e1 = ob1 + p * sa, e2 = oc0 + p * sa, e3 = oc1 + p * sa;
-      st<T>(u, e0, ld<T>(u, e0) + ap[0]);
-      st<T>(u, e1, ld<T>(u, e1) + ap[1]);
-      st<T>(u, e2, ld<T>(u, e2) + ap[2]);
-      st<T>(u, e3, ld<T>(u, e3) + ap[3]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) an[i] = ap[i];
-  }
-  {
-    // edge 0 between the boundary node 0 and node 1
-    const double* r = rec;
-    const cxd q0(r[28 * stride], r[29 * stride]);
-    const cxd rd0(r[30 * stride], r[31 * stride]);
-    cxd dv = cxd(0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; i++) dv += an[i] * v[i];
-    const cxd de = q0 + dv * rd0;
-    st<T>(u, oa, ld<T>(u, oa) + de);
   }
 }
 
@@ -579,43 +293,6 @@ void launch_sum_partials(const double* partial, int n, double* out, hipStream_t 
 }
 
 template <typename T>
-void line_colour_counts(const Level<T>& L, int a, int cb, int cc, int* nlo, int* nhi) {
-  const int b = (a + 1) % 3, c = (a + 2) % 3;
-  auto cnt = [](int n, int par) { const int s = par ? 1 : 2; return n - 1 >= s ? (n - 1 - s) / 2 + 1 : 0; };
-  const int nb = cnt(L.n[b], cb), ncc = cnt(L.n[c], cc);
-  if (b < c) { *nlo = nb; *nhi = ncc; } else { *nlo = ncc; *nhi = nb; }
-}
-
-template <typename T>
-void launch_line_gs(const Level<T>& L, int a, int colour, void* u, const void* f, double* scratch, hipStream_t s) {
-  using V = typename cvec<T>::type;
-  const int cb = colour & 1, cc = (colour >> 1) & 1;
-  int nlo, nhi;
-  line_colour_counts(L, a, cb, cc, &nlo, &nhi);
-  const int nl = nlo * nhi;
-  if (nl <= 0 || L.n[a] < 2) return;
-  const dim3 g(nblk(nl, 64)), b(64);
-  // rows of the launched lines: u, f and sbar read, u written
-  const double rows = (double)nl * (5.0 * L.n[a] - 4.0);
-  timing::Scope ts(timing::LINE, s, rows * (3.0 * sizeof(typename cvec<T>::type) + sizeof(T)));
-  if (a == 0) hipLaunchKernelGGL((k_line_gs<T, 0>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
-  else if (a == 1) hipLaunchKernelGGL((k_line_gs<T, 1>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
-  else hipLaunchKernelGGL((k_line_gs<T, 2>), g, b, 0, s, L, (V*)u, (const V*)f, scratch, cb, cc, nlo, nhi);
-}
-
-template <typename T>
-size_t line_scratch_reals(const int* n) {
-  size_t m = 0;
-  for (int a = 0; a < 3; a++) {
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
-    const size_t lines = (size_t)((n[b] + 1) / 2 + 1) * ((n[c] + 1) / 2 + 1);
-    const size_t need = lines * (size_t)(n[a] + 1) * LREC;
-    if (need > m) m = need;
-  }
-  return m;
-}
-
-template <typename T>
 void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s) {
   using V = typename cvec<T>::type;
   timing::Scope ts(timing::RESTRICT, s, 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
@@ -672,8 +349,6 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 
 #define EMMI_INSTANTIATE(T)                                                                                    \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
-  template void launch_line_gs<T>(const Level<T>&, int, int, void*, const void*, double*, hipStream_t);      \
-  template size_t line_scratch_reals<T>(const int*);                                                           \
   template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, \
                                    hipStream_t);                                                               \
   template void launch_prolong<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*,  \

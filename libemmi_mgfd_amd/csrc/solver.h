@@ -7,6 +7,7 @@
 
 #include "emmi_dev.h"
 #include "mg_kernels.h"
+#include "line_kernels.h"
 
 namespace emmi {
 
@@ -51,6 +52,12 @@ struct DevLevel {
   const T* D(int a) const { return geo + (5 * a + 4) * gstride; }
 };
 
+// factored line systems of one (level, axis), per colour, for one frequency
+struct LineFactors {
+  double* fac[4] = {nullptr, nullptr, nullptr, nullptr};
+  double omu = -1.0;   // frequency the records were built for
+};
+
 template <typename T>
 struct Hier {
   std::vector<std::vector<DevLevel<T>>> pat;  // [pattern][level]
@@ -84,8 +91,10 @@ class Solver {
   // kernel-level access for the element-wise checks (FP64 hierarchy)
   const Hier<double>& h64() const { return h64_; }
   const Hier<float>& h32() const { return h32_; }
-  double* scratch() { return scratch64_; }
   hipStream_t stream() const { return s_; }
+  // one line sweep (all four colours) along `axis` on vectors u, f of level
+  // `lev` of pattern `pat`, in the working precision T of the solver
+  template <typename T> void line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f);
 
  private:
   template <typename T> Hier<T>& hier();
@@ -93,6 +102,7 @@ class Solver {
   template <typename T> void smooth(int pat, int lev, int iter);
   template <typename T> void line_sweep(int pat, int lev, int axis, int iter);
   template <typename T> bool coarsens(int pat, int lev);
+  template <typename T> LineFactors& factors(int pat, int lev, int axis);
   template <typename T> bool v_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool f_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool w_cycle(int pat, int lev, bool norm_here, bool zero_guess);
@@ -112,7 +122,8 @@ class Solver {
   Hier<double> h64_;
   Hier<float> h32_;
   DevLevel<double> l0_64_;    // level-0 FP64 operator of the refinement
-  double* scratch64_ = nullptr;
+  std::vector<LineFactors> facs_;   // [(pattern*lmax + level)*3 + axis]; level 0 under pattern 0
+  double* lscr_ = nullptr;          // substitution scratch of the line solves
   void* u64_ = nullptr;
   void* f64_ = nullptr;
   void* r64_ = nullptr;

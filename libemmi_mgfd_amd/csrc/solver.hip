@@ -10,6 +10,7 @@
 // coarsest solve exact when a single line remains.
 #include "solver.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -197,15 +198,14 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
     l0_64_ = h64_.pat[0][0];
   }
 
-  // scratch of the line solves: max over levels, patterns and axes
+  // scratch of the line solves: max over levels, patterns, axes and colours
   size_t sreal = 0;
   for (int c = 0; c < npat_; c++)
-    for (int lev = 0; lev < lmax_; lev++) {
-      const size_t s = p_.precision ? line_scratch_reals<float>(dims_[c][lev].data())
-                                    : line_scratch_reals<double>(dims_[c][lev].data());
-      if (s > sreal) sreal = s;
-    }
-  scratch64_ = (double*)dalloc(sizeof(double) * sreal, owned_);
+    for (int lev = 0; lev < lmax_; lev++)
+      for (int a = 0; a < 3; a++)
+        for (int col = 0; col < 4; col++) sreal = std::max(sreal, line_scratch_doubles(dims_[c][lev].data(), a, col));
+  lscr_ = (double*)dalloc(sizeof(double) * sreal, owned_);
+  facs_.assign((size_t)npat_ * lmax_ * 3, LineFactors());
 
   partial_ = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
   dnorm_ = (double*)dalloc(sizeof(double) * 2, owned_);
@@ -294,15 +294,57 @@ bool Solver::coarsens(int pat, int lev) {
   return sc[0] * sc[1] * sc[2] > 1;
 }
 
+// factor records of (pattern, level, axis), allocated on first use and
+// rebuilt when the frequency changed since they were made
+template <typename T>
+LineFactors& Solver::factors(int pat, int lev, int axis) {
+  const int pe = lev == 0 ? 0 : pat;
+  LineFactors& F = facs_[((size_t)pe * lmax_ + lev) * 3 + axis];
+  if (F.omu == omu_) return F;
+  const DevLevel<T>& L = hier<T>().pat[pe][lev];
+  LinePack<T> P;
+  P.njobs = 4;
+  for (int col = 0; col < 4; col++) {
+    if (!F.fac[col]) F.fac[col] = (double*)dalloc(sizeof(double) * std::max<size_t>(1, line_factor_doubles(L.n, axis, col)), owned_);
+    LineJob<T>& J = P.job[col];
+    J.L = L.view(omu_);
+    J.u = nullptr;
+    J.f = nullptr;
+    J.fac = F.fac[col];
+    J.scr = nullptr;
+    line_job_colour(J, axis, col);
+  }
+  launch_line_factor<T>(P, axis, s_);
+  F.omu = omu_;
+  return F;
+}
+
+template <typename T>
+void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f) {
+  const LineFactors& F = factors<T>(pat, lev, axis);
+  const DevLevel<T>& L = hier<T>().pat[pat][lev];
+  LinePack<T> P;
+  P.njobs = 1;
+  LineJob<T>& J = P.job[0];
+  J.L = L.view(omu_);
+  J.u = (typename cvec<T>::type*)u;
+  J.f = (const typename cvec<T>::type*)f;
+  J.scr = lscr_;
+  for (int k = 0; k < 4; k++) {
+    const int colour = (iter % 2 == 1) ? 3 - k : k;
+    J.fac = F.fac[colour];
+    line_job_colour(J, axis, colour);
+    launch_line_solve<T>(P, axis, s_);
+  }
+}
+
+template void Solver::line_sweep_on<double>(int, int, int, int, void*, const void*);
+template void Solver::line_sweep_on<float>(int, int, int, int, void*, const void*);
+
 template <typename T>
 void Solver::line_sweep(int pat, int lev, int axis, int iter) {
   Hier<T>& H = hier<T>();
-  const DevLevel<T>& L = H.pat[pat][lev];
-  const Level<T> V = L.view(omu_);
-  for (int k = 0; k < 4; k++) {
-    const int colour = (iter % 2 == 1) ? 3 - k : k;
-    launch_line_gs<T>(V, axis, colour, H.u[lev], H.f[lev], scratch64_, s_);
-  }
+  line_sweep_on<T>(pat, lev, axis, iter, H.u[lev], H.f[lev]);
 }
 
 // src/gmg.c:1301-1313
