@@ -174,6 +174,25 @@ int emmi_solver_set_omega(emmi_solver_t *s, float omega);
 /* residual norms of the last solve, one per cycle; returns the count */
 int emmi_solver_history(emmi_solver_t *s, double *out, int maxn);
 void emmi_solver_destroy(emmi_solver_t *s);
+
+/* Batched solves (no counterpart in the reference, which solves one
+ * frequency per gmg_init/gmg_apply pair in the loop of
+ * src/do_modelling.c:49-67).  A batch solver holds nslots right-hand sides
+ * that share the grid and medium of one shot, each with its own frequency
+ * omegas[k].  emmi_solver_solve_batch solves slots 0..n-1 together: every
+ * kernel launch of a cycle serves all unconverged slots, and each slot stops
+ * when its own residual test passes, so slot k gets exactly the iterates a
+ * single solve at omegas[k] would produce.  b, x, h, x0 are tables of n
+ * device vectors (x0 may be NULL or hold NULL entries); st receives n
+ * records.  Returns the worst status of the batch or EMMI_ERROR. */
+emmi_solver_t *emmi_solver_create_batch(int n1, int n2, int n3, const float *x1, const float *x2, const float *x3,
+                                        const float *sig11, const float *sig22, const float *sig33,
+                                        int sig_on_device, const float *omegas, int nslots, const emmi_params_t *p);
+int emmi_solver_solve_batch(emmi_solver_t *s, int n, const void *const *b, void *const *x, void *const *h,
+                            const void *const *x0, emmi_stats_t *st);
+int emmi_solver_set_omega_slot(emmi_solver_t *s, int slot, float omega);
+int emmi_solver_history_slot(emmi_solver_t *s, int slot, double *out, int maxn);
+int emmi_solver_slots(emmi_solver_t *s);
 int emmi_solver_levels(emmi_solver_t *s, int *npatterns);
 void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
 

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -63,7 +64,7 @@ static void validate(const SolveParams& q) {
 struct emmi_solver {
   Solver* s = nullptr;
   hipStream_t stream = nullptr;
-  std::vector<double> hist;
+  std::vector<std::vector<double>> hist;   // per slot, of the last solve
   int precision = 0;
 };
 
@@ -84,18 +85,20 @@ void emmi_default_params(emmi_params_t* p) {
 
 const char* emmi_last_error(void) { return g_err.c_str(); }
 
-emmi_solver_t* emmi_solver_create(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3,
-                                  const float* sig11, const float* sig22, const float* sig33, int sig_on_device,
-                                  float omega, const emmi_params_t* p) {
+emmi_solver_t* emmi_solver_create_batch(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3,
+                                        const float* sig11, const float* sig22, const float* sig33, int sig_on_device,
+                                        const float* omegas, int nslots, const emmi_params_t* p) {
   emmi_solver* h = new emmi_solver;
   try {
     SolveParams q = to_params(p);
     validate(q);
     if (n1 < 2 || n2 < 2 || n3 < 2) throw std::runtime_error("n1, n2, n3 must be >= 2");
     if (!x1 || !x2 || !x3 || !sig11 || !sig22 || !sig33) throw std::runtime_error("null grid or medium");
+    if (!omegas || nslots < 1) throw std::runtime_error("need at least one frequency");
     check_hip(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     h->precision = q.precision;
-    h->s = new Solver(n1, n2, n3, x1, x2, x3, sig11, sig22, sig33, sig_on_device, omega, q, h->stream);
+    h->s = new Solver(n1, n2, n3, x1, x2, x3, sig11, sig22, sig33, sig_on_device, omegas, nslots, q, h->stream);
+    h->hist.assign(nslots, {});
     return h;
   } catch (const std::exception& e) {
     set_err(e.what());
@@ -105,37 +108,68 @@ emmi_solver_t* emmi_solver_create(int n1, int n2, int n3, const float* x1, const
   }
 }
 
-int emmi_solver_solve(emmi_solver_t* h, const void* b, void* x, void* hout, const void* x0, emmi_stats_t* st) {
+emmi_solver_t* emmi_solver_create(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3,
+                                  const float* sig11, const float* sig22, const float* sig33, int sig_on_device,
+                                  float omega, const emmi_params_t* p) {
+  return emmi_solver_create_batch(n1, n2, n3, x1, x2, x3, sig11, sig22, sig33, sig_on_device, &omega, 1, p);
+}
+
+int emmi_solver_solve_batch(emmi_solver_t* h, int n, const void* const* b, void* const* x, void* const* hout,
+                            const void* const* x0, emmi_stats_t* st) {
   if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
   try {
-    SolveStats ss;
-    h->s->solve(b, x, hout, x0, &ss);
-    h->hist = ss.history;
-    if (st) {
-      st->cycles = ss.cycles;
-      st->status = ss.status;
-      st->rnorm0 = ss.rnorm0;
-      st->rnorm = ss.rnorm;
+    if (n < 1 || n > h->s->nslots()) throw std::runtime_error("batch size must be 1..nslots");
+    if (!b || !x || !hout) throw std::runtime_error("null vector table");
+    for (int k = 0; k < n; k++)
+      if (!b[k] || !x[k] || !hout[k]) throw std::runtime_error("null device vector in the batch");
+    std::vector<SolveStats> ss(n);
+    h->s->solve_batch(n, b, x, hout, x0, ss.data());
+    int worst = 0;
+    for (int k = 0; k < n; k++) {
+      h->hist[k] = ss[k].history;
+      if (st) {
+        st[k].cycles = ss[k].cycles;
+        st[k].status = ss[k].status;
+        st[k].rnorm0 = ss[k].rnorm0;
+        st[k].rnorm = ss[k].rnorm;
+      }
+      worst = std::max(worst, ss[k].status);
     }
-    return ss.status;
+    return worst;
   } catch (const std::exception& e) {
     set_err(e.what());
     return EMMI_ERROR;
   }
 }
 
-int emmi_solver_set_omega(emmi_solver_t* h, float omega) {
-  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
-  h->s->set_omega(omega);
-  return 0;
+int emmi_solver_solve(emmi_solver_t* h, const void* b, void* x, void* hout, const void* x0, emmi_stats_t* st) {
+  return emmi_solver_solve_batch(h, 1, &b, &x, &hout, &x0, st);
 }
 
-int emmi_solver_history(emmi_solver_t* h, double* out, int maxn) {
-  if (!h) return 0;
-  const int n = (int)h->hist.size();
-  for (int i = 0; i < n && i < maxn; i++) out[i] = h->hist[i];
+int emmi_solver_set_omega_slot(emmi_solver_t* h, int slot, float omega) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    h->s->set_omega(slot, omega);
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
+int emmi_solver_set_omega(emmi_solver_t* h, float omega) { return emmi_solver_set_omega_slot(h, 0, omega); }
+
+int emmi_solver_history_slot(emmi_solver_t* h, int slot, double* out, int maxn) {
+  if (!h || slot < 0 || slot >= (int)h->hist.size()) return 0;
+  const std::vector<double>& v = h->hist[slot];
+  const int n = (int)v.size();
+  for (int i = 0; i < n && i < maxn; i++) out[i] = v[i];
   return n;
 }
+
+int emmi_solver_history(emmi_solver_t* h, double* out, int maxn) { return emmi_solver_history_slot(h, 0, out, maxn); }
+
+int emmi_solver_slots(emmi_solver_t* h) { return (h && h->s) ? h->s->nslots() : 0; }
 
 void emmi_solver_destroy(emmi_solver_t* h) {
   if (!h) return;

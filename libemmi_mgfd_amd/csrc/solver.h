@@ -1,4 +1,11 @@
 // solver.h -- multigrid Maxwell solver on one GPU (host orchestration).
+//
+// One Solver owns the level hierarchy of one grid and medium (shared by all
+// frequencies of a shot) and a number of right-hand-side slots, each with its
+// own frequency, level vectors, factored line systems and convergence state.
+// solve_batch runs the slots in lockstep -- every kernel launch of a cycle
+// serves all active slots -- and retires each slot when its residual test
+// passes, so every slot follows exactly the iteration it would follow alone.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,25 +59,40 @@ struct DevLevel {
   const T* D(int a) const { return geo + (5 * a + 4) * gstride; }
 };
 
+template <typename T>
+struct Hier {
+  std::vector<std::vector<DevLevel<T>>> pat;  // [pattern][level]
+  std::vector<long> maxnc;                    // per level, max over patterns
+};
+
 // factored line systems of one (level, axis), per colour, for one frequency
 struct LineFactors {
   double* fac[4] = {nullptr, nullptr, nullptr, nullptr};
   double omu = -1.0;   // frequency the records were built for
 };
 
-template <typename T>
-struct Hier {
-  std::vector<std::vector<DevLevel<T>>> pat;  // [pattern][level]
-  std::vector<void*> u, f, r;                 // per level, sized to the max over patterns
+// one right-hand side
+struct Slot {
+  double omu = 0;
+  std::vector<void*> u, f, r;        // per level, working precision
+  void* u64 = nullptr;               // level-0 FP64 state (aliases u/f/r[0] in FP64 mode)
+  void* f64 = nullptr;
+  void* r64 = nullptr;
+  std::vector<LineFactors> facs;     // [(pattern*lmax + level)*3 + axis]; level 0 under pattern 0
+  double* lscr = nullptr;            // substitution scratch of the line solves
+  double* partial = nullptr;         // residual-norm partial sums
+  double rnorm0 = 0;
+  SolveStats* st = nullptr;
 };
 
 class Solver {
  public:
   // x1..x3: float node coordinates (n+1); sig: float [n3][n2][n1] conductivities
-  // (host pointers when sig_on_device == 0, device pointers otherwise)
+  // (host pointers when sig_on_device == 0, device pointers otherwise);
+  // omegas: the angular frequency of each of the nslots right-hand sides
   Solver(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3, const float* sig11,
-         const float* sig22, const float* sig33, int sig_on_device, float omega, const SolveParams& p,
-         hipStream_t stream);
+         const float* sig22, const float* sig33, int sig_on_device, const float* omegas, int nslots,
+         const SolveParams& p, hipStream_t stream);
   ~Solver();
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -79,13 +101,18 @@ class Solver {
   // source term (before the dual-volume scaling); x receives E, h receives H
   // (h may alias b).  x0 (device, may be null) is an initial guess.
   int solve(const void* b, void* x, void* h, const void* x0, SolveStats* st);
+  // slots 0..n-1 together; x0 may be null or hold null entries
+  int solve_batch(int n, const void* const* b, void* const* x, void* const* h, const void* const* x0,
+                  SolveStats* st);
 
   long nvec() const { return 3 * nc0_; }
   int lmax() const { return lmax_; }
   int npatterns() const { return npat_; }
-  double omu() const { return omu_; }
+  int nslots() const { return (int)slots_.size(); }
+  double omu(int slot = 0) const { return slots_[slot].omu; }
   // the hierarchy depends on the grid and medium only: switch frequency
-  void set_omega(float omega) { omu_ = (double)omega * (4. * 3.14159265358979323846264 * 1e-7); }
+  void set_omega(float omega) { set_omega(0, omega); }
+  void set_omega(int slot, float omega);
   const int* dims(int pat, int lev) const { return dims_[pat][lev].data(); }
 
   // kernel-level access for the element-wise checks (FP64 hierarchy)
@@ -93,28 +120,32 @@ class Solver {
   const Hier<float>& h32() const { return h32_; }
   hipStream_t stream() const { return s_; }
   // one line sweep (all four colours) along `axis` on vectors u, f of level
-  // `lev` of pattern `pat`, in the working precision T of the solver
+  // `lev` of pattern `pat`, with the frequency of slot 0, in the working
+  // precision T of the solver
   template <typename T> void line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f);
 
  private:
   template <typename T> Hier<T>& hier();
   template <typename T> void build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev);
+  template <typename T> void alloc_vectors(Slot& S, const Hier<T>& H);
   template <typename T> void smooth(int pat, int lev, int iter);
   template <typename T> void line_sweep(int pat, int lev, int axis, int iter);
   template <typename T> bool coarsens(int pat, int lev);
-  template <typename T> LineFactors& factors(int pat, int lev, int axis);
+  bool exact_coarsest(int pat) const;
+  template <typename T> LineFactors& factors(Slot& S, int pat, int lev, int axis);
   template <typename T> bool v_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool f_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool w_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool coarse_correction(int pat, int lev, bool norm_here, bool zero_guess, int kind);
-  bool check(double rn);
-  double norm_of(const DevLevel<double>& L, const void* u, const void* f, void* r);
-  double vec_norm(const void* v);
+  // level-0 FP64 residual norms of the active slots (r64 <- f64 - A u64);
+  // retires the converged ones and returns true when none is left
+  bool check_active(int pat);
+  void norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,
+             std::vector<double>& out);
 
   int n_[3];
   long nc0_;
   int lmax_, npat_;
-  double omu_;
   SolveParams p_;
   hipStream_t s_;
   std::vector<std::vector<std::array<int, 3>>> dims_, scs_;   // [pattern][level]
@@ -122,20 +153,14 @@ class Solver {
   Hier<double> h64_;
   Hier<float> h32_;
   DevLevel<double> l0_64_;    // level-0 FP64 operator of the refinement
-  std::vector<LineFactors> facs_;   // [(pattern*lmax + level)*3 + axis]; level 0 under pattern 0
-  double* lscr_ = nullptr;          // substitution scratch of the line solves
-  void* u64_ = nullptr;
-  void* f64_ = nullptr;
-  void* r64_ = nullptr;
-  double* partial_ = nullptr;
-  double* dnorm_ = nullptr;
-  double* hnorm_ = nullptr;   // pinned host
+  std::vector<Slot> slots_;
+  size_t lscr_doubles_ = 0;
+  double* dnorm_ = nullptr;   // one per slot
+  double* hnorm_ = nullptr;   // pinned host, one per slot
   std::vector<void*> owned_;
-  // convergence state of the running solve
+  // state of the running batch
+  std::vector<int> act_;      // active slots
   int icycle_ = 0;
-  double rnorm0_ = 0;
-  SolveStats* st_ = nullptr;
-  bool stopped_ = false;
 };
 
 }  // namespace emmi

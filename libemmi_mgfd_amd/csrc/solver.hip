@@ -95,19 +95,21 @@ static void* dalloc(size_t bytes, std::vector<void*>& owned) {
 template <> Hier<double>& Solver::hier<double>() { return h64_; }
 template <> Hier<float>& Solver::hier<float>() { return h32_; }
 
+
+static constexpr double MU0 = 4. * 3.14159265358979323846264 * 1e-7;
+
 Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3, const float* sig11,
-               const float* sig22, const float* sig33, int sig_on_device, float omega, const SolveParams& p,
-               hipStream_t stream)
+               const float* sig22, const float* sig33, int sig_on_device, const float* omegas, int nslots,
+               const SolveParams& p, hipStream_t stream)
     : p_(p), s_(stream) {
   n_[0] = n1; n_[1] = n2; n_[2] = n3;
   for (int a = 0; a < 3; a++)
     if (n_[a] < 2) throw std::runtime_error("multigrid size must be at least 2 cells per axis");
+  if (nslots < 1) throw std::runtime_error("a solver needs at least one right-hand-side slot");
   nc0_ = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
   lmax_ = std::max(depth_of(n1), std::max(depth_of(n2), depth_of(n3)));
   if (lmax_ < 1) lmax_ = 1;
   npat_ = p_.isemicoarsen ? 3 : 1;
-  // I omega mu0 with omega stored as float (src/gmg.c:1709)
-  omu_ = (double)omega * (4. * 3.14159265358979323846264 * 1e-7);
 
   // ---- host geometry of all levels and patterns ----
   const float* xf[3] = {x1, x2, x3};
@@ -181,35 +183,44 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
 
   if (p_.precision == 0) {
     build_hier<double>(h64_, cells, lmax_);
+    l0_64_ = h64_.pat[0][0];
   } else {
     build_hier<float>(h32_, cells, lmax_);
-    // FP64 level-0 operator and state of the refinement
+    // FP64 level-0 operator of the refinement
     Hier<double> one;
     build_hier<double>(one, cells, 1);
     l0_64_ = one.pat[0][0];
-    u64_ = one.u[0];
-    f64_ = one.f[0];
-    r64_ = one.r[0];
-  }
-  if (p_.precision == 0) {
-    u64_ = h64_.u[0];
-    f64_ = h64_.f[0];
-    r64_ = h64_.r[0];
-    l0_64_ = h64_.pat[0][0];
   }
 
   // scratch of the line solves: max over levels, patterns, axes and colours
-  size_t sreal = 0;
   for (int c = 0; c < npat_; c++)
     for (int lev = 0; lev < lmax_; lev++)
       for (int a = 0; a < 3; a++)
-        for (int col = 0; col < 4; col++) sreal = std::max(sreal, line_scratch_doubles(dims_[c][lev].data(), a, col));
-  lscr_ = (double*)dalloc(sizeof(double) * sreal, owned_);
-  facs_.assign((size_t)npat_ * lmax_ * 3, LineFactors());
+        for (int col = 0; col < 4; col++)
+          lscr_doubles_ = std::max(lscr_doubles_, line_scratch_doubles(dims_[c][lev].data(), a, col));
 
-  partial_ = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
-  dnorm_ = (double*)dalloc(sizeof(double) * 2, owned_);
-  HIPCHECK(hipHostMalloc((void**)&hnorm_, sizeof(double) * 2, hipHostMallocDefault));
+  slots_.resize(nslots);
+  for (int k = 0; k < nslots; k++) {
+    Slot& S = slots_[k];
+    S.omu = (double)omegas[k] * MU0;   // omega stored as float (src/gmg.c:1709)
+    if (p_.precision == 0) {
+      alloc_vectors<double>(S, h64_);
+      S.u64 = S.u[0];
+      S.f64 = S.f[0];
+      S.r64 = S.r[0];
+    } else {
+      alloc_vectors<float>(S, h32_);
+      const size_t b = sizeof(double2) * 3 * (size_t)nc0_;
+      S.u64 = dalloc(b, owned_);
+      S.f64 = dalloc(b, owned_);
+      S.r64 = dalloc(b, owned_);
+    }
+    S.facs.assign((size_t)npat_ * lmax_ * 3, LineFactors());
+    S.lscr = (double*)dalloc(sizeof(double) * std::max<size_t>(1, lscr_doubles_), owned_);
+    S.partial = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
+  }
+  dnorm_ = (double*)dalloc(sizeof(double) * nslots, owned_);
+  HIPCHECK(hipHostMalloc((void**)&hnorm_, sizeof(double) * nslots, hipHostMallocDefault));
   HIPCHECK(hipStreamSynchronize(s_));
   for (void* t : tmp) HIPCHECK(hipFree(t));
 }
@@ -220,17 +231,22 @@ Solver::~Solver() {
   if (hnorm_) hipHostFree(hnorm_);
 }
 
+void Solver::set_omega(int slot, float omega) {
+  if (slot < 0 || slot >= nslots()) throw std::runtime_error("slot out of range");
+  slots_[slot].omu = (double)omega * MU0;
+}
+
 template <typename T>
 void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev) {
   H.pat.assign(npat_, std::vector<DevLevel<T>>(nlev));
-  std::vector<long> maxnc(nlev, 0);
+  H.maxnc.assign(nlev, 0);
   for (int c = 0; c < npat_; c++)
     for (int lev = 0; lev < nlev; lev++) {
       DevLevel<T>& L = H.pat[c][lev];
       const auto& n = dims_[c][lev];
       for (int a = 0; a < 3; a++) { L.n[a] = n[a]; L.sc[a] = scs_[c][lev][a]; }
       L.nc = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
-      if (L.nc > maxnc[lev]) maxnc[lev] = L.nc;
+      if (L.nc > H.maxnc[lev]) H.maxnc[lev] = L.nc;
       // level 0 is shared by the patterns
       if (c > 0 && lev == 0) { L = H.pat[0][0]; continue; }
       const int nmax = std::max(n[0], std::max(n[1], n[2]));
@@ -270,22 +286,27 @@ void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cel
       const auto& cl = cells[c * lmax_ + lev];
       launch_edge_sigma<T>(n[0], n[1], n[2], cl[0], cl[1], cl[2], L.sbar, s_);
     }
-  H.u.assign(nlev, nullptr);
-  H.f.assign(nlev, nullptr);
-  H.r.assign(nlev, nullptr);
+}
+
+template <typename T>
+void Solver::alloc_vectors(Slot& S, const Hier<T>& H) {
+  const int nlev = (int)H.maxnc.size();
+  S.u.assign(nlev, nullptr);
+  S.f.assign(nlev, nullptr);
+  S.r.assign(nlev, nullptr);
   for (int lev = 0; lev < nlev; lev++) {
-    const size_t bytes = sizeof(typename cvec<T>::type) * 3 * (size_t)maxnc[lev];
-    H.u[lev] = dalloc(bytes, owned_);
-    H.f[lev] = dalloc(bytes, owned_);
-    H.r[lev] = dalloc(bytes, owned_);
-    launch_zero<T>(H.u[lev], 3 * maxnc[lev], s_);
-    launch_zero<T>(H.f[lev], 3 * maxnc[lev], s_);
-    launch_zero<T>(H.r[lev], 3 * maxnc[lev], s_);
+    const size_t bytes = sizeof(typename cvec<T>::type) * 3 * (size_t)H.maxnc[lev];
+    S.u[lev] = dalloc(bytes, owned_);
+    S.f[lev] = dalloc(bytes, owned_);
+    S.r[lev] = dalloc(bytes, owned_);
+    launch_zero<T>(S.u[lev], 3 * H.maxnc[lev], s_);
+    launch_zero<T>(S.f[lev], 3 * H.maxnc[lev], s_);
+    launch_zero<T>(S.r[lev], 3 * H.maxnc[lev], s_);
   }
 }
 
 // ---------------------------------------------------------------------------
-// cycles
+// smoothing
 // ---------------------------------------------------------------------------
 template <typename T>
 bool Solver::coarsens(int pat, int lev) {
@@ -294,20 +315,29 @@ bool Solver::coarsens(int pat, int lev) {
   return sc[0] * sc[1] * sc[2] > 1;
 }
 
-// factor records of (pattern, level, axis), allocated on first use and
-// rebuilt when the frequency changed since they were made
+// the coarsest level of a semi-coarsening pattern is two cells wide across
+// the kept axis: its interior unknowns form a single line along that axis,
+// so one line sweep is an exact solve
+bool Solver::exact_coarsest(int pat) const {
+  if (!p_.isemicoarsen) return false;
+  const auto& n = dims_[pat][lmax_ - 1];
+  return n[(pat + 1) % 3] == 2 && n[(pat + 2) % 3] == 2;
+}
+
+// factor records of (pattern, level, axis) for slot S, allocated on first use
+// and rebuilt when the slot's frequency changed since they were made
 template <typename T>
-LineFactors& Solver::factors(int pat, int lev, int axis) {
+LineFactors& Solver::factors(Slot& S, int pat, int lev, int axis) {
   const int pe = lev == 0 ? 0 : pat;
-  LineFactors& F = facs_[((size_t)pe * lmax_ + lev) * 3 + axis];
-  if (F.omu == omu_) return F;
+  LineFactors& F = S.facs[((size_t)pe * lmax_ + lev) * 3 + axis];
+  if (F.omu == S.omu) return F;
   const DevLevel<T>& L = hier<T>().pat[pe][lev];
   LinePack<T> P;
   P.njobs = 4;
   for (int col = 0; col < 4; col++) {
     if (!F.fac[col]) F.fac[col] = (double*)dalloc(sizeof(double) * std::max<size_t>(1, line_factor_doubles(L.n, axis, col)), owned_);
     LineJob<T>& J = P.job[col];
-    J.L = L.view(omu_);
+    J.L = L.view(S.omu);
     J.u = nullptr;
     J.f = nullptr;
     J.fac = F.fac[col];
@@ -315,21 +345,22 @@ LineFactors& Solver::factors(int pat, int lev, int axis) {
     line_job_colour(J, axis, col);
   }
   launch_line_factor<T>(P, axis, s_);
-  F.omu = omu_;
+  F.omu = S.omu;
   return F;
 }
 
 template <typename T>
 void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f) {
-  const LineFactors& F = factors<T>(pat, lev, axis);
+  Slot& S = slots_[0];
+  const LineFactors& F = factors<T>(S, pat, lev, axis);
   const DevLevel<T>& L = hier<T>().pat[pat][lev];
   LinePack<T> P;
   P.njobs = 1;
   LineJob<T>& J = P.job[0];
-  J.L = L.view(omu_);
+  J.L = L.view(S.omu);
   J.u = (typename cvec<T>::type*)u;
   J.f = (const typename cvec<T>::type*)f;
-  J.scr = lscr_;
+  J.scr = S.lscr;
   for (int k = 0; k < 4; k++) {
     const int colour = (iter % 2 == 1) ? 3 - k : k;
     J.fac = F.fac[colour];
@@ -341,10 +372,31 @@ void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const 
 template void Solver::line_sweep_on<double>(int, int, int, int, void*, const void*);
 template void Solver::line_sweep_on<float>(int, int, int, int, void*, const void*);
 
+// one sweep along `axis` for every active slot: per colour, the slots share
+// launches of up to LINE_PACK_MAX jobs
 template <typename T>
 void Solver::line_sweep(int pat, int lev, int axis, int iter) {
-  Hier<T>& H = hier<T>();
-  line_sweep_on<T>(pat, lev, axis, iter, H.u[lev], H.f[lev]);
+  const DevLevel<T>& L = hier<T>().pat[pat][lev];
+  std::vector<const LineFactors*> F(act_.size());
+  for (size_t i = 0; i < act_.size(); i++) F[i] = &factors<T>(slots_[act_[i]], pat, lev, axis);
+  for (int k = 0; k < 4; k++) {
+    const int colour = (iter % 2 == 1) ? 3 - k : k;
+    for (size_t i0 = 0; i0 < act_.size(); i0 += LINE_PACK_MAX) {
+      LinePack<T> P;
+      P.njobs = (int)std::min<size_t>(LINE_PACK_MAX, act_.size() - i0);
+      for (int j = 0; j < P.njobs; j++) {
+        Slot& S = slots_[act_[i0 + j]];
+        LineJob<T>& J = P.job[j];
+        J.L = L.view(S.omu);
+        J.u = (typename cvec<T>::type*)S.u[lev];
+        J.f = (const typename cvec<T>::type*)S.f[lev];
+        J.fac = F[i0 + j]->fac[colour];
+        J.scr = S.lscr;
+        line_job_colour(J, axis, colour);
+      }
+      launch_line_solve<T>(P, axis, s_);
+    }
+  }
 }
 
 // src/gmg.c:1301-1313
@@ -361,57 +413,96 @@ void Solver::smooth(int pat, int lev, int iter) {
   }
 }
 
-bool Solver::check(double rn) {
-  if (st_) st_->history.push_back(rn);
-  if (p_.verb) printf("icycle=%d rnorm=%e\n", icycle_, rn);
-  if (!std::isfinite(rn)) {
-    if (st_) st_->status = 2;
-    return true;
+// ---------------------------------------------------------------------------
+// convergence tests
+// ---------------------------------------------------------------------------
+void Solver::norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,
+                   std::vector<double>& out) {
+  for (size_t i = 0; i < which.size(); i++) {
+    Slot& S = slots_[which[i]];
+    launch_residual<double>(l0_64_.view(S.omu), u[i], f[i], r[i], S.partial, s_);
+    launch_sum_partials(S.partial, residual_blocks(l0_64_.nc), dnorm_ + which[i], s_);
   }
-  if (st_) st_->rnorm = rn;
-  return rn < rnorm0_ * p_.tol;
-}
-
-double Solver::norm_of(const DevLevel<double>& L, const void* u, const void* f, void* r) {
-  launch_residual<double>(L.view(omu_), u, f, r, partial_, s_);
-  launch_sum_partials(partial_, residual_blocks(L.nc), dnorm_, s_);
-  HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double), hipMemcpyDeviceToHost, s_));
+  HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double) * slots_.size(), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
-  return std::sqrt(hnorm_[0]);
+  out.resize(which.size());
+  for (size_t i = 0; i < which.size(); i++) out[i] = std::sqrt(hnorm_[which[i]]);
 }
 
+bool Solver::check_active(int pat) {
+  (void)pat;
+  std::vector<const void*> u, f;
+  std::vector<void*> r;
+  for (int k : act_) {
+    u.push_back(slots_[k].u64);
+    f.push_back(slots_[k].f64);
+    r.push_back(slots_[k].r64);
+  }
+  std::vector<double> rn;
+  norms(act_, u.data(), f.data(), r.data(), rn);
+  std::vector<int> keep;
+  for (size_t i = 0; i < act_.size(); i++) {
+    Slot& S = slots_[act_[i]];
+    SolveStats* st = S.st;
+    st->history.push_back(rn[i]);
+    if (p_.verb) printf("slot=%d icycle=%d rnorm=%e\n", act_[i], icycle_, rn[i]);
+    if (!std::isfinite(rn[i])) {
+      st->status = 2;
+      st->cycles = icycle_;
+      continue;
+    }
+    st->rnorm = rn[i];
+    if (rn[i] < S.rnorm0 * p_.tol) {
+      st->cycles = icycle_;
+      continue;
+    }
+    keep.push_back(act_[i]);
+  }
+  act_.swap(keep);
+  return act_.empty();
+}
+
+// ---------------------------------------------------------------------------
+// cycles
+// ---------------------------------------------------------------------------
 // restriction of the level's residual (or right-hand side when the guess is
 // zero) followed by the recursive coarse solve and the prolongation; returns
-// true when the level-0 convergence test stopped the solve
+// true when the level-0 convergence test retired every slot
 template <typename T>
 bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess, int kind) {
-  Hier<T>& H = hier<T>();
-  const DevLevel<T>& L = H.pat[pat][lev];
-  const DevLevel<T>& C = H.pat[pat][lev + 1];
-  const void* src;
+  const DevLevel<T>& L = hier<T>().pat[pat][lev];
+  const DevLevel<T>& C = hier<T>().pat[pat][lev + 1];
+  bool from_f = false;
   if (lev == 0 && norm_here) {
-    // level-0 residual with its norm (only in the FP64 direct form)
-    const double rn = norm_of(h64_.pat[pat][0], h64_.u[0], h64_.f[0], h64_.r[0]);
-    if (check(rn)) { stopped_ = true; return true; }
-    src = H.r[lev];
+    // level-0 residual with its norm (FP64 direct form: r64 is the level's r)
+    if (check_active(pat)) return true;
   } else if (zero_guess) {
-    src = H.f[lev];
+    from_f = true;
   } else {
-    launch_residual<T>(L.view(omu_), H.u[lev], H.f[lev], H.r[lev], nullptr, s_);
-    src = H.r[lev];
+    for (int k : act_) launch_residual<T>(L.view(slots_[k].omu), slots_[k].u[lev], slots_[k].f[lev], slots_[k].r[lev], nullptr, s_);
   }
-  launch_restrict<T>(L.view(omu_), C.view(omu_), C.transfer(), src, H.f[lev + 1], s_);
-  launch_zero<T>(H.u[lev + 1], 3 * C.nc, s_);
+  for (int k : act_) {
+    Slot& S = slots_[k];
+    launch_restrict<T>(L.view(S.omu), C.view(S.omu), C.transfer(), from_f ? S.f[lev] : S.r[lev], S.f[lev + 1], s_);
+    launch_zero<T>(S.u[lev + 1], 3 * C.nc, s_);
+  }
   if (kind == 2) f_cycle<T>(pat, lev + 1, false, true);
   else if (kind == 3) { w_cycle<T>(pat, lev + 1, false, true); w_cycle<T>(pat, lev + 1, false, false); }
   else v_cycle<T>(pat, lev + 1, false, true);
-  launch_prolong<T>(L.view(omu_), C.view(omu_), C.transfer(), H.u[lev + 1], H.u[lev], s_);
+  for (int k : act_) {
+    Slot& S = slots_[k];
+    launch_prolong<T>(L.view(S.omu), C.view(S.omu), C.transfer(), S.u[lev + 1], S.u[lev], s_);
+  }
   return false;
 }
 
 // src/gmg.c:1586-1612
 template <typename T>
 bool Solver::v_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
+  if (lev == lmax_ - 1 && lev > 0 && exact_coarsest(pat)) {
+    line_sweep<T>(pat, lev, pat, 0);
+    return false;
+  }
   for (int i = 0; i < p_.v1; i++) { smooth<T>(pat, lev, i); zero_guess = false; }
   if (coarsens<T>(pat, lev)) {
     if (coarse_correction<T>(pat, lev, norm_here && p_.cycleopt == 1, zero_guess, 1)) return true;
@@ -423,9 +514,8 @@ bool Solver::v_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
 // src/gmg.c:1615-1640
 template <typename T>
 bool Solver::f_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
-  Hier<T>& H = hier<T>();
   if (lev == lmax_ - 1) {
-    launch_zero<T>(H.u[lev], 3 * H.pat[pat][lev].nc, s_);
+    for (int k : act_) launch_zero<T>(slots_[k].u[lev], 3 * hier<T>().pat[pat][lev].nc, s_);
     zero_guess = true;
   } else if (coarsens<T>(pat, lev)) {
     if (coarse_correction<T>(pat, lev, norm_here && p_.cycleopt == 2, zero_guess, 2)) return true;
@@ -437,6 +527,10 @@ bool Solver::f_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
 // src/gmg.c:1643-1668
 template <typename T>
 bool Solver::w_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
+  if (lev == lmax_ - 1 && lev > 0 && exact_coarsest(pat)) {
+    line_sweep<T>(pat, lev, pat, 0);
+    return false;
+  }
   for (int i = 0; i < p_.v1; i++) { smooth<T>(pat, lev, i); zero_guess = false; }
   if (coarsens<T>(pat, lev)) {
     if (coarse_correction<T>(pat, lev, norm_here && p_.cycleopt == 3, zero_guess, 3)) return true;
@@ -445,65 +539,86 @@ bool Solver::w_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
   return false;
 }
 
-double Solver::vec_norm(const void* v) {
-  // ||v|| over all entries: the residual of u = 0 is f itself
-  launch_zero<double>(r64_, 3 * nc0_, s_);
-  return norm_of(l0_64_, r64_, v, r64_);
-}
-
+// ---------------------------------------------------------------------------
+// solve (src/gmg.c:1894-1932)
+// ---------------------------------------------------------------------------
 int Solver::solve(const void* b, void* x, void* h, const void* x0, SolveStats* st) {
   SolveStats local;
-  st_ = st ? st : &local;
-  *st_ = SolveStats();
-  stopped_ = false;
-  const Level<double> V0 = l0_64_.view(omu_);
+  return solve_batch(1, &b, &x, &h, &x0, st ? st : &local);
+}
+
+int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const* h, const void* const* x0,
+                        SolveStats* st) {
+  if (n < 1 || n > nslots()) throw std::runtime_error("batch size out of range of the solver's slots");
   const long nv = 3 * nc0_;
-  // f = b * dual volume (src/gmg.c:1899-1910)
-  launch_scale_rhs<double>(V0, l0_64_.D(0), l0_64_.D(1), l0_64_.D(2), b, f64_, s_);
+  act_.clear();
+  for (int k = 0; k < n; k++) {
+    Slot& S = slots_[k];
+    S.st = &st[k];
+    *S.st = SolveStats();
+    S.st->cycles = -1;
+    // f = b * dual volume (src/gmg.c:1899-1910)
+    launch_scale_rhs<double>(l0_64_.view(S.omu), l0_64_.D(0), l0_64_.D(1), l0_64_.D(2), b[k], S.f64, s_);
+    launch_zero<double>(S.u64, nv, s_);
+    act_.push_back(k);
+  }
   // ||f||: the reference's rnorm0 (its first residual, taken at u = 0)
   {
-    void* z = u64_;
-    launch_zero<double>(z, nv, s_);
-    rnorm0_ = norm_of(l0_64_, z, f64_, r64_);
+    std::vector<const void*> u, f;
+    std::vector<void*> r;
+    for (int k : act_) { u.push_back(slots_[k].u64); f.push_back(slots_[k].f64); r.push_back(slots_[k].r64); }
+    std::vector<double> rn;
+    norms(act_, u.data(), f.data(), r.data(), rn);
+    for (size_t i = 0; i < act_.size(); i++) {
+      slots_[act_[i]].rnorm0 = rn[i];
+      slots_[act_[i]].st->rnorm0 = rn[i];
+    }
   }
-  st_->rnorm0 = rnorm0_;
-  if (x0) HIPCHECK(hipMemcpyAsync(u64_, x0, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+  for (int k = 0; k < n; k++)
+    if (x0 && x0[k]) HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
 
   const int kind = p_.cycleopt;
-  for (icycle_ = 0; icycle_ < p_.ncycle; icycle_++) {
+  for (icycle_ = 0; icycle_ < p_.ncycle && !act_.empty(); icycle_++) {
     const int pat = p_.isemicoarsen ? icycle_ % 3 : 0;
     if (p_.precision == 0) {
       // direct form: the level-0 residual and its norm open the cycle
-      bool stop;
-      if (kind == 1) stop = v_cycle<double>(pat, 0, true, false);
-      else if (kind == 3) stop = w_cycle<double>(pat, 0, true, false);
-      else stop = f_cycle<double>(pat, 0, true, false);
-      if (stop || stopped_) break;
+      if (kind == 1) v_cycle<double>(pat, 0, true, false);
+      else if (kind == 3) w_cycle<double>(pat, 0, true, false);
+      else f_cycle<double>(pat, 0, true, false);
+      if (act_.empty()) break;
       if (lmax_ == 1 || !coarsens<double>(pat, 0)) {
         // no coarse level carries the norm test: test after the cycle
-        const double rn = norm_of(l0_64_, u64_, f64_, r64_);
-        if (check(rn)) break;
+        icycle_++;
+        const bool done = check_active(pat);
+        icycle_--;
+        if (done) break;
       }
     } else {
       // refinement: FP64 residual, FP32 cycle on it from zero, FP64 update
-      const double rn = norm_of(l0_64_, u64_, f64_, r64_);
-      if (check(rn)) break;
-      launch_demote(r64_, h32_.f[0], nv, s_);
-      launch_zero<float>(h32_.u[0], nv, s_);
+      if (check_active(pat)) break;
+      for (int k : act_) {
+        launch_demote(slots_[k].r64, slots_[k].f[0], nv, s_);
+        launch_zero<float>(slots_[k].u[0], nv, s_);
+      }
       if (kind == 1) v_cycle<float>(pat, 0, false, true);
       else if (kind == 3) w_cycle<float>(pat, 0, false, true);
       else f_cycle<float>(pat, 0, false, true);
-      launch_promote_add(u64_, h32_.u[0], nv, s_);
+      for (int k : act_) launch_promote_add(slots_[k].u64, slots_[k].u[0], nv, s_);
     }
   }
-  st_->cycles = icycle_ < p_.ncycle ? icycle_ : p_.ncycle;
-  if (st_->status == 0 && !(st_->rnorm < rnorm0_ * p_.tol)) st_->status = 1;
-  if (st_->history.empty()) st_->status = 1;
-  HIPCHECK(hipMemcpyAsync(x, u64_, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
-  launch_curl_h<double>(V0, u64_, h, s_);
+  for (int k = 0; k < n; k++) {
+    Slot& S = slots_[k];
+    SolveStats* s = S.st;
+    if (s->cycles < 0) s->cycles = p_.ncycle;
+    if (s->status == 0 && !(s->rnorm < S.rnorm0 * p_.tol)) s->status = 1;
+    if (s->history.empty()) s->status = 1;
+    HIPCHECK(hipMemcpyAsync(x[k], S.u64, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+    launch_curl_h<double>(l0_64_.view(S.omu), S.u64, h[k], s_);
+  }
   HIPCHECK(hipStreamSynchronize(s_));
   HIPCHECK(hipGetLastError());
-  st_ = nullptr;
+  for (int k = 0; k < n; k++) slots_[k].st = nullptr;
+  act_.clear();
   return 0;
 }
 

@@ -49,6 +49,17 @@ def lib() -> C.CDLL:
         L.emmi_solver_solve.argtypes = [C.c_void_p] * 5 + [C.POINTER(Stats)]
         L.emmi_solver_set_omega.restype = C.c_int
         L.emmi_solver_set_omega.argtypes = [C.c_void_p, C.c_float]
+        L.emmi_solver_create_batch.restype = C.c_void_p
+        L.emmi_solver_create_batch.argtypes = ([C.c_int] * 3 + [C.c_void_p] * 6 +
+                                               [C.c_int, C.c_void_p, C.c_int, C.POINTER(Params)])
+        L.emmi_solver_solve_batch.restype = C.c_int
+        L.emmi_solver_solve_batch.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.POINTER(Stats)]
+        L.emmi_solver_set_omega_slot.restype = C.c_int
+        L.emmi_solver_set_omega_slot.argtypes = [C.c_void_p, C.c_int, C.c_float]
+        L.emmi_solver_history_slot.restype = C.c_int
+        L.emmi_solver_history_slot.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.emmi_solver_slots.restype = C.c_int
+        L.emmi_solver_slots.argtypes = [C.c_void_p]
         from . import carriers as K
         PE, PA, PF = C.POINTER(K.Emf), C.POINTER(K.Acq), C.POINTER(K.Fwi)
         L.regrid_init.argtypes = [PA, PE]
@@ -142,7 +153,8 @@ class DeviceBuffer:
 
 
 class Solver:
-    """A multigrid solver for one (MG grid, medium, frequency) on the GPU."""
+    """A multigrid solver for one (MG grid, medium) on the GPU with one
+    right-hand-side slot per frequency in `omega` (a number or a sequence)."""
 
     def __init__(self, n, x1, x2, x3, sig11, sig22, sig33, omega, sig_on_device=False, **kw):
         self.n = tuple(int(v) for v in n)
@@ -154,10 +166,14 @@ class Solver:
             sig = [np.ascontiguousarray(a, np.float32) for a in (sig11, sig22, sig33)]
             self._keep += sig
             ptrs = [a.ctypes.data for a in sig]
-        self.h = lib().emmi_solver_create(self.n[0], self.n[1], self.n[2],
-                                          self._keep[0].ctypes.data, self._keep[1].ctypes.data,
-                                          self._keep[2].ctypes.data, ptrs[0], ptrs[1], ptrs[2],
-                                          1 if sig_on_device else 0, np.float32(omega), C.byref(self.p))
+        om = np.atleast_1d(np.asarray(omega, np.float32)).copy()
+        self._keep.append(om)
+        self.nslots = int(om.size)
+        self.h = lib().emmi_solver_create_batch(self.n[0], self.n[1], self.n[2],
+                                                self._keep[0].ctypes.data, self._keep[1].ctypes.data,
+                                                self._keep[2].ctypes.data, ptrs[0], ptrs[1], ptrs[2],
+                                                1 if sig_on_device else 0, om.ctypes.data, self.nslots,
+                                                C.byref(self.p))
         if not self.h:
             raise NativeError(last_error())
 
@@ -182,12 +198,24 @@ class Solver:
         st = self.solve_device(db.ptr, dx.ptr, db.ptr, d0.ptr if d0 else None)
         return dx.download(np.complex128), db.download(np.complex128), st
 
-    def set_omega(self, omega):
-        lib().emmi_solver_set_omega(self.h, np.float32(omega))
+    def solve_batch_device(self, b_ptrs, x_ptrs, h_ptrs, x0_ptrs=None):
+        """Solves slots 0..len(b_ptrs)-1 together; returns one Stats per slot."""
+        n = len(b_ptrs)
+        P = C.c_void_p * n
+        st = (Stats * n)()
+        x0 = P(*x0_ptrs) if x0_ptrs is not None else None
+        rc = lib().emmi_solver_solve_batch(self.h, n, P(*b_ptrs), P(*x_ptrs), P(*h_ptrs), x0, st)
+        if rc < 0:
+            raise NativeError(last_error())
+        return list(st)
 
-    def history(self) -> np.ndarray:
+    def set_omega(self, omega, slot=0):
+        if lib().emmi_solver_set_omega_slot(self.h, int(slot), np.float32(omega)):
+            raise NativeError(last_error())
+
+    def history(self, slot=0) -> np.ndarray:
         buf = (C.c_double * 4096)()
-        n = lib().emmi_solver_history(self.h, buf, 4096)
+        n = lib().emmi_solver_history_slot(self.h, int(slot), buf, 4096)
         return np.array(buf[:min(n, 4096)])
 
     def levels(self):
