@@ -210,6 +210,12 @@ class Level:
         lib().emo_line_gs(self.h, omu, axis, it, colour, u, _c(f))
         return u
 
+    def line_solve(self, omu, axis, pb, pc, u, f):
+        """Exact solve of the single line (axis; pb, pc) for the current residual."""
+        u = _c(u).copy()
+        lib().emo_line_solve(self.h, omu, axis, pb, pc, u, _c(f))
+        return u
+
     def point_gs(self, omu, it, u, f, colour=0):
         u = _c(u).copy()
         lib().emo_point_gs(self.h, omu, it, colour, u, _c(f))
@@ -311,3 +317,32 @@ def extract_data(mg: MG, receivers, chrec, E, H):
     out = np.zeros((len(ch), len(x)), np.complex64)
     lib().emo_extract_data(C.byref(mg), len(x), x, y, z, az, dip, len(ch), ch, _c(E), _c(H), out)
     return out
+
+
+def extract_efield(mg: MG, s, x):
+    """E on the input-grid cell centres of survey s: (E1, E2, E3) complex64 [nz][ny][nx]."""
+    shp = (s.nz, s.ny, s.nx)
+    E = [np.zeros(shp, np.complex64) for _ in range(3)]
+    rho33 = np.ascontiguousarray(s.rho33, np.float32)
+    lib().emo_extract_efield(C.byref(mg), s.nx, s.ny, s.nz, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, rho33, _c(x), *E)
+    return E
+
+
+def extract_hfield(mg: MG, s, h):
+    shp = (s.nz, s.ny, s.nx)
+    H = [np.zeros(shp, np.complex64) for _ in range(3)]
+    lib().emo_extract_hfield(C.byref(mg), s.nx, s.ny, s.nz, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, _c(h), *H)
+    return H
+
+
+def build_gradient(spec: GradSpec, nfreq, E, sources, receivers):
+    """grad, phess [npar][nz][ny][nx] from the forward (E?f) and adjoint (E?a)
+    fields, complex64 [nfreq][nz][ny][nx] each, in dict E."""
+    n = spec.nx * spec.ny * spec.nz
+    grad = np.zeros(spec.npar * n, np.float32)
+    phess = np.zeros(spec.npar * n, np.float32)
+    sx, sy, sz, _, _ = _acq_arrays(sources)
+    rx, ry, rz, _, _ = _acq_arrays(receivers)
+    f = [np.ascontiguousarray(E[k], np.complex64).reshape(-1) for k in ("E1f", "E2f", "E3f", "E1a", "E2a", "E3a")]
+    lib().emo_build_gradient(C.byref(spec), nfreq, *f, len(sx), sx, sy, sz, len(rx), rx, ry, rz, grad, phess)
+    return grad.reshape(spec.npar, spec.nz, spec.ny, spec.nx), phess.reshape(spec.npar, spec.nz, spec.ny, spec.nx)

@@ -14,7 +14,8 @@ synthetic surveys of libemmi_mgfd_amd.configs and keeps only small outputs:
   config1.npz      grid arrays, medium hashes, receiver data and residual
                    histories of the 64^3 survey (default and tol=1e-10)
   config2.npz      (with --config2) the 128^3 three-frequency survey: grid,
-                   medium hashes, receiver data and residual histories
+                   medium hashes, receiver data and residual histories at the
+                   default tolerance and at tol=1e-10
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -181,6 +182,10 @@ def config2(tmp):
         out[k + "_sha256"] = np.array(sha(g[k]))
     out["dcal"] = ld(d, "dcal", np.complex64)
     out["hist"] = pad_hist(hist)
+    # near-exact discrete solution: the same solve to tol=1e-10
+    d2, hist2, _ = run(s, "model", os.path.join(tmp, "c2_tight"), ["tol=1e-10", "ncycle=200"])
+    out["dcal_tight"] = ld(d2, "dcal", np.complex64)
+    out["hist_tight"] = pad_hist(hist2)
     np.savez_compressed(os.path.join(HERE, "config2.npz"), **out)
 
 
