@@ -17,14 +17,13 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemmi_mgfd.so")
 
-HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "batch.hip", "path_kernels.hip"]
-HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp", "batch_api.cpp"]
+HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "path_kernels.hip"]
+HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp"]
 
 ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-lambda-capture"]
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def hipcc() -> str:
@@ -47,13 +46,9 @@ def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = []
     for s in HIP_SOURCES:
-        p = os.path.join(CSRC, s)
-        if os.path.exists(p):
-            jobs.append((p, os.path.join(BUILD, s + ".o"), COMMON + DEVICE + ["-x", "hip"]))
+        jobs.append((os.path.join(CSRC, s), os.path.join(BUILD, s + ".o"), COMMON + DEVICE + ["-x", "hip"]))
     for s in HOST_SOURCES:
-        p = os.path.join(CSRC, s)
-        if os.path.exists(p):
-            jobs.append((p, os.path.join(BUILD, s + ".o"), COMMON + ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"]))
+        jobs.append((os.path.join(CSRC, s), os.path.join(BUILD, s + ".o"), COMMON + ["-ffp-contract=off"]))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
