@@ -187,23 +187,75 @@ __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
   }
 }
 
-// z = W^{-1} y with the packed symmetric inverse of node record r
-__device__ __forceinline__ void symv(const double* r, long stride, const cxd (&y)[4], cxd (&z)[4]) {
-  cxd w[10];
-#pragma unroll
-  for (int s = 0; s < 10; s++) w[s] = cxd(r[(2 * s) * stride], r[(2 * s + 1) * stride]);
+// z = W^{-1} y with the packed symmetric inverse held in w[0..19]
+__device__ __forceinline__ void symv(const double (&w)[FREC], const cxd (&y)[4], cxd (&z)[4]) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     cxd acc(0, 0);
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc += w[symidx(i, j)] * y[j];
+    for (int j = 0; j < 4; j++) {
+      const int s = symidx(i, j);
+      acc += cxd(w[2 * s], w[2 * s + 1]) * y[j];
+    }
     z[i] = acc;
   }
 }
 
+// Right-hand sides of the lines of one colour: one thread per (node, line),
+// lines fastest, so the records [p][SREC][nlines] are written contiguously.
+//   record p, slots 0..7: sigma_p, the residual of the four perpendicular
+//                          edges of node p (p >= 1)
+//   record p, slots 8..9: rho_p, the residual of the along-line edge p
+// Lines of one colour are not coupled, so these are the residuals each line
+// solve of the colour starts from.
+template <typename T, int A>
+__global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
+  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const long nl = (long)J.nlo * J.nhi;
+  const int N = L.n[A];
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nl * N) return;
+  const int line = (int)(t % nl);
+  const int p = (int)(t / nl);
+  int pb, pc;
+  line_of<T, A>(J, line, pb, pc);
+  const typename cvec<T>::type* __restrict__ u = J.u;
+  const typename cvec<T>::type* __restrict__ f = J.f;
+  const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
+  double* rec = J.scr + line + (long)p * SREC * nl;
+  {
+    int q[3];
+    q[A] = p; q[B] = pb; q[C] = pc;
+    const cxd rho = ld<T>(f, A * L.nc + base) - apply_row<T, A>(L, u, q[A], q[B], q[C], base);
+    rec[8 * nl] = rho.re;
+    rec[9 * nl] = rho.im;
+  }
+  if (p >= 1) {
+    int q[3];
+    cxd g;
+    const long ob0 = base - L.es[B], oc0 = base - L.es[C];
+    q[A] = p; q[B] = pb - 1; q[C] = pc;
+    g = ld<T>(f, B * L.nc + ob0) - apply_row<T, B>(L, u, q[B], q[C], q[A], ob0);
+    rec[0 * nl] = g.re; rec[1 * nl] = g.im;
+    q[B] = pb;
+    g = ld<T>(f, B * L.nc + base) - apply_row<T, B>(L, u, q[B], q[C], q[A], base);
+    rec[2 * nl] = g.re; rec[3 * nl] = g.im;
+    q[C] = pc - 1;
+    g = ld<T>(f, C * L.nc + oc0) - apply_row<T, C>(L, u, q[C], q[A], q[B], oc0);
+    rec[4 * nl] = g.re; rec[5 * nl] = g.im;
+    q[C] = pc;
+    g = ld<T>(f, C * L.nc + base) - apply_row<T, C>(L, u, q[C], q[A], q[B], base);
+    rec[6 * nl] = g.re; rec[7 * nl] = g.im;
+  }
+}
+
+// Block substitution of the lines of one colour, one thread per line.  The
+// records of node p+1 (forward) or p-1 (backward) are loaded before node p is
+// processed, so their memory latency overlaps the arithmetic of node p.
 template <typename T, int A>
 __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
-  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -211,44 +263,36 @@ __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
   if (!line_of<T, A>(J, tid, pb, pc)) return;
   const LineGeo<T, A> G(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
-  const double* fac = J.fac + tid;
-  double* scr = J.scr + tid;
+  const double* __restrict__ fac = J.fac + tid;
+  double* __restrict__ scr = J.scr + tid;
   typename cvec<T>::type* __restrict__ u = J.u;
-  const typename cvec<T>::type* __restrict__ f = J.f;
   const int N = G.N;
 
-  auto res_a = [&](int e) -> cxd {
-    int q[3];
-    q[A] = e; q[B] = pb; q[C] = pc;
-    return ld<T>(f, G.oa + e * G.sa) - apply_row<T, A>(L, u, q[A], q[B], q[C], G.base + e * G.sa);
-  };
-
-  // ---- forward substitution ----
+  // ---- forward substitution: z_p = W_p^-1 (g_p - B_p z_{p-1}) ----
   cxd rd_prev(fac[20 * stride], fac[21 * stride]);
-  cxd q_prev = res_a(0) * rd_prev;
+  cxd q_prev = cxd(scr[8 * stride], scr[9 * stride]) * rd_prev;
   scr[8 * stride] = q_prev.re;
   scr[9 * stride] = q_prev.im;
+  double cf[FREC], cr[SREC];
+#pragma unroll
+  for (int k = 0; k < FREC; k++) cf[k] = fac[((long)1 * FREC + k) * stride];
+#pragma unroll
+  for (int k = 0; k < SREC; k++) cr[k] = scr[((long)1 * SREC + k) * stride];
   cxd z[4];
   for (int p = 1; p <= N - 1; p++) {
-    const double* r = fac + (long)p * FREC * stride;
-    const cxd rd(r[20 * stride], r[21 * stride]);
-    const cxd q = res_a(p) * rd;
-    cxd g[4];
-    {
-      const long off = p * G.sa;
-      int qq[3];
-      qq[A] = p; qq[B] = pb - 1; qq[C] = pc;
-      g[0] = ld<T>(f, G.ob0 + off) - apply_row<T, B>(L, u, qq[B], qq[C], qq[A], G.ob0 + off - B * L.nc);
-      qq[B] = pb;
-      g[1] = ld<T>(f, G.ob1 + off) - apply_row<T, B>(L, u, qq[B], qq[C], qq[A], G.ob1 + off - B * L.nc);
-      qq[C] = pc - 1;
-      g[2] = ld<T>(f, G.oc0 + off) - apply_row<T, C>(L, u, qq[C], qq[A], qq[B], G.oc0 + off - C * L.nc);
-      qq[C] = pc;
-      g[3] = ld<T>(f, G.oc1 + off) - apply_row<T, C>(L, u, qq[C], qq[A], qq[B], G.oc1 + off - C * L.nc);
-    }
-    const cxd dq = q_prev - q;
+    double nf[FREC], nr[SREC];
+    if (p + 1 <= N - 1) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) g[i] += dq * G.v[i];
+      for (int k = 0; k < FREC; k++) nf[k] = fac[((long)(p + 1) * FREC + k) * stride];
+#pragma unroll
+      for (int k = 0; k < SREC; k++) nr[k] = scr[((long)(p + 1) * SREC + k) * stride];
+    }
+    const cxd rd(cf[20], cf[21]);
+    const cxd q = cxd(cr[8], cr[9]) * rd;
+    const cxd dq = q_prev - q;
+    cxd g[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) g[i] = cxd(cr[2 * i], cr[2 * i + 1]) + dq * G.v[i];
     if (p >= 2) {
       double l[4];
       G.lcoef(L, p - 1, l);
@@ -256,7 +300,7 @@ __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
 #pragma unroll
       for (int i = 0; i < 4; i++) g[i] -= z[i] * l[i] + (vz * rd_prev) * G.v[i];
     }
-    symv(r, stride, g, z);
+    symv(cf, g, z);
     double* s = scr + (long)p * SREC * stride;
 #pragma unroll
     for (int i = 0; i < 4; i++) { s[(2 * i) * stride] = z[i].re; s[(2 * i + 1) * stride] = z[i].im; }
@@ -264,20 +308,40 @@ __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
     s[9 * stride] = q.im;
     rd_prev = rd;
     q_prev = q;
+#pragma unroll
+    for (int k = 0; k < FREC; k++) cf[k] = nf[k];
+#pragma unroll
+    for (int k = 0; k < SREC; k++) cr[k] = nr[k];
   }
 
-  // ---- back substitution and update ----
+  // ---- back substitution: da_p = z_p - W_p^-1 B_{p+1} da_{p+1}; update ----
   cxd an[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) an[i] = cxd(0, 0);
+  const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
+  cxd cu[5];
+#pragma unroll
+  for (int k = 0; k < FREC; k++) cf[k] = fac[((long)(N - 1) * FREC + k) * stride];
+#pragma unroll
+  for (int k = 0; k < SREC; k++) cr[k] = scr[((long)(N - 1) * SREC + k) * stride];
+#pragma unroll
+  for (int k = 0; k < 5; k++) cu[k] = ld<T>(u, off_u[k] + (long)(N - 1) * G.sa);
   for (int p = N - 1; p >= 1; p--) {
-    const double* r = fac + (long)p * FREC * stride;
-    const double* s = scr + (long)p * SREC * stride;
-    const cxd rd(r[20 * stride], r[21 * stride]);
-    const cxd q(s[8 * stride], s[9 * stride]);
+    double nf[FREC], nr[SREC];
+    cxd nu[5];
+    if (p - 1 >= 1) {
+#pragma unroll
+      for (int k = 0; k < FREC; k++) nf[k] = fac[((long)(p - 1) * FREC + k) * stride];
+#pragma unroll
+      for (int k = 0; k < SREC; k++) nr[k] = scr[((long)(p - 1) * SREC + k) * stride];
+#pragma unroll
+      for (int k = 0; k < 5; k++) nu[k] = ld<T>(u, off_u[k] + (long)(p - 1) * G.sa);
+    }
+    const cxd rd(cf[20], cf[21]);
+    const cxd q(cr[8], cr[9]);
     cxd ap[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) ap[i] = cxd(s[(2 * i) * stride], s[(2 * i + 1) * stride]);
+    for (int i = 0; i < 4; i++) ap[i] = cxd(cr[2 * i], cr[2 * i + 1]);
     if (p < N - 1) {
       double l[4];
       G.lcoef(L, p, l);
@@ -285,29 +349,28 @@ __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
       cxd t[4], wt[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) t[i] = an[i] * l[i] + (va * rd) * G.v[i];
-      symv(r, stride, t, wt);
+      symv(cf, t, wt);
 #pragma unroll
       for (int i = 0; i < 4; i++) ap[i] -= wt[i];
     }
-    {
-      cxd dv(0, 0);
+    cxd dv(0, 0);
 #pragma unroll
-      for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
-      const cxd de = q + dv * rd;
-      const long o = G.oa + p * G.sa;
-      st<T>(u, o, ld<T>(u, o) + de);
-    }
-    {
-      const long off = p * G.sa;
-      st<T>(u, G.ob0 + off, ld<T>(u, G.ob0 + off) + ap[0]);
-      st<T>(u, G.ob1 + off, ld<T>(u, G.ob1 + off) + ap[1]);
-      st<T>(u, G.oc0 + off, ld<T>(u, G.oc0 + off) + ap[2]);
-      st<T>(u, G.oc1 + off, ld<T>(u, G.oc1 + off) + ap[3]);
-    }
+    for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
+    const long o = (long)p * G.sa;
+    st<T>(u, G.oa + o, cu[0] + (q + dv * rd));
+#pragma unroll
+    for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cu[i + 1] + ap[i]);
 #pragma unroll
     for (int i = 0; i < 4; i++) an[i] = ap[i];
+#pragma unroll
+    for (int k = 0; k < FREC; k++) cf[k] = nf[k];
+#pragma unroll
+    for (int k = 0; k < SREC; k++) cr[k] = nr[k];
+#pragma unroll
+    for (int k = 0; k < 5; k++) cu[k] = nu[k];
   }
   {
+    // along-line edge 0, between the boundary node and node 1
     const cxd q0(scr[8 * stride], scr[9 * stride]);
     const cxd rd0(fac[20 * stride], fac[21 * stride]);
     cxd dv(0, 0);
@@ -343,24 +406,40 @@ size_t line_scratch_doubles(const int* n, int a, int colour) {
 template <typename T>
 static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
   int maxl = 0;
-  double bytes = 0;
+  long maxnodes = 0;
+  double rhs_bytes = 0, solve_bytes = 0;
+  const double es = sizeof(typename cvec<T>::type), cs = sizeof(T);
   for (int j = 0; j < P.njobs; j++) {
-    const int nl = P.job[j].nlo * P.job[j].nhi;
-    if (nl > maxl) maxl = nl;
-    // compulsory traffic of one colour: a quarter of a full sweep, which
-    // reads u and f and writes u once (3 Ne s) and reads the medium (Ne c)
-    const double ne = 3.0 * (double)P.job[j].L.nc;
-    bytes += 0.25 * ne * (3.0 * sizeof(typename cvec<T>::type) + sizeof(T));
+    const long nl = (long)P.job[j].nlo * P.job[j].nhi;
+    const int na = P.job[j].L.n[axis];
+    if (nl > maxl) maxl = (int)nl;
+    if (nl * na > maxnodes) maxnodes = nl * na;
+    const double rows = (double)nl * (5.0 * na - 4.0);
+    // right-hand sides: the colour's rows read f and sigma-bar once, u is
+    // read once over the level, the records are written once
+    rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * 8.0;
+    // substitution: records and factors read once, the rows of u read and written
+    solve_bytes += (double)nl * na * (SREC + FREC) * 8.0 + rows * 2.0 * es;
   }
   if (maxl == 0 || P.njobs == 0) return;
-  const dim3 g((maxl + 63) / 64, P.njobs), b(64);
   if (factor) {
+    const dim3 g((maxl + 63) / 64, P.njobs), b(64);
     timing::Scope ts(timing::OTHER, s, 0);
     if (axis == 0) hipLaunchKernelGGL((k_line_factor<T, 0>), g, b, 0, s, P);
     else if (axis == 1) hipLaunchKernelGGL((k_line_factor<T, 1>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_line_factor<T, 2>), g, b, 0, s, P);
-  } else {
-    timing::Scope ts(timing::LINE, s, bytes);
+    return;
+  }
+  {
+    const dim3 g((unsigned)((maxnodes + 255) / 256), P.njobs), b(256);
+    timing::Scope ts(timing::LINE_RHS, s, rhs_bytes);
+    if (axis == 0) hipLaunchKernelGGL((k_line_rhs<T, 0>), g, b, 0, s, P);
+    else if (axis == 1) hipLaunchKernelGGL((k_line_rhs<T, 1>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((k_line_rhs<T, 2>), g, b, 0, s, P);
+  }
+  {
+    const dim3 g((maxl + 63) / 64, P.njobs), b(64);
+    timing::Scope ts(timing::LINE, s, solve_bytes);
     if (axis == 0) hipLaunchKernelGGL((k_line_solve<T, 0>), g, b, 0, s, P);
     else if (axis == 1) hipLaunchKernelGGL((k_line_solve<T, 1>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_line_solve<T, 2>), g, b, 0, s, P);

@@ -7,7 +7,7 @@
 namespace emmi {
 namespace timing {
 
-enum Class { RESIDUAL = 0, LINE = 1, RESTRICT = 2, PROLONG = 3, OTHER = 4, NCLASS = 5 };
+enum Class { RESIDUAL = 0, LINE = 1, RESTRICT = 2, PROLONG = 3, OTHER = 4, LINE_RHS = 5, NCLASS = 6 };
 
 void enable(bool on);
 bool enabled();

@@ -265,7 +265,7 @@ def timing_enable(on=True):
     lib().emmi_timing_enable(1 if on else 0)
 
 
-TIMING_CLASSES = ["residual", "line_sweep", "restriction", "prolongation", "other"]
+TIMING_CLASSES = ["residual", "line_solve", "restriction", "prolongation", "other", "line_rhs"]
 
 
 def timing_read():
