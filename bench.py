@@ -112,22 +112,27 @@ VARIANTS = {"plain": (0, False), "mixed": (1, False), "batched": (0, True), "mix
 
 
 def time_variant(survey, variant, steps, warmup, timing=False):
+    """Warmup, then `steps` timed steps (cycle bodies replayed as HIP graphs);
+    with `timing`, one more step runs eagerly with per-launch HIP events for
+    the kernel-level figures (the same kernels with the same arguments)."""
     from libemmi_mgfd_amd import native
     precision, batched = VARIANTS[variant]
     run = ShotRun(survey, precision, batched)
     for _ in range(warmup):
         run.solve_all()
     native.lib().emmi_device_sync()
-    if timing:
-        native.timing_enable(True)
     t0 = time.perf_counter()
     stats = []
     for _ in range(steps):
         stats += run.solve_all()
     native.lib().emmi_device_sync()
     dt = time.perf_counter() - t0
-    kt = native.timing_read() if timing else None
+    kt = None
     if timing:
+        native.timing_enable(True)
+        run.solve_all()
+        native.lib().emmi_device_sync()
+        kt = native.timing_read()
         native.timing_enable(False)
     nsolve = steps * len(survey.freqs)
     return {"solves_per_s": nsolve / dt, "ms_per_step": 1e3 * dt / steps, "setup_s": run.setup_s,

@@ -141,6 +141,7 @@ typedef struct {
   int precision;      /* 0 complex FP64; 1 complex FP32 cycles in FP64 refinement */
   int coarse_sweeps;  /* line sweeps per coarsest-level smoothing, default 2 */
   int verb;
+  int graphs;         /* 1 (default): replay each cycle as a captured HIP graph */
 } emmi_params_t;
 
 typedef struct {

@@ -50,6 +50,7 @@ static SolveParams to_params(const emmi_params_t* p) {
     q.precision = p->precision;
     q.coarse_sweeps = p->coarse_sweeps;
     q.verb = p->verb;
+    q.graphs = p->graphs;
   }
   return q;
 }
@@ -81,6 +82,7 @@ void emmi_default_params(emmi_params_t* p) {
   p->precision = q.precision;
   p->coarse_sweeps = q.coarse_sweeps;
   p->verb = q.verb;
+  p->graphs = q.graphs;
 }
 
 const char* emmi_last_error(void) { return g_err.c_str(); }
@@ -330,6 +332,7 @@ extern "C" void gmg_init(emf_t* emf, int ifreq) {
   g_legacy.p.precision = par_int("precision", 0);
   g_legacy.p.coarse_sweeps = par_int("coarse_sweeps", 2);
   g_legacy.p.verb = emf->verb;
+  g_legacy.p.graphs = par_int("graphs", 1);
   g_legacy.init = true;
 }
 

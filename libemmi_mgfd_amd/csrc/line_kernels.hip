@@ -23,6 +23,8 @@
 // Lines of one colour (pb%2, pc%2) are not coupled, so a launch sweeps one
 // colour; every kernel computes in FP64 whatever the storage precision.
 #include "line_kernels.h"
+
+#include <algorithm>
 #include "stencil.h"
 #include "timing.h"
 
@@ -251,6 +253,76 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
   }
 }
 
+// Right-hand sides for lines along x (axis 0): consecutive lines lie in
+// different x rows, so a thread per (node, line) with lines fastest would
+// load one cache line per lane.  A block takes a tile of 16 nodes x 16 lines,
+// computes with nodes fastest (loads contiguous along x) and transposes the
+// records through LDS so the stores run along the lines.
+constexpr int XT = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_line_rhs_x(LinePack<T> P, int nlb) {
+  constexpr int A = 0, B = 1, C = 2;
+  __shared__ double tile[SREC][XT][XT + 1];
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const long nl = (long)J.nlo * J.nhi;
+  const int N = L.n[A];
+  const int lb = blockIdx.x % nlb, pbk = blockIdx.x / nlb;
+  const long l0 = (long)lb * XT;
+  const int p0 = pbk * XT;
+  if (l0 >= nl || p0 >= N) return;   // uniform over the block
+  {
+    const int pl = threadIdx.x % XT, ll = threadIdx.x / XT;
+    const int p = p0 + pl;
+    const long line = l0 + ll;
+    double v[SREC];
+#pragma unroll
+    for (int k = 0; k < SREC; k++) v[k] = 0.0;
+    if (p < N && line < nl) {
+      int pb, pc;
+      line_of<T, A>(J, (int)line, pb, pc);
+      const typename cvec<T>::type* __restrict__ u = J.u;
+      const typename cvec<T>::type* __restrict__ f = J.f;
+      const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
+      int q[3];
+      q[A] = p; q[B] = pb; q[C] = pc;
+      const cxd rho = ld<T>(f, A * L.nc + base) - apply_row<T, A>(L, u, q[A], q[B], q[C], base);
+      v[8] = rho.re;
+      v[9] = rho.im;
+      if (p >= 1) {
+        cxd g;
+        const long ob0 = base - L.es[B], oc0 = base - L.es[C];
+        q[B] = pb - 1;
+        g = ld<T>(f, B * L.nc + ob0) - apply_row<T, B>(L, u, q[B], q[C], q[A], ob0);
+        v[0] = g.re; v[1] = g.im;
+        q[B] = pb;
+        g = ld<T>(f, B * L.nc + base) - apply_row<T, B>(L, u, q[B], q[C], q[A], base);
+        v[2] = g.re; v[3] = g.im;
+        q[C] = pc - 1;
+        g = ld<T>(f, C * L.nc + oc0) - apply_row<T, C>(L, u, q[C], q[A], q[B], oc0);
+        v[4] = g.re; v[5] = g.im;
+        q[C] = pc;
+        g = ld<T>(f, C * L.nc + base) - apply_row<T, C>(L, u, q[C], q[A], q[B], base);
+        v[6] = g.re; v[7] = g.im;
+      }
+#pragma unroll
+      for (int k = 0; k < SREC; k++) tile[k][pl][ll] = v[k];
+    }
+  }
+  __syncthreads();
+  {
+    const int ll = threadIdx.x % XT, pl = threadIdx.x / XT;
+    const int p = p0 + pl;
+    const long line = l0 + ll;
+    if (p < N && line < nl) {
+      double* rec = J.scr + line + (long)p * SREC * nl;
+#pragma unroll
+      for (int k = 0; k < SREC; k++) rec[k * nl] = tile[k][pl][ll];
+    }
+  }
+}
+
 // Block substitution of the lines of one colour, one thread per line.  The
 // records of node p+1 (forward) or p-1 (backward) are loaded before node p is
 // processed, so their memory latency overlaps the arithmetic of node p.
@@ -430,11 +502,17 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     else hipLaunchKernelGGL((k_line_factor<T, 2>), g, b, 0, s, P);
     return;
   }
-  {
+  if (axis == 0) {
+    int maxn = 0;
+    for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
+    const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
+    const dim3 g((unsigned)(nlb * npb), P.njobs), b(XT * XT);
+    timing::Scope ts(timing::LINE_RHS, s, rhs_bytes);
+    hipLaunchKernelGGL((k_line_rhs_x<T>), g, b, 0, s, P, nlb);
+  } else {
     const dim3 g((unsigned)((maxnodes + 255) / 256), P.njobs), b(256);
     timing::Scope ts(timing::LINE_RHS, s, rhs_bytes);
-    if (axis == 0) hipLaunchKernelGGL((k_line_rhs<T, 0>), g, b, 0, s, P);
-    else if (axis == 1) hipLaunchKernelGGL((k_line_rhs<T, 1>), g, b, 0, s, P);
+    if (axis == 1) hipLaunchKernelGGL((k_line_rhs<T, 1>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_line_rhs<T, 2>), g, b, 0, s, P);
   }
   {

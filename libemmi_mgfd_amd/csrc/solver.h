@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 #include <array>
+#include <map>
 #include <vector>
 
 #include "emmi_dev.h"
@@ -27,6 +28,7 @@ struct SolveParams {
   int precision = 0;     // 0: complex FP64 throughout; 1: FP32 cycles inside FP64 refinement
   int coarse_sweeps = 2; // line sweeps per smoothing call on the coarsest level
   int verb = 0;
+  int graphs = 1;        // replay cycle bodies as captured HIP graphs
 };
 
 struct SolveStats {
@@ -83,6 +85,7 @@ struct Slot {
   double* partial = nullptr;         // residual-norm partial sums
   double rnorm0 = 0;
   SolveStats* st = nullptr;
+  double seen_omu[3] = {-1.0, -1.0, -1.0};  // frequency whose factors a cycle of pattern p has built
 };
 
 class Solver {
@@ -140,6 +143,10 @@ class Solver {
   // level-0 FP64 residual norms of the active slots (r64 <- f64 - A u64);
   // retires the converged ones and returns true when none is left
   bool check_active(int pat);
+  // one cycle after its convergence test, as a replayed graph when possible
+  void cycle_body(int pat);
+  void cycle_body_eager(int pat);
+  void drop_graphs();
   void norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,
              std::vector<double>& out);
 
@@ -161,6 +168,11 @@ class Solver {
   // state of the running batch
   std::vector<int> act_;      // active slots
   int icycle_ = 0;
+  bool r0_ready_ = false;     // r[0] holds f - A u of the level-0 test (FP64 form)
+  // captured cycle bodies, keyed by pattern and active-slot mask; valid for
+  // the slots' frequencies at capture time
+  std::map<std::pair<int, unsigned>, hipGraphExec_t> graphs_;
+  std::vector<double> graph_omu_;
 };
 
 }  // namespace emmi

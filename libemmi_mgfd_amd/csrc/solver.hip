@@ -9,6 +9,7 @@
 // level the lines run along the axis that is never coarsened, which makes the
 // coarsest solve exact when a single line remains.
 #include "solver.h"
+#include "timing.h"
 
 #include <algorithm>
 #include <cmath>
@@ -227,6 +228,7 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
 
 Solver::~Solver() {
   hipStreamSynchronize(s_);
+  drop_graphs();
   for (void* p : owned_) hipFree(p);
   if (hnorm_) hipHostFree(hnorm_);
 }
@@ -476,6 +478,9 @@ bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess
   if (lev == 0 && norm_here) {
     // level-0 residual with its norm (FP64 direct form: r64 is the level's r)
     if (check_active(pat)) return true;
+  } else if (lev == 0 && r0_ready_) {
+    // the cycle's convergence test has just computed r[0] = f - A u
+    r0_ready_ = false;
   } else if (zero_guess) {
     from_f = true;
   } else {
@@ -540,6 +545,77 @@ bool Solver::w_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
 }
 
 // ---------------------------------------------------------------------------
+// cycle bodies and their graphs
+// ---------------------------------------------------------------------------
+void Solver::cycle_body_eager(int pat) {
+  const int kind = p_.cycleopt;
+  const long nv = 3 * nc0_;
+  if (p_.precision == 0) {
+    // F-cycle after its opening test, which left r[0] = f - A u
+    r0_ready_ = true;
+    f_cycle<double>(pat, 0, false, false);
+    r0_ready_ = false;
+    return;
+  }
+  // refinement: the test left r64 = f - A u64; FP32-storage cycle on it from
+  // zero, FP64 update
+  for (int k : act_) {
+    launch_demote(slots_[k].r64, slots_[k].f[0], nv, s_);
+    launch_zero<float>(slots_[k].u[0], nv, s_);
+  }
+  if (kind == 1) v_cycle<float>(pat, 0, false, true);
+  else if (kind == 3) w_cycle<float>(pat, 0, false, true);
+  else f_cycle<float>(pat, 0, false, true);
+  for (int k : act_) launch_promote_add(slots_[k].u64, slots_[k].u[0], nv, s_);
+}
+
+void Solver::drop_graphs() {
+  for (auto& g : graphs_) hipGraphExecDestroy(g.second);
+  graphs_.clear();
+}
+
+// The sequence of launches of a cycle body depends only on the pattern, the
+// active slots and their frequencies (kernel arguments carry omega mu0), so
+// the body is captured once per (pattern, active set) and replayed.  The
+// first cycle of a pattern at a new frequency runs eagerly: it builds the
+// line factors, which must not be captured.
+void Solver::cycle_body(int pat) {
+  if (!p_.graphs || timing::enabled() || act_.size() > 32 || slots_.size() > 32) {
+    cycle_body_eager(pat);
+    return;
+  }
+  bool fresh = false;
+  for (int k : act_)
+    if (slots_[k].seen_omu[pat] != slots_[k].omu) fresh = true;
+  if (fresh) {
+    cycle_body_eager(pat);
+    for (int k : act_) slots_[k].seen_omu[pat] = slots_[k].omu;
+    return;
+  }
+  std::vector<double> om(slots_.size());
+  for (size_t k = 0; k < slots_.size(); k++) om[k] = slots_[k].omu;
+  if (om != graph_omu_) {
+    drop_graphs();
+    graph_omu_ = om;
+  }
+  unsigned mask = 0;
+  for (int k : act_) mask |= 1u << k;
+  const auto key = std::make_pair(pat, mask);
+  auto it = graphs_.find(key);
+  if (it == graphs_.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    cycle_body_eager(pat);
+    HIPCHECK(hipStreamEndCapture(s_, &g));
+    HIPCHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(g));
+    it = graphs_.emplace(key, ge).first;
+  }
+  HIPCHECK(hipGraphLaunch(it->second, s_));
+}
+
+// ---------------------------------------------------------------------------
 // solve (src/gmg.c:1894-1932)
 // ---------------------------------------------------------------------------
 int Solver::solve(const void* b, void* x, void* h, const void* x0, SolveStats* st) {
@@ -580,30 +656,25 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
   const int kind = p_.cycleopt;
   for (icycle_ = 0; icycle_ < p_.ncycle && !act_.empty(); icycle_++) {
     const int pat = p_.isemicoarsen ? icycle_ % 3 : 0;
-    if (p_.precision == 0) {
-      // direct form: the level-0 residual and its norm open the cycle
-      if (kind == 1) v_cycle<double>(pat, 0, true, false);
-      else if (kind == 3) w_cycle<double>(pat, 0, true, false);
-      else f_cycle<double>(pat, 0, true, false);
-      if (act_.empty()) break;
-      if (lmax_ == 1 || !coarsens<double>(pat, 0)) {
-        // no coarse level carries the norm test: test after the cycle
-        icycle_++;
-        const bool done = check_active(pat);
-        icycle_--;
-        if (done) break;
-      }
-    } else {
-      // refinement: FP64 residual, FP32 cycle on it from zero, FP64 update
+    const bool test_first = p_.precision != 0 || (kind == 2 && coarsens<double>(pat, 0));
+    if (test_first) {
+      // the convergence test opens the cycle (src/gmg.c:1629-1630 for the
+      // F-cycle); the rest of the cycle runs as one body
       if (check_active(pat)) break;
-      for (int k : act_) {
-        launch_demote(slots_[k].r64, slots_[k].f[0], nv, s_);
-        launch_zero<float>(slots_[k].u[0], nv, s_);
-      }
-      if (kind == 1) v_cycle<float>(pat, 0, false, true);
-      else if (kind == 3) w_cycle<float>(pat, 0, false, true);
-      else f_cycle<float>(pat, 0, false, true);
-      for (int k : act_) launch_promote_add(slots_[k].u64, slots_[k].u[0], nv, s_);
+      cycle_body(pat);
+      continue;
+    }
+    // FP64 V/W cycles test after the pre-smoothing (src/gmg.c:1599,1656)
+    if (kind == 1) v_cycle<double>(pat, 0, true, false);
+    else if (kind == 3) w_cycle<double>(pat, 0, true, false);
+    else f_cycle<double>(pat, 0, true, false);
+    if (act_.empty()) break;
+    if (lmax_ == 1 || !coarsens<double>(pat, 0)) {
+      // no coarse level carries the norm test: test after the cycle
+      icycle_++;
+      const bool done = check_active(pat);
+      icycle_--;
+      if (done) break;
     }
   }
   for (int k = 0; k < n; k++) {

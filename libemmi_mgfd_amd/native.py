@@ -21,7 +21,7 @@ c128p = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
 class Params(C.Structure):
     _fields_ = [("cycleopt", C.c_int), ("ncycle", C.c_int), ("v1", C.c_int), ("v2", C.c_int),
                 ("isemicoarsen", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
-                ("coarse_sweeps", C.c_int), ("verb", C.c_int)]
+                ("coarse_sweeps", C.c_int), ("verb", C.c_int), ("graphs", C.c_int)]
 
 
 class Stats(C.Structure):
