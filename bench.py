@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--variants", default="plain,mixed,mixed_batched")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-cycles", type=int, default=2, help="F-cycles of the bounded CPU sample")
+    p.add_argument("--no-graphs", action="store_true",
+                   help="launch every kernel eagerly instead of replaying captured cycle graphs "
+                        "(for tracing tools that do not follow graph launches)")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -52,7 +55,7 @@ class ShotRun:
     """One shot on the GPU: device-resident medium, solver and vectors.
     batched: one solver slot per frequency, all frequencies in one solve."""
 
-    def __init__(self, survey, precision, batched=False, shot=0):
+    def __init__(self, survey, precision, batched=False, shot=0, graphs=True):
         from libemmi_mgfd_amd import carriers, native
         self.native = native
         self.batched = batched
@@ -72,7 +75,8 @@ class ShotRun:
         om = list(self.sc.omegas[:nf]) if batched else [self.sc.omegas[0]]
         t0 = time.perf_counter()
         self.solver = native.Solver(self.n, x1, x2, x3, *[d.ptr for d in self.dsig], om,
-                                    sig_on_device=True, precision=precision, tol=1e-6, ncycle=60)
+                                    sig_on_device=True, precision=precision, tol=1e-6, ncycle=60,
+                                    graphs=1 if graphs else 0)
         native.lib().emmi_device_sync()
         self.setup_s = time.perf_counter() - t0
         nb = 16 * self.sc.nvec
@@ -111,13 +115,13 @@ class ShotRun:
 VARIANTS = {"plain": (0, False), "mixed": (1, False), "batched": (0, True), "mixed_batched": (1, True)}
 
 
-def time_variant(survey, variant, steps, warmup, timing=False):
+def time_variant(survey, variant, steps, warmup, timing=False, graphs=True):
     """Warmup, then `steps` timed steps (cycle bodies replayed as HIP graphs);
     with `timing`, one more step runs eagerly with per-launch HIP events for
     the kernel-level figures (the same kernels with the same arguments)."""
     from libemmi_mgfd_amd import native
     precision, batched = VARIANTS[variant]
-    run = ShotRun(survey, precision, batched)
+    run = ShotRun(survey, precision, batched, graphs=graphs)
     for _ in range(warmup):
         run.solve_all()
     native.lib().emmi_device_sync()
@@ -182,7 +186,7 @@ def main():
     survey = configs.CONFIGS[a.config]()
     res = {}
     for v in a.variants.split(","):
-        res[v] = time_variant(survey, v, a.steps, a.warmup, timing=True)
+        res[v] = time_variant(survey, v, a.steps, a.warmup, timing=True, graphs=not a.no_graphs)
         if a.verbose:
             print(v, {k: res[v][k] for k in ("solves_per_s", "ms_per_step", "cycles", "status", "rel_res")},
                   file=sys.stderr)
