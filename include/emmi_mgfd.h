@@ -219,6 +219,52 @@ int emmi_extract_emf_data_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_d
                               void *stream);
 const char *emmi_path_last_error(void);
 
+/* ------------------------------------------------------------------------- */
+/* 3. initial-guess network kernels (no counterpart in the reference)        */
+/* ------------------------------------------------------------------------- */
+/* A 3D U-Net predicts an initial field x0 for the solve from the medium, the
+ * source position and the frequency.  Tensors are channel-last
+ * [Z][Y][X][C], C a multiple of 16, BF16 (16-bit storage) unless noted;
+ * accumulation is FP32.  Convolutions are 3x3x3, zero padding 1, implicit
+ * GEMM on the matrix cores.  Weights: FP32 master [Cout][27*Cin]
+ * (tap = 9 dz + 3 dy + dx), converted by emmi_x0_prep_weights to the BF16
+ * forward layout [Cout][kpad(Cin)] and the flipped, transposed layout
+ * [Cin][kpad(Cout)] used for the input gradient.  Epilogues (epi): 0 bias +
+ * LeakyReLU(slope) -> BF16 [V][Cout]; 1 bias, FP32 [V][cout_store]; 2 input
+ * gradient times the LeakyReLU derivative of act -> BF16; 3 plain BF16.
+ * All return 0 or -1; stream may be NULL. */
+int emmi_x0_kpad(int cin);
+int emmi_x0_conv_fwd(const void *x, int Z, int Y, int X, int Cin, const void *w, int Cout, const float *bias, int epi,
+                     float slope, const void *act, void *out_bf16, float *out_f32, int cout_store, void *stream);
+/* weight gradient [Cout][27*Cin] (and bias gradient, db may be NULL) of a
+ * convolution with input x [V][Cin] and output gradient dy [V][Cout];
+ * part: scratch of ceil(Z/zslab)*27*Cin*Cout floats */
+int emmi_x0_conv_dw(const void *x, const void *dy, int Z, int Y, int X, int Cin, int Cout, float *part, int zslab,
+                    float *dw, float *db, void *stream);
+int emmi_x0_prep_weights(const float *master, int Cin, int Cout, void *w, void *wt, void *stream);
+int emmi_x0_pool2(const void *in, int Z, int Y, int X, int C, void *out, void *stream);
+int emmi_x0_pool2_bwd(const void *dout, int Z, int Y, int X, int C, void *din, int accumulate, void *stream);
+/* out [Z][Y][X][C1+C2] = concat(nearest 2x upsampling of in, skip) and its backward */
+int emmi_x0_up2_cat(const void *in, const void *skip, int Z, int Y, int X, int C1, int C2, void *out, void *stream);
+int emmi_x0_up2_cat_bwd(const void *dout, int Z, int Y, int X, int C1, int C2, void *din, void *dskip, int accumulate,
+                        void *stream);
+int emmi_x0_dact(void *g, const void *a, long n, float slope, void *stream);
+int emmi_x0_mse(const float *out, const float *tgt, const float *mask, long nv, int cs, float inv_count, void *grad,
+                float *part, void *stream);
+int emmi_x0_adam(float *p, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps,
+                 int step, float clip, void *stream);
+int emmi_x0_pack_bf16(const float *in, long nv, int C, int Cp, void *out, void *stream);
+/* network output [n3][n2][n1][6] (asinh(E/eref), Re/Im per component) to a
+ * complex double edge vector, and a solved edge field to the same targets */
+int emmi_x0_out_to_edges(const float *o, int n1, int n2, int n3, double eref, void *x0, void *stream);
+int emmi_x0_edges_to_target(const void *E, int n1, int n2, int n3, double eref, float *tgt, float *mask, void *stream);
+int emmi_x0_features(const float *s11, const float *s33, const float *xs1, const float *xs2, const float *xs3,
+                     const float *d1, const float *d2, const float *d3, int n1, int n2, int n3, float sx, float sy,
+                     float sz, float delta, float hsrc, void *feat, void *stream);
+int emmi_x0_scale(void *v, long n, double s, void *stream);
+/* fill nwords 32-bit words with value */
+int emmi_x0_fill(void *p, long nwords, unsigned value, void *stream);
+
 /* device memory helpers for hosts without a GPU runtime of their own */
 void *emmi_device_alloc(size_t bytes);
 void emmi_device_free(void *p);

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemmi_mgfd.so")
 
-HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "path_kernels.hip"]
+HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "path_kernels.hip", "x0net_kernels.hip"]
 HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp"]
 
 ARCH = "gfx950"

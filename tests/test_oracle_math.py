@@ -278,3 +278,44 @@ def test_create_nugrid_geometric():
     x2 = np.zeros(n + 1, np.float32)
     lib.emo_create_nugrid(n, 500.0, dx, x2)
     assert np.allclose(np.diff(x2), dx, rtol=1e-5)
+
+
+# ---- the initial-guess network's oracle (oracle/x0net_ref.py) -------------
+def test_x0net_oracle_adjoints_and_brute_force():
+    """Convolution, its input and weight gradients are mutual adjoints;
+    pooling and upsampling are adjoints; conv_fwd equals a direct sum over
+    taps and channels."""
+    import x0net_ref as R
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 4, 5, 6))
+    w = rng.standard_normal((7, 27 * 6))
+    g = rng.standard_normal((3, 4, 5, 7))
+    a = np.sum(R.conv_fwd(x, w) * g)
+    assert abs(a - np.sum(x * R.conv_dx(g, w, 6))) <= 1e-10 * abs(a)
+    assert abs(a - np.sum(w * R.conv_dw(x, g))) <= 1e-10 * abs(a)
+    p = rng.standard_normal((4, 4, 6, 3))
+    q = rng.standard_normal((2, 2, 3, 3))
+    assert abs(np.sum(R.pool2(p) * q) - np.sum(p * R.pool2_bwd(q))) <= 1e-12
+    assert abs(np.sum(R.up2(q) * p) - np.sum(q * R.up2_bwd(p))) <= 1e-12
+    # brute force at one output point, with the tap numbering 9 dz + 3 dy + dx
+    y = R.conv_fwd(x, w)
+    z0, y0, x0, co = 1, 0, 4, 3
+    s = 0.0
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                zz, yy, xx = z0 + dz - 1, y0 + dy - 1, x0 + dx - 1
+                if 0 <= zz < 3 and 0 <= yy < 4 and 0 <= xx < 5:
+                    t = 9 * dz + 3 * dy + dx
+                    s += np.dot(x[zz, yy, xx], w[co, t * 6:(t + 1) * 6])
+    assert abs(s - y[z0, y0, x0, co]) <= 1e-12 * max(1.0, abs(s))
+
+
+def test_x0net_oracle_bf16_rounding():
+    import x0net_ref as R
+    v = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 0.0, 1e-30], np.float32)
+    b = R.to_bf16_bits(v)
+    back = R.from_bf16_bits(b)
+    # 1 + 2^-8 is half way between 1 and 1 + 2^-7: ties go to the even mantissa
+    assert back[0] == 1.0 and back[1] == 1.0 and back[2] == 1.0078125
+    assert np.all(np.abs(back - v) <= np.abs(v) * 2.0 ** -8 + 1e-38)
