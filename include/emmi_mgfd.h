@@ -149,6 +149,8 @@ typedef struct {
   int status;         /* EMMI_CONVERGED, EMMI_NOT_CONVERGED, EMMI_NONFINITE */
   double rnorm0;      /* ||f|| */
   double rnorm;       /* last ||f - A u|| */
+  int x0_rejected;    /* 1: the initial guess had the larger residual after its
+                         first cycle and the solve restarted from zero */
 } emmi_stats_t;
 
 enum { EMMI_CONVERGED = 0, EMMI_NOT_CONVERGED = 1, EMMI_NONFINITE = 2, EMMI_ERROR = -1 };
@@ -169,6 +171,8 @@ emmi_solver_t *emmi_solver_create(int n1, int n2, int n3, const float *x1, const
  * i omega mu0 J / vol as inject_source writes it, x <- E, h <- H (h may be b),
  * x0 initial guess or NULL.  Returns the status code. */
 int emmi_solver_solve(emmi_solver_t *s, const void *b, void *x, void *h, const void *x0, emmi_stats_t *st);
+/* (Solves first wait for all work queued on the device, so b and x0 may come
+ * from kernels on any stream; they return when x and h are complete.) */
 /* switch the frequency of a solver (the level hierarchy depends on the grid
  * and medium only, so the frequencies of one shot share it) */
 int emmi_solver_set_omega(emmi_solver_t *s, float omega);
@@ -270,6 +274,7 @@ void *emmi_device_alloc(size_t bytes);
 void emmi_device_free(void *p);
 int emmi_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int emmi_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int emmi_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int emmi_device_sync(void);
 /* kernel timing: accumulated milliseconds, launch counts and algorithmic
  * bytes (each input read once, each output written once per launch) per

@@ -134,6 +134,7 @@ int emmi_solver_solve_batch(emmi_solver_t* h, int n, const void* const* b, void*
         st[k].status = ss[k].status;
         st[k].rnorm0 = ss[k].rnorm0;
         st[k].rnorm = ss[k].rnorm;
+        st[k].x0_rejected = ss[k].x0_rejected;
       }
       worst = std::max(worst, ss[k].status);
     }
@@ -202,6 +203,11 @@ int emmi_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int emmi_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+int emmi_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  // complete before returning, whatever stream later reads dst
+  if (hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) != hipSuccess) return -1;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
 }
 int emmi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
 

@@ -35,6 +35,7 @@ struct SolveStats {
   int cycles = 0;
   int status = 0;        // 0 converged, 1 not converged, 2 non-finite residual
   double rnorm0 = 0, rnorm = 0;
+  int x0_rejected = 0;   // the initial guess lost to the zero guess and was dropped
   std::vector<double> history;
 };
 
@@ -85,6 +86,7 @@ struct Slot {
   double* partial = nullptr;         // residual-norm partial sums
   double rnorm0 = 0;
   SolveStats* st = nullptr;
+  bool x0_live = false;              // started from a caller's guess, not yet judged
   double seen_omu[3] = {-1.0, -1.0, -1.0};  // frequency whose factors a cycle of pattern p has built
 };
 

@@ -443,9 +443,22 @@ bool Solver::check_active(int pat) {
   std::vector<double> rn;
   norms(act_, u.data(), f.data(), r.data(), rn);
   std::vector<int> keep;
+  const long nv = 3 * nc0_;
   for (size_t i = 0; i < act_.size(); i++) {
     Slot& S = slots_[act_[i]];
     SolveStats* st = S.st;
+    if (S.x0_live && (icycle_ >= 1 || !std::isfinite(rn[i]))) {
+      // The guess is judged at the first test after a cycle from it: when
+      // its residual is not below the zero guess's (||f||), the slot
+      // restarts from zero; the cycle spent on it still counts.
+      S.x0_live = false;
+      if (!(rn[i] <= S.rnorm0)) {
+        launch_zero<double>(S.u64, nv, s_);
+        HIPCHECK(hipMemcpyAsync(S.r64, S.f64, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+        st->x0_rejected = 1;
+        rn[i] = S.rnorm0;
+      }
+    }
     st->history.push_back(rn[i]);
     if (p_.verb) printf("slot=%d icycle=%d rnorm=%e\n", act_[i], icycle_, rn[i]);
     if (!std::isfinite(rn[i])) {
@@ -627,6 +640,9 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
                         SolveStats* st) {
   if (n < 1 || n > nslots()) throw std::runtime_error("batch size out of range of the solver's slots");
   const long nv = 3 * nc0_;
+  // The solver's stream does not order itself after other streams: wait for
+  // the work the caller queued elsewhere (b and x0 may still be in flight).
+  HIPCHECK(hipDeviceSynchronize());
   act_.clear();
   for (int k = 0; k < n; k++) {
     Slot& S = slots_[k];
@@ -650,8 +666,10 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
       slots_[act_[i]].st->rnorm0 = rn[i];
     }
   }
-  for (int k = 0; k < n; k++)
-    if (x0 && x0[k]) HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+  for (int k = 0; k < n; k++) {
+    slots_[k].x0_live = x0 && x0[k];
+    if (slots_[k].x0_live) HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+  }
 
   const int kind = p_.cycleopt;
   for (icycle_ = 0; icycle_ < p_.ncycle && !act_.empty(); icycle_++) {

@@ -25,7 +25,8 @@ class Params(C.Structure):
 
 
 class Stats(C.Structure):
-    _fields_ = [("cycles", C.c_int), ("status", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double)]
+    _fields_ = [("cycles", C.c_int), ("status", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
+                ("x0_rejected", C.c_int)]
 
 
 STATUS = {0: "converged", 1: "not converged", 2: "non-finite residual", -1: "error"}
@@ -96,6 +97,8 @@ def lib() -> C.CDLL:
         L.emmi_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.emmi_memcpy_d2h.restype = C.c_int
         L.emmi_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.emmi_memcpy_d2d.restype = C.c_int
+        L.emmi_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.emmi_device_sync.restype = C.c_int
         L.emmi_timing_enable.argtypes = [C.c_int]
         L.emmi_timing_read.restype = C.c_int
