@@ -198,6 +198,8 @@ int emmi_solver_solve_batch(emmi_solver_t *s, int n, const void *const *b, void 
 int emmi_solver_set_omega_slot(emmi_solver_t *s, int slot, float omega);
 int emmi_solver_history_slot(emmi_solver_t *s, int slot, double *out, int maxn);
 int emmi_solver_slots(emmi_solver_t *s);
+/* cycle limit of the following solves (the ncycle of the parameters) */
+int emmi_solver_set_ncycle(emmi_solver_t *s, int ncycle);
 int emmi_solver_levels(emmi_solver_t *s, int *npatterns);
 void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
 
@@ -258,13 +260,20 @@ int emmi_x0_mse(const float *out, const float *tgt, const float *mask, long nv, 
 int emmi_x0_adam(float *p, const float *g, float *m1, float *m2, long n, float lr, float b1, float b2, float eps,
                  int step, float clip, void *stream);
 int emmi_x0_pack_bf16(const float *in, long nv, int C, int Cp, void *out, void *stream);
-/* network output [n3][n2][n1][6] (asinh(E/eref), Re/Im per component) to a
- * complex double edge vector, and a solved edge field to the same targets */
-int emmi_x0_out_to_edges(const float *o, int n1, int n2, int n3, double eref, void *x0, void *stream);
-int emmi_x0_edges_to_target(const void *E, int n1, int n2, int n3, double eref, float *tgt, float *mask, void *stream);
+/* The network refines x1, the field after one multigrid cycle: per field
+ * component it outputs dL and a relative phasor (1+q1, q2), and
+ * x0_a = x1_a exp(dL) (1+q1 + i q2)/|1+q1 + i q2| (zero outputs give x1).
+ * out_to_edges builds x0 from the output [n3][n2][n1][9]; edges_to_target
+ * builds the training targets from a solved field E and x1, with loss
+ * weights 1 (dL) and wp (phasor) where |E| and |x1| exceed eref. */
+int emmi_x0_out_to_edges(const float *o, const void *x1, int n1, int n2, int n3, void *x0, void *stream);
+int emmi_x0_edges_to_target(const void *E, const void *x1, int n1, int n2, int n3, double eref, float wp, float *tgt,
+                            float *mask, void *stream);
+/* input features [n1*n2*n3][32] from the medium, the source offset in skin
+ * depths and, when x1 is given, the field after one multigrid cycle */
 int emmi_x0_features(const float *s11, const float *s33, const float *xs1, const float *xs2, const float *xs3,
                      const float *d1, const float *d2, const float *d3, int n1, int n2, int n3, float sx, float sy,
-                     float sz, float delta, float hsrc, void *feat, void *stream);
+                     float sz, float delta, float hsrc, const void *x1, double eref, void *feat, void *stream);
 int emmi_x0_scale(void *v, long n, double s, void *stream);
 /* fill nwords 32-bit words with value */
 int emmi_x0_fill(void *p, long nwords, unsigned value, void *stream);

@@ -162,6 +162,13 @@ int emmi_solver_set_omega_slot(emmi_solver_t* h, int slot, float omega) {
 
 int emmi_solver_set_omega(emmi_solver_t* h, float omega) { return emmi_solver_set_omega_slot(h, 0, omega); }
 
+int emmi_solver_set_ncycle(emmi_solver_t* h, int ncycle) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  if (ncycle < 0) { set_err("negative cycle count"); return EMMI_ERROR; }
+  h->s->set_ncycle(ncycle);
+  return 0;
+}
+
 int emmi_solver_history_slot(emmi_solver_t* h, int slot, double* out, int maxn) {
   if (!h || slot < 0 || slot >= (int)h->hist.size()) return 0;
   const std::vector<double>& v = h->hist[slot];

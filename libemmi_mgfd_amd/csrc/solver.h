@@ -118,6 +118,8 @@ class Solver {
   // the hierarchy depends on the grid and medium only: switch frequency
   void set_omega(float omega) { set_omega(0, omega); }
   void set_omega(int slot, float omega);
+  // cycle limit of the following solves
+  void set_ncycle(int ncycle) { p_.ncycle = ncycle; }
   const int* dims(int pat, int lev) const { return dims_[pat][lev].data(); }
 
   // kernel-level access for the element-wise checks (FP64 hierarchy)

@@ -352,10 +352,15 @@ __global__ void k_pack_bf16(const float* __restrict__ in, long nv, int C, int Cp
   out[i] = c < C ? f2bf(in[v * C + c]) : (bf16)0;
 }
 
-// Network output [n3][n2][n1][6] (asinh-compressed Re/Im of E1, E2, E3 per
-// cell index p) -> complex double edge vector x0 [3][n3+1][n2+1][n1+1]:
-// E_a(p) = eref * (sinh(o[2a]) + i sinh(o[2a+1])) on interior edges, 0 elsewhere
-__global__ void k_out_to_edges(const float* __restrict__ o, int n1, int n2, int n3, double eref,
+// The network refines a field x1 (the field after one multigrid cycle): per
+// component a it outputs three channels, a change of log-amplitude dL and a
+// relative phasor (1 + q1) + i q2, and the refined field is
+//   E_a = x1_a exp(dL) ((1 + q1) + i q2) / |(1 + q1) + i q2|.
+// Zero outputs reproduce x1 exactly.
+
+// Network output [n3][n2][n1][9] and x1 -> complex double edge vector x0
+// [3][n3+1][n2+1][n1+1] (interior edges; others 0)
+__global__ void k_out_to_edges(const float* __restrict__ o, const double2* __restrict__ x1, int n1, int n2, int n3,
                                double2* __restrict__ x0) {
   const long nc = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -369,16 +374,24 @@ __global__ void k_out_to_edges(const float* __restrict__ o, int n1, int n2, int 
   double2 v = make_double2(0.0, 0.0);
   if (p[a] < n[a] && p[b] >= 1 && p[b] <= n[b] - 1 && p[c] >= 1 && p[c] <= n[c] - 1) {
     const long cell = ((long)k * n2 + j) * n1 + i;
-    v.x = eref * sinh((double)o[cell * 6 + 2 * a]);
-    v.y = eref * sinh((double)o[cell * 6 + 2 * a + 1]);
+    const double dL = fmin(fmax((double)o[cell * 9 + 3 * a], -20.0), 20.0);
+    const double qr = 1.0 + (double)o[cell * 9 + 3 * a + 1], qi = o[cell * 9 + 3 * a + 2];
+    const double nrm = sqrt(qr * qr + qi * qi);
+    const double2 u = x1[t];
+    if (nrm > 1e-6) {
+      const double s = exp(dL) / nrm;
+      v.x = s * (u.x * qr - u.y * qi);
+      v.y = s * (u.x * qi + u.y * qr);
+    }
   }
   x0[t] = v;
 }
 
-// Training target from a solved edge field: t[cell][2a(+1)] = asinh(E_a(p)/eref)
-// with mask 1 on interior edges, 0 on cells whose edge a is not interior
-__global__ void k_edges_to_target(const double2* __restrict__ E, int n1, int n2, int n3, double eref,
-                                  float* __restrict__ tgt, float* __restrict__ mask) {
+// Training targets [V][9] for refining x1 towards the solved field E, and the
+// loss weights: 1 on dL, wp on the phasor channels, where both |E_a| and
+// |x1_a| exceed eref (elsewhere 0: the field is negligible there)
+__global__ void k_edges_to_target(const double2* __restrict__ E, const double2* __restrict__ x1, int n1, int n2,
+                                  int n3, double eref, float wp, float* __restrict__ tgt, float* __restrict__ mask) {
   const long ncell = (long)n1 * n2 * n3;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= ncell) return;
@@ -389,28 +402,40 @@ __global__ void k_edges_to_target(const double2* __restrict__ E, int n1, int n2,
   for (int a = 0; a < 3; a++) {
     const int b = (a + 1) % 3, c = (a + 2) % 3;
     const bool in = p[b] >= 1 && p[b] <= n[b] - 1 && p[c] >= 1 && p[c] <= n[c] - 1;
-    const double2 v = in ? E[a * nc + i + es1 * j + es2 * k] : make_double2(0.0, 0.0);
-    tgt[t * 6 + 2 * a] = (float)asinh(v.x / eref);
-    tgt[t * 6 + 2 * a + 1] = (float)asinh(v.y / eref);
-    mask[t * 6 + 2 * a] = in ? 1.f : 0.f;
-    mask[t * 6 + 2 * a + 1] = in ? 1.f : 0.f;
+    const long o = a * nc + i + es1 * j + es2 * k;
+    const double2 v = in ? E[o] : make_double2(0.0, 0.0);
+    const double2 u = in ? x1[o] : make_double2(0.0, 0.0);
+    const double av = sqrt(v.x * v.x + v.y * v.y), au = sqrt(u.x * u.x + u.y * u.y);
+    const bool ok = in && av > eref && au > eref;
+    // relative phasor (E/|E|) conj(x1/|x1|)
+    const double rr = ok ? (v.x * u.x + v.y * u.y) / (av * au) : 1.0;
+    const double ri = ok ? (v.y * u.x - v.x * u.y) / (av * au) : 0.0;
+    tgt[t * 9 + 3 * a] = ok ? (float)log(av / au) : 0.f;
+    tgt[t * 9 + 3 * a + 1] = (float)(rr - 1.0);
+    tgt[t * 9 + 3 * a + 2] = (float)ri;
+    mask[t * 9 + 3 * a] = ok ? 1.f : 0.f;
+    mask[t * 9 + 3 * a + 1] = ok ? wp : 0.f;
+    mask[t * 9 + 3 * a + 2] = ok ? wp : 0.f;
   }
 }
 
-// Input features on the cells [n3][n2][n1][16]:
+// Input features on the cells [n3][n2][n1][32]:
 //   0 log10 sigma11, 1 log10 sigma33, 2..4 (x - xs)/delta per axis (clamped
 //   to +-8), 5 log10(r/delta + 1e-3), 6 log10(cell size / delta),
-//   7 source blob exp(-(r/h)^2) with h the source cell size, 8 constant 1
+//   7 source blob exp(-(r/h)^2) with h the source cell size, 8 constant 1,
+//   9..17 when x1 is given: per component a of the field x1 after one
+//   multigrid cycle, ln(max(|x1_a|, eref)/eref)/10 and its unit phasor
 __global__ void k_features(const float* __restrict__ s11, const float* __restrict__ s33, const float* __restrict__ xs1,
                            const float* __restrict__ xs2, const float* __restrict__ xs3, const float* __restrict__ d1,
                            const float* __restrict__ d2, const float* __restrict__ d3, int n1, int n2, int n3,
-                           float sx, float sy, float sz, float delta, float hsrc, bf16* __restrict__ feat) {
+                           float sx, float sy, float sz, float delta, float hsrc, const double2* __restrict__ x1,
+                           double eref, bf16* __restrict__ feat) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const long nc = (long)n1 * n2 * n3;
   if (t >= nc) return;
   const int i = (int)(t % n1), j = (int)((t / n1) % n2), k = (int)(t / ((long)n1 * n2));
-  float f[16];
-  for (int c = 0; c < 16; c++) f[c] = 0.f;
+  float f[32];
+  for (int c = 0; c < 32; c++) f[c] = 0.f;
   f[0] = log10f(fmaxf(s11[t], 1e-12f));
   f[1] = log10f(fmaxf(s33[t], 1e-12f));
   const float dx = xs1[i] - sx, dy = xs2[j] - sy, dz = xs3[k] - sz;
@@ -422,7 +447,21 @@ __global__ void k_features(const float* __restrict__ s11, const float* __restric
   f[6] = log10f(cbrtf(d1[i] * d2[j] * d3[k]) / delta);
   f[7] = expf(-(r / hsrc) * (r / hsrc));
   f[8] = 1.f;
-  for (int c = 0; c < 16; c++) feat[t * 16 + c] = f2bf(f[c]);
+  if (x1) {
+    const int p[3] = {i, j, k};
+    const int n[3] = {n1, n2, n3};
+    const long es1 = n1 + 1, es2 = (long)(n1 + 1) * (n2 + 1), ncc = es2 * (n3 + 1);
+    for (int a = 0; a < 3; a++) {
+      const int b = (a + 1) % 3, c = (a + 2) % 3;
+      const bool in = p[b] >= 1 && p[b] <= n[b] - 1 && p[c] >= 1 && p[c] <= n[c] - 1;
+      const double2 v = in ? x1[a * ncc + i + es1 * j + es2 * k] : make_double2(0.0, 0.0);
+      const double amp = sqrt(v.x * v.x + v.y * v.y);
+      f[9 + 3 * a] = (float)(0.1 * log(fmax(amp, eref) / eref));
+      f[10 + 3 * a] = amp > eref ? (float)(v.x / amp) : 0.f;
+      f[11 + 3 * a] = amp > eref ? (float)(v.y / amp) : 0.f;
+    }
+  }
+  for (int c = 0; c < 32; c++) feat[t * 32 + c] = f2bf(f[c]);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, long n, uint32_t v) {
@@ -540,25 +579,27 @@ int emmi_x0_pack_bf16(const float* in, long nv, int C, int Cp, void* out, void* 
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int emmi_x0_out_to_edges(const float* o, int n1, int n2, int n3, double eref, void* x0, void* stream) {
+int emmi_x0_out_to_edges(const float* o, const void* x1, int n1, int n2, int n3, void* x0, void* stream) {
   const long n = 3L * (n1 + 1) * (n2 + 1) * (n3 + 1);
-  hipLaunchKernelGGL(k_out_to_edges, dim3(blocks(n, 256)), dim3(256), 0, S(stream), o, n1, n2, n3, eref, (double2*)x0);
+  hipLaunchKernelGGL(k_out_to_edges, dim3(blocks(n, 256)), dim3(256), 0, S(stream), o, (const double2*)x1, n1, n2, n3,
+                     (double2*)x0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int emmi_x0_edges_to_target(const void* E, int n1, int n2, int n3, double eref, float* tgt, float* mask, void* stream) {
+int emmi_x0_edges_to_target(const void* E, const void* x1, int n1, int n2, int n3, double eref, float wp, float* tgt,
+                            float* mask, void* stream) {
   const long n = (long)n1 * n2 * n3;
-  hipLaunchKernelGGL(k_edges_to_target, dim3(blocks(n, 256)), dim3(256), 0, S(stream), (const double2*)E, n1, n2, n3,
-                     eref, tgt, mask);
+  hipLaunchKernelGGL(k_edges_to_target, dim3(blocks(n, 256)), dim3(256), 0, S(stream), (const double2*)E,
+                     (const double2*)x1, n1, n2, n3, eref, wp, tgt, mask);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int emmi_x0_features(const float* s11, const float* s33, const float* xs1, const float* xs2, const float* xs3,
                      const float* d1, const float* d2, const float* d3, int n1, int n2, int n3, float sx, float sy,
-                     float sz, float delta, float hsrc, void* feat, void* stream) {
+                     float sz, float delta, float hsrc, const void* x1, double eref, void* feat, void* stream) {
   const long n = (long)n1 * n2 * n3;
   hipLaunchKernelGGL(k_features, dim3(blocks(n, 256)), dim3(256), 0, S(stream), s11, s33, xs1, xs2, xs3, d1, d2, d3,
-                     n1, n2, n3, sx, sy, sz, delta, hsrc, (bf16*)feat);
+                     n1, n2, n3, sx, sy, sz, delta, hsrc, (const double2*)x1, eref, (bf16*)feat);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
         L.emmi_solver_set_omega_slot.argtypes = [C.c_void_p, C.c_int, C.c_float]
         L.emmi_solver_history_slot.restype = C.c_int
         L.emmi_solver_history_slot.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int]
+        L.emmi_solver_set_ncycle.restype = C.c_int
+        L.emmi_solver_set_ncycle.argtypes = [C.c_void_p, C.c_int]
         L.emmi_solver_slots.restype = C.c_int
         L.emmi_solver_slots.argtypes = [C.c_void_p]
         from . import carriers as K
@@ -211,6 +213,10 @@ class Solver:
         if rc < 0:
             raise NativeError(last_error())
         return list(st)
+
+    def set_ncycle(self, ncycle):
+        if lib().emmi_solver_set_ncycle(self.h, int(ncycle)):
+            raise NativeError(last_error())
 
     def set_omega(self, omega, slot=0):
         if lib().emmi_solver_set_omega_slot(self.h, int(slot), np.float32(omega)):

@@ -12,13 +12,19 @@ Architecture (all convolutions 3x3x3, zero padding 1, LeakyReLU 0.1):
     encoder: two convolutions per level, 2x average pooling between levels
     decoder: 2x nearest upsampling, concatenation with the encoder tensor of
              the same level, two convolutions
-    output:  one linear convolution to 6 channels (Re, Im of E1, E2, E3)
+    output:  one linear convolution to 9 channels: per component of E, a
+             change of log-amplitude and a relative phasor that refine the
+             field after one multigrid cycle (zero output = that field)
 
-Inputs (16 channels, 9 used): log10 sigma11, log10 sigma33, the offset to the
-source in skin depths of the source medium (3), log10 of the distance,
-log10 of the cell size, a source blob and a constant.  The target is
-asinh(E / EREF) per component, so that the many decades of the field are
-learnt with a relative accuracy.
+Inputs (32 channels, 18 used): log10 sigma11, log10 sigma33, the offset to
+the source in skin depths of the source medium (3), log10 of the distance,
+log10 of the cell size, a source blob, a constant, and the log-amplitude and
+unit phasor of each component of the field x1 after one multigrid cycle (9).
+The network learns the correction from x1 to the converged field; with the
+output convolution initialised to zero it starts as the identity on x1.  Amplitude and phase
+are learnt separately: the log-amplitude spans the many decades of the field
+with a relative accuracy, and the phasor varies smoothly where the real and
+imaginary parts change sign.
 """
 from __future__ import annotations
 
@@ -31,8 +37,9 @@ from . import native
 SLOPE = 0.1
 EREF = 1e-15
 LEVELS = (16, 32, 48, 64)
-CIN = 16
-COUT_STORE = 6
+CIN = 32
+COUT_STORE = 9
+PHASOR_WEIGHT = 4.0
 
 
 def _lib():
@@ -53,9 +60,9 @@ def _lib():
         "emmi_x0_mse": (i, [vp, vp, vp, l, i, f, vp, vp, vp]),
         "emmi_x0_adam": (i, [vp, vp, vp, vp, l, f, f, f, f, i, f, vp]),
         "emmi_x0_pack_bf16": (i, [vp, l, i, i, vp, vp]),
-        "emmi_x0_out_to_edges": (i, [vp, i, i, i, d, vp, vp]),
-        "emmi_x0_edges_to_target": (i, [vp, i, i, i, d, vp, vp, vp]),
-        "emmi_x0_features": (i, [vp] * 8 + [i, i, i, f, f, f, f, f, vp, vp]),
+        "emmi_x0_out_to_edges": (i, [vp, vp, i, i, i, vp, vp]),
+        "emmi_x0_edges_to_target": (i, [vp, vp, i, i, i, d, f, vp, vp, vp]),
+        "emmi_x0_features": (i, [vp] * 8 + [i, i, i, f, f, f, f, f, vp, d, vp, vp]),
         "emmi_x0_scale": (i, [vp, l, d, vp]),
         "emmi_x0_fill": (i, [vp, l, C.c_uint, vp]),
     }
@@ -134,6 +141,8 @@ class Conv:
         w = rng.standard_normal((cout, 27 * cin)).astype(np.float32) * np.float32(std)
         if cout_store is not None:
             w[cout_store:] = 0.0   # channels beyond cout_store stay zero (zero gradients)
+        if linear:
+            w[:] = 0.0             # the output layer starts as "no correction"
         self.nw = w.size
         self.w = Buf.of(w)
         self.b = Buf(4 * cout)
@@ -279,7 +288,7 @@ class UNet:
         return self.y
 
     def loss_and_grad(self, tgt, mask, count):
-        """Masked mean-square error of the last forward against tgt [V][6]
+        """Weighted mean-square error of the last forward against tgt [V][9]
         (mask with `count` ones); writes the output gradient and returns the
         loss (synchronises)."""
         _ok(_lib().emmi_x0_mse(self.y.ptr, tgt.ptr, mask.ptr, self.nv, COUT_STORE, 1.0 / count,

@@ -7,9 +7,10 @@ Data: libemmi_mgfd_amd.x0data (random earths on the 128^3 survey grid, three
 frequencies each).  Training: the U-Net of libemmi_mgfd_amd.x0net on its own
 HIP kernels, Adam with cosine decay, one sample per step.  Evaluation: held-out
 models solved to relative residual 1e-6 from a zero guess and from the
-network's prediction (FP32 cycles in FP64 refinement, all three frequencies
-batched), and the cycles saved; the solver judges the prediction and falls
-back to zero when it loses.
+network's prediction, which refines the field after one multigrid cycle (FP32
+cycles in FP64 refinement, all three frequencies batched); the learned path
+is charged that extra cycle.  The solver judges the prediction and falls back
+to zero when it loses.
 """
 from __future__ import annotations
 
@@ -24,12 +25,13 @@ import numpy as np
 from . import native, x0data, x0net
 
 
-def predict(net, grid, feat):
-    """Network output for one sample as a device edge vector (complex double)."""
+def predict(net, grid, feat, x1):
+    """The network's refinement of the one-cycle field x1, as a device edge
+    vector (complex double)."""
     y = net.forward(feat)
     x0 = native.DeviceBuffer(16 * grid.sc.nvec)
     n1, n2, n3 = grid.n
-    x0net._ok(x0net._lib().emmi_x0_out_to_edges(y.ptr, n1, n2, n3, x0net.EREF, x0.ptr, None), "out_to_edges")
+    x0net._ok(x0net._lib().emmi_x0_out_to_edges(y.ptr, x1.ptr, n1, n2, n3, x0.ptr, None), "out_to_edges")
     native.lib().emmi_device_sync()
     return x0
 
@@ -49,6 +51,9 @@ def evaluate(net, grid, nmodels, seed, log=print):
 
 
 def solve_pair(net, grid, s11, s33, freqs, tol=1e-6):
+    """Cycles from zero, and for the learned path 1 + cycles from the network's
+    refinement of the one-cycle field, for one medium at several frequencies
+    (batched)."""
     n = grid.n
     ds11, ds33 = x0net.Buf.of(s11), x0net.Buf.of(s33)
     xs = grid.sc.x_arrays()
@@ -56,30 +61,31 @@ def solve_pair(net, grid, s11, s33, freqs, tol=1e-6):
     S = native.Solver(n, xs[0], xs[1], xs[2], ds11.ptr, ds11.ptr, ds33.ptr, om, sig_on_device=True, precision=1,
                       tol=tol, ncycle=60)
     nb = 16 * grid.sc.nvec
-    bs = []
-    for w in om:
-        grid.sc.omegas[0] = w
-        b = native.DeviceBuffer(nb)
-        if native.lib().emmi_inject_source_dev(grid.sc.acq, grid.sc.emf, 0, b.ptr, None):
-            raise native.NativeError(native.lib().emmi_path_last_error().decode())
-        bs.append(b)
+    bs = x0data.source_terms(grid, om)
     hs = [native.DeviceBuffer(nb) for _ in om]
-    xs_ = [native.DeviceBuffer(nb) for _ in om]
-    # the solves overwrite h, so each run gets the source terms afresh
+    xo = [native.DeviceBuffer(nb) for _ in om]
+
     def run(x0s):
-        for b, h in zip(bs, hs):
-            if native.lib().emmi_memcpy_d2d(h.ptr, b.ptr, nb):
-                raise native.NativeError("device copy failed")
-        return S.solve_batch_device([h.ptr for h in hs], [x.ptr for x in xs_], [h.ptr for h in hs], x0s)
+        x0data.copy_all(hs, bs)
+        return S.solve_batch_device([h.ptr for h in hs], [x.ptr for x in xo], [h.ptr for h in hs], x0s)
+
     st0 = run(None)
-    x0 = [predict(net, grid, x0data.features(grid, ds11, ds33, f)) for f in freqs]
+    x0 = learned_guess(S, net, grid, ds11, ds33, freqs, run, xo)
     st1 = run([p.ptr for p in x0])
     S.close()
-    return [{"freq": float(f), "cycles_zero": a.cycles, "cycles_x0": b.cycles, "rejected": int(b.x0_rejected),
+    return [{"freq": float(f), "cycles_zero": a.cycles, "cycles_x0": 1 + b.cycles, "rejected": int(b.x0_rejected),
              "status": [a.status, b.status]} for f, a, b in zip(freqs, st0, st1)]
 
 
-def train(net, samples, mask, count, steps, lr0, log=print, val=None):
+def learned_guess(S, net, grid, ds11, ds33, freqs, run, xo, ncycle=60):
+    """One batched cycle from zero, then the network's refinement of it."""
+    S.set_ncycle(1)
+    run(None)
+    S.set_ncycle(ncycle)
+    return [predict(net, grid, x0data.features(grid, ds11, ds33, f, x), x) for f, x in zip(freqs, xo)]
+
+
+def train(net, samples, steps, lr0, log=print, val=None):
     rng = np.random.default_rng(0)
     hist = []
     t0 = time.perf_counter()
@@ -87,7 +93,7 @@ def train(net, samples, mask, count, steps, lr0, log=print, val=None):
         sm = samples[int(rng.integers(len(samples)))]
         lr = lr0 * 0.5 * (1 + math.cos(math.pi * it / steps)) + 0.02 * lr0
         net.forward(sm.feat)
-        loss = net.loss_and_grad(sm.tgt, mask, count)
+        loss = net.loss_and_grad(sm.tgt, sm.mask, sm.count)
         net.backward()
         net.adam(lr=lr)
         hist.append(loss)
@@ -97,7 +103,7 @@ def train(net, samples, mask, count, steps, lr0, log=print, val=None):
                 vl = []
                 for v in val:
                     net.forward(v.feat)
-                    vl.append(net.loss_and_grad(v.tgt, mask, count))
+                    vl.append(net.loss_and_grad(v.tgt, v.mask, v.count))
                 msg += f" val {np.mean(vl):.4f}"
             log(msg)
     return hist
@@ -120,13 +126,13 @@ def main(argv=None):
 
     grid = x0data.Grid()
     t0 = time.perf_counter()
-    samples, mask, cnt = x0data.generate(grid, a.models, seed=a.seed, log=log)
-    val, _, _ = x0data.generate(grid, a.val_models, seed=10_000 + a.seed, log=log)
+    samples = x0data.generate(grid, a.models, seed=a.seed, log=log)
+    val = x0data.generate(grid, a.val_models, seed=10_000 + a.seed, log=log)
     log(f"data: {len(samples)} training and {len(val)} validation samples in {time.perf_counter() - t0:.0f}s")
     net = x0net.UNet((grid.n[2], grid.n[1], grid.n[0]), seed=a.seed)
     if a.init:
         net.load(np.load(a.init))
-    hist = train(net, samples, mask, cnt, a.steps, a.lr, log=log, val=val)
+    hist = train(net, samples, a.steps, a.lr, log=log, val=val)
     state = net.state()
     state["train_loss"] = np.array(hist, np.float32)
     np.savez_compressed(a.out, **state)
