@@ -2,11 +2,15 @@
 //
 // A sweep along axis a visits the lines (a; pb, pc) in four colours
 // (pb%2, pc%2); lines of one colour are independent.  Per colour the line
-// systems are factored once (k_line_factor: records of FREC doubles per node,
-// depending on grid, medium and frequency only) and every sweep then runs
-// the residual and a forward/back substitution (k_line_solve, scratch of
-// SREC doubles per node).  A launch takes a pack of up to LINE_PACK_MAX jobs
-// (one per right-hand side or per level) that share the axis and colour.
+// systems are factored once (k_line_factor: FREC values per node, depending
+// on grid, medium and frequency only) and every sweep then computes the
+// right-hand sides (k_line_rhs, SREC values per node) and runs a forward/back
+// substitution (k_line_solve).  Records hold values of type LineRec<T>::type
+// in 16-byte chunks: value k of node p of line l is element k % V of chunk
+// (p * nchunks + k / V) * nlines + l, V = 16 / sizeof(value), so a lane reads
+// its node with 16-byte loads and a wave's load covers contiguous memory.  A launch takes a pack of up to
+// LINE_PACK_MAX jobs (one per right-hand side or per level) that share the
+// axis and colour.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,13 +21,23 @@ namespace emmi {
 
 constexpr int LINE_PACK_MAX = 8;
 
+// Value type of the line records for cycles of storage precision T.  FP64
+// for both: in air the line blocks are close to singular, W^-1 and the
+// forward intermediates z are large and cancel in the back substitution, and
+// FP32 records lose the correction there (the FP32-storage cycle diverges).
+// EMMI_LINE_REC_F32 selects FP32 records for FP32 cycles (experiments only).
+template <typename T> struct LineRec { using type = double; };
+#ifdef EMMI_LINE_REC_F32
+template <> struct LineRec<float> { using type = float; };
+#endif
+
 template <typename T>
 struct LineJob {
   Level<T> L;
   typename cvec<T>::type* u;
   const typename cvec<T>::type* f;
-  double* fac;   // factor records of this colour: [n_a][FREC][nlines]
-  double* scr;   // substitution scratch: [n_a][SREC][nlines]
+  typename LineRec<T>::type* fac;  // factor records of this colour, chunked: [n_a][FREC/V][nlines][V]
+  typename LineRec<T>::type* scr;  // right-hand side / substitution records, chunked: [n_a][SREC/V][nlines][V]
   int cb, cc;    // colour: parity of pb and pc (1 = odd)
   int nlo, nhi;  // lines of this colour along the low- and high-stride perpendicular axes
 };
@@ -36,9 +50,10 @@ struct LinePack {
 
 // number of lines of a colour, split by perpendicular axis (low stride first)
 void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi);
-// doubles of one colour's factor records / substitution scratch
-size_t line_factor_doubles(const int* n, int a, int colour);
-size_t line_scratch_doubles(const int* n, int a, int colour);
+// bytes of one colour's factor records / substitution records for values of
+// esize bytes (4 or 8)
+size_t line_factor_bytes(const int* n, int a, int colour, size_t esize);
+size_t line_scratch_bytes(const int* n, int a, int colour, size_t esize);
 // fills the colour and counts of a job from its level shape
 template <typename T>
 inline void line_job_colour(LineJob<T>& J, int a, int colour) {

@@ -15,13 +15,17 @@
 //     W_1 = D_1,  W_p = D_p - B_p W_{p-1}^{-1} B_p
 // whose factors depend on the grid, the medium and the frequency only.
 // k_line_factor stores, per node, W_p^{-1} (symmetric, 10 entries) and
-// 1/d_p once; k_line_solve then runs, per sweep,
+// 1/d_p once; k_line_rhs computes the residuals sigma_p, rho_p of the lines
+// and k_line_solve then runs, per sweep,
 //     y_p = g_p - B_p z_{p-1},  z_p = W_p^{-1} y_p          (forward)
 //     da_p = z_p - W_p^{-1} B_{p+1} da_{p+1},
 //     dE_a(p) = rho_p/d_p + v.(da_{p+1} - da_p) / d_p        (backward)
 // with g_p = sigma_p + v (rho_{p-1}/d_{p-1} - rho_p/d_p) from the residual.
 // Lines of one colour (pb%2, pc%2) are not coupled, so a launch sweeps one
 // colour; every kernel computes in FP64 whatever the storage precision.
+// Records hold LineRec<T> values in 16-byte chunks (layout in line_kernels.h);
+// the substitution keeps a ring of the next D nodes' chunks in registers, so
+// D nodes of loads are in flight while a node is computed.
 #include "line_kernels.h"
 
 #include <algorithm>
@@ -30,8 +34,44 @@
 
 namespace emmi {
 
-constexpr int FREC = 22;  // doubles per node in the factor records
-constexpr int SREC = 10;  // doubles per node in the substitution scratch
+constexpr int FREC = 22;  // values per node in the factor records: W^-1 (10 complex), 1/d
+constexpr int SREC = 10;  // values per node in the line records: sigma or z (4 complex), rho
+
+// 16-byte chunk of record values
+template <typename R>
+struct alignas(16) Ch {
+  static constexpr int V = 16 / (int)sizeof(R);
+  R v[V];
+};
+template <typename R> constexpr int fchunks() { return (FREC + Ch<R>::V - 1) / Ch<R>::V; }
+template <typename R> constexpr int schunks() { return (SREC + Ch<R>::V - 1) / Ch<R>::V; }
+template <typename T> using Rec = typename LineRec<T>::type;
+
+// nodes of records in flight in the substitution (build-time tunable)
+// for FP32-storage and FP64 cycles
+#ifndef EMMI_LINE_RING_F32
+#define EMMI_LINE_RING_F32 2
+#endif
+#ifndef EMMI_LINE_RING_F64
+#define EMMI_LINE_RING_F64 2
+#endif
+template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_LINE_RING_F32 : EMMI_LINE_RING_F64; }
+
+// value k of a node's chunks, as double
+template <typename R, int NC>
+__device__ __forceinline__ double val(const Ch<R> (&c)[NC], int k) {
+  return (double)c[k / Ch<R>::V].v[k % Ch<R>::V];
+}
+template <typename R, int NC>
+__device__ __forceinline__ void put(Ch<R> (&c)[NC], int k, double x) {
+  c[k / Ch<R>::V].v[k % Ch<R>::V] = (R)x;
+}
+// loads the NC chunks of one node; src points at chunk 0 of that node's line
+template <typename R, int NC>
+__device__ __forceinline__ void load_node(Ch<R> (&c)[NC], const Ch<R>* __restrict__ src, long stride) {
+#pragma unroll
+  for (int k = 0; k < NC; k++) c[k] = src[k * stride];
+}
 
 __device__ __forceinline__ int symidx(int i, int j) {
   const int a = i < j ? i : j, b = i < j ? j : i;
@@ -117,11 +157,18 @@ __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
   int pb, pc;
   if (!line_of<T, A>(J, tid, pb, pc)) return;
   const LineGeo<T, A> G(L, pb, pc);
+  using R = Rec<T>;
+  constexpr int FC = fchunks<R>(), V = Ch<R>::V;
   const long stride = (long)J.nlo * J.nhi;
-  double* rec = J.fac + tid;
+  Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.fac) + tid;
   cxd rd_prev = cinv(G.d_edge(L, 0));
-  rec[20 * stride] = rd_prev.re;
-  rec[21 * stride] = rd_prev.im;
+  {
+    // node 0 holds only 1/d_0 (values 20, 21: one chunk)
+    Ch<R> c[FC] = {};
+    put(c, 20, rd_prev.re);
+    put(c, 21, rd_prev.im);
+    rec[(20 / V) * stride] = c[20 / V];
+  }
   cxd Wi[4][4];
   for (int p = 1; p <= G.N - 1; p++) {
     const cxd rd = cinv(G.d_edge(L, p));
@@ -173,38 +220,42 @@ __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
 #pragma unroll
       for (int j = 0; j < 4; j++) Wi[i][j] = D[i][j];
     inv4(Wi);
-    double* r = rec + (long)p * FREC * stride;
+    Ch<R> c[FC] = {};
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = i; j < 4; j++) {
         const int sidx = symidx(i, j);
         const cxd w = (i == j) ? Wi[i][j] : (Wi[i][j] + Wi[j][i]) * 0.5;
-        r[(2 * sidx) * stride] = w.re;
-        r[(2 * sidx + 1) * stride] = w.im;
+        put(c, 2 * sidx, w.re);
+        put(c, 2 * sidx + 1, w.im);
       }
-    r[20 * stride] = rd.re;
-    r[21 * stride] = rd.im;
+    put(c, 20, rd.re);
+    put(c, 21, rd.im);
+    Ch<R>* r = rec + (long)p * FC * stride;
+#pragma unroll
+    for (int k = 0; k < FC; k++) r[k * stride] = c[k];
     rd_prev = rd;
   }
 }
 
-// z = W^{-1} y with the packed symmetric inverse held in w[0..19]
-__device__ __forceinline__ void symv(const double (&w)[FREC], const cxd (&y)[4], cxd (&z)[4]) {
+// z = W^{-1} y with the packed symmetric inverse held in values 0..19
+template <typename R, int NC>
+__device__ __forceinline__ void symv(const Ch<R> (&w)[NC], const cxd (&y)[4], cxd (&z)[4]) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     cxd acc(0, 0);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int s = symidx(i, j);
-      acc += cxd(w[2 * s], w[2 * s + 1]) * y[j];
+      acc += cxd(val(w, 2 * s), val(w, 2 * s + 1)) * y[j];
     }
     z[i] = acc;
   }
 }
 
 // Right-hand sides of the lines of one colour: one thread per (node, line),
-// lines fastest, so the records [p][SREC][nlines] are written contiguously.
+// lines fastest, so the chunked records are written contiguously.
 //   record p, slots 0..7: sigma_p, the residual of the four perpendicular
 //                          edges of node p (p >= 1)
 //   record p, slots 8..9: rho_p, the residual of the along-line edge p
@@ -226,13 +277,16 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
   const typename cvec<T>::type* __restrict__ u = J.u;
   const typename cvec<T>::type* __restrict__ f = J.f;
   const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
-  double* rec = J.scr + line + (long)p * SREC * nl;
+  using R = Rec<T>;
+  constexpr int SC = schunks<R>(), V = Ch<R>::V;
+  Ch<R> c[SC] = {};
+  Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
   {
     int q[3];
     q[A] = p; q[B] = pb; q[C] = pc;
     const cxd rho = ld<T>(f, A * L.nc + base) - apply_row<T, A>(L, u, q[A], q[B], q[C], base);
-    rec[8 * nl] = rho.re;
-    rec[9 * nl] = rho.im;
+    put(c, 8, rho.re);
+    put(c, 9, rho.im);
   }
   if (p >= 1) {
     int q[3];
@@ -240,16 +294,20 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
     const long ob0 = base - L.es[B], oc0 = base - L.es[C];
     q[A] = p; q[B] = pb - 1; q[C] = pc;
     g = ld<T>(f, B * L.nc + ob0) - apply_row<T, B>(L, u, q[B], q[C], q[A], ob0);
-    rec[0 * nl] = g.re; rec[1 * nl] = g.im;
+    put(c, 0, g.re); put(c, 1, g.im);
     q[B] = pb;
     g = ld<T>(f, B * L.nc + base) - apply_row<T, B>(L, u, q[B], q[C], q[A], base);
-    rec[2 * nl] = g.re; rec[3 * nl] = g.im;
+    put(c, 2, g.re); put(c, 3, g.im);
     q[C] = pc - 1;
     g = ld<T>(f, C * L.nc + oc0) - apply_row<T, C>(L, u, q[C], q[A], q[B], oc0);
-    rec[4 * nl] = g.re; rec[5 * nl] = g.im;
+    put(c, 4, g.re); put(c, 5, g.im);
     q[C] = pc;
     g = ld<T>(f, C * L.nc + base) - apply_row<T, C>(L, u, q[C], q[A], q[B], base);
-    rec[6 * nl] = g.re; rec[7 * nl] = g.im;
+    put(c, 6, g.re); put(c, 7, g.im);
+#pragma unroll
+    for (int k = 0; k < SC; k++) rec[k * nl] = c[k];
+  } else {
+    rec[(8 / V) * nl] = c[8 / V];   // node 0: rho_0 only (values 8, 9: one chunk)
   }
 }
 
@@ -316,18 +374,35 @@ __global__ void __launch_bounds__(256) k_line_rhs_x(LinePack<T> P, int nlb) {
     const int p = p0 + pl;
     const long line = l0 + ll;
     if (p < N && line < nl) {
-      double* rec = J.scr + line + (long)p * SREC * nl;
+      using R = Rec<T>;
+      constexpr int SC = schunks<R>(), V = Ch<R>::V;
+      Ch<R> c[SC] = {};
 #pragma unroll
-      for (int k = 0; k < SREC; k++) rec[k * nl] = tile[k][pl][ll];
+      for (int k = 0; k < SREC; k++) put(c, k, tile[k][pl][ll]);
+      Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
+      if (p >= 1) {
+#pragma unroll
+        for (int k = 0; k < SC; k++) rec[k * nl] = c[k];
+      } else {
+        rec[(8 / V) * nl] = c[8 / V];
+      }
     }
   }
 }
 
 // Block substitution of the lines of one colour, one thread per line.  The
-// records of node p+1 (forward) or p-1 (backward) are loaded before node p is
-// processed, so their memory latency overlaps the arithmetic of node p.
+// chunks of the next D nodes stay in flight in a register ring: node p's ring
+// entry is refilled with node p+D (forward) or p-D (backward) as soon as node
+// p has been taken from it, so D nodes of memory latency overlap the
+// arithmetic.  Ring loads past the end of the line are clamped to its last
+// node and never used.  The forward pass overwrites sigma_p with z_p and keeps
+// rho_p, from which the backward pass recomputes q_p = rho_p / d_p.
 template <typename T, int A>
 __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
+  using E = typename cvec<T>::type;
+  using R = Rec<T>;
+  constexpr int FC = fchunks<R>(), SC = schunks<R>(), V = Ch<R>::V, D = ring_depth<T>();
+  constexpr int ZC = 8 / V;   // chunks holding values 0..7 (sigma, then z)
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,116 +410,144 @@ __global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
   if (!line_of<T, A>(J, tid, pb, pc)) return;
   const LineGeo<T, A> G(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
-  const double* __restrict__ fac = J.fac + tid;
-  double* __restrict__ scr = J.scr + tid;
-  typename cvec<T>::type* __restrict__ u = J.u;
+  const Ch<R>* __restrict__ fac = reinterpret_cast<const Ch<R>*>(J.fac) + tid;
+  Ch<R>* __restrict__ scr = reinterpret_cast<Ch<R>*>(J.scr) + tid;
+  E* __restrict__ u = J.u;
+  const T* __restrict__ iw = L.IW[A];
   const int N = G.N;
+  if (N < 2) return;   // no interior node (levels keep at least 2 cells per axis)
+  const long fstep = (long)FC * stride, sstep = (long)SC * stride;
+
+  // node 0: 1/d_0 and rho_0
+  const Ch<R> f0 = fac[(20 / V) * stride];
+  const Ch<R> s0 = scr[(8 / V) * stride];
+  const cxd rd0((double)f0.v[20 % V], (double)f0.v[21 % V]);
+  const cxd q0 = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd0;
+
+  Ch<R> rf[D][FC], rs[D][SC];
+  T riw[D];
 
   // ---- forward substitution: z_p = W_p^-1 (g_p - B_p z_{p-1}) ----
-  cxd rd_prev(fac[20 * stride], fac[21 * stride]);
-  cxd q_prev = cxd(scr[8 * stride], scr[9 * stride]) * rd_prev;
-  scr[8 * stride] = q_prev.re;
-  scr[9 * stride] = q_prev.im;
-  double cf[FREC], cr[SREC];
 #pragma unroll
-  for (int k = 0; k < FREC; k++) cf[k] = fac[((long)1 * FREC + k) * stride];
-#pragma unroll
-  for (int k = 0; k < SREC; k++) cr[k] = scr[((long)1 * SREC + k) * stride];
+  for (int d = 0; d < D; d++) {
+    const int pn = min(1 + d, N - 1);
+    load_node(rf[d], fac + pn * fstep, stride);
+    load_node(rs[d], scr + pn * sstep, stride);
+    riw[d] = iw[pn - 1];
+  }
+  cxd rd_prev = rd0, q_prev = q0;
   cxd z[4];
-  for (int p = 1; p <= N - 1; p++) {
-    double nf[FREC], nr[SREC];
-    if (p + 1 <= N - 1) {
+  for (int p0 = 1; p0 <= N - 1; p0 += D) {
 #pragma unroll
-      for (int k = 0; k < FREC; k++) nf[k] = fac[((long)(p + 1) * FREC + k) * stride];
+    for (int d = 0; d < D; d++) {
+      const int p = p0 + d;
+      if (p <= N - 1) {
+        Ch<R> cf[FC], cr[SC];
 #pragma unroll
-      for (int k = 0; k < SREC; k++) nr[k] = scr[((long)(p + 1) * SREC + k) * stride];
+        for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+#pragma unroll
+        for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+        const double iWa = (double)riw[d];
+        {
+          const int pn = min(p + D, N - 1);
+          load_node(rf[d], fac + pn * fstep, stride);
+          load_node(rs[d], scr + pn * sstep, stride);
+          riw[d] = iw[pn - 1];
+        }
+        const cxd rd(val(cf, 20), val(cf, 21));
+        const cxd q = cxd(val(cr, 8), val(cr, 9)) * rd;
+        const cxd dq = q_prev - q;
+        cxd g[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) g[i] = cxd(val(cr, 2 * i), val(cr, 2 * i + 1)) + dq * G.v[i];
+        if (p >= 2) {
+          // B_p z_{p-1} with l = lcoef(p - 1)
+          const double l[4] = {-G.Wb0 * G.Hc * iWa, -G.Wb1 * G.Hc * iWa, -G.Wc0 * G.Hb * iWa, -G.Wc1 * G.Hb * iWa};
+          const cxd vz = z[0] * G.v[0] + z[1] * G.v[1] + z[2] * G.v[2] + z[3] * G.v[3];
+#pragma unroll
+          for (int i = 0; i < 4; i++) g[i] -= z[i] * l[i] + (vz * rd_prev) * G.v[i];
+        }
+        symv(cf, g, z);
+        Ch<R> zc[SC] = {};
+#pragma unroll
+        for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
+        Ch<R>* sp = scr + p * sstep;
+#pragma unroll
+        for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
+        rd_prev = rd;
+        q_prev = q;
+      }
     }
-    const cxd rd(cf[20], cf[21]);
-    const cxd q = cxd(cr[8], cr[9]) * rd;
-    const cxd dq = q_prev - q;
-    cxd g[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) g[i] = cxd(cr[2 * i], cr[2 * i + 1]) + dq * G.v[i];
-    if (p >= 2) {
-      double l[4];
-      G.lcoef(L, p - 1, l);
-      const cxd vz = z[0] * G.v[0] + z[1] * G.v[1] + z[2] * G.v[2] + z[3] * G.v[3];
-#pragma unroll
-      for (int i = 0; i < 4; i++) g[i] -= z[i] * l[i] + (vz * rd_prev) * G.v[i];
-    }
-    symv(cf, g, z);
-    double* s = scr + (long)p * SREC * stride;
-#pragma unroll
-    for (int i = 0; i < 4; i++) { s[(2 * i) * stride] = z[i].re; s[(2 * i + 1) * stride] = z[i].im; }
-    s[8 * stride] = q.re;
-    s[9 * stride] = q.im;
-    rd_prev = rd;
-    q_prev = q;
-#pragma unroll
-    for (int k = 0; k < FREC; k++) cf[k] = nf[k];
-#pragma unroll
-    for (int k = 0; k < SREC; k++) cr[k] = nr[k];
   }
 
   // ---- back substitution: da_p = z_p - W_p^-1 B_{p+1} da_{p+1}; update ----
+  const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
+  E ru[D][5];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int pn = max(N - 1 - d, 1);
+    load_node(rf[d], fac + pn * fstep, stride);
+    load_node(rs[d], scr + pn * sstep, stride);
+    riw[d] = iw[pn];
+#pragma unroll
+    for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+  }
   cxd an[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) an[i] = cxd(0, 0);
-  const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
-  cxd cu[5];
+  for (int p0 = N - 1; p0 >= 1; p0 -= D) {
 #pragma unroll
-  for (int k = 0; k < FREC; k++) cf[k] = fac[((long)(N - 1) * FREC + k) * stride];
+    for (int d = 0; d < D; d++) {
+      const int p = p0 - d;
+      if (p >= 1) {
+        Ch<R> cf[FC], cr[SC];
+        E cu[5];
 #pragma unroll
-  for (int k = 0; k < SREC; k++) cr[k] = scr[((long)(N - 1) * SREC + k) * stride];
+        for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
-  for (int k = 0; k < 5; k++) cu[k] = ld<T>(u, off_u[k] + (long)(N - 1) * G.sa);
-  for (int p = N - 1; p >= 1; p--) {
-    double nf[FREC], nr[SREC];
-    cxd nu[5];
-    if (p - 1 >= 1) {
+        for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
 #pragma unroll
-      for (int k = 0; k < FREC; k++) nf[k] = fac[((long)(p - 1) * FREC + k) * stride];
+        for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+        const double iWa = (double)riw[d];
+        {
+          const int pn = max(p - D, 1);
+          load_node(rf[d], fac + pn * fstep, stride);
+          load_node(rs[d], scr + pn * sstep, stride);
+          riw[d] = iw[pn];
 #pragma unroll
-      for (int k = 0; k < SREC; k++) nr[k] = scr[((long)(p - 1) * SREC + k) * stride];
+          for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+        }
+        const cxd rd(val(cf, 20), val(cf, 21));
+        const cxd q = cxd(val(cr, 8), val(cr, 9)) * rd;
+        cxd ap[4];
 #pragma unroll
-      for (int k = 0; k < 5; k++) nu[k] = ld<T>(u, off_u[k] + (long)(p - 1) * G.sa);
+        for (int i = 0; i < 4; i++) ap[i] = cxd(val(cr, 2 * i), val(cr, 2 * i + 1));
+        if (p < N - 1) {
+          // W_p^-1 B_{p+1} da_{p+1} with l = lcoef(p)
+          const double l[4] = {-G.Wb0 * G.Hc * iWa, -G.Wb1 * G.Hc * iWa, -G.Wc0 * G.Hb * iWa, -G.Wc1 * G.Hb * iWa};
+          const cxd va = an[0] * G.v[0] + an[1] * G.v[1] + an[2] * G.v[2] + an[3] * G.v[3];
+          cxd t[4], wt[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) t[i] = an[i] * l[i] + (va * rd) * G.v[i];
+          symv(cf, t, wt);
+#pragma unroll
+          for (int i = 0; i < 4; i++) ap[i] -= wt[i];
+        }
+        cxd dv(0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
+        const long o = (long)p * G.sa;
+        const cxd e0 = cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd);
+        st<T>(u, G.oa + o, e0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) an[i] = ap[i];
+      }
     }
-    const cxd rd(cf[20], cf[21]);
-    const cxd q(cr[8], cr[9]);
-    cxd ap[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) ap[i] = cxd(cr[2 * i], cr[2 * i + 1]);
-    if (p < N - 1) {
-      double l[4];
-      G.lcoef(L, p, l);
-      const cxd va = an[0] * G.v[0] + an[1] * G.v[1] + an[2] * G.v[2] + an[3] * G.v[3];
-      cxd t[4], wt[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) t[i] = an[i] * l[i] + (va * rd) * G.v[i];
-      symv(cf, t, wt);
-#pragma unroll
-      for (int i = 0; i < 4; i++) ap[i] -= wt[i];
-    }
-    cxd dv(0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
-    const long o = (long)p * G.sa;
-    st<T>(u, G.oa + o, cu[0] + (q + dv * rd));
-#pragma unroll
-    for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cu[i + 1] + ap[i]);
-#pragma unroll
-    for (int i = 0; i < 4; i++) an[i] = ap[i];
-#pragma unroll
-    for (int k = 0; k < FREC; k++) cf[k] = nf[k];
-#pragma unroll
-    for (int k = 0; k < SREC; k++) cr[k] = nr[k];
-#pragma unroll
-    for (int k = 0; k < 5; k++) cu[k] = nu[k];
   }
   {
     // along-line edge 0, between the boundary node and node 1
-    const cxd q0(scr[8 * stride], scr[9 * stride]);
-    const cxd rd0(fac[20 * stride], fac[21 * stride]);
     cxd dv(0, 0);
 #pragma unroll
     for (int i = 0; i < 4; i++) dv += an[i] * G.v[i];
@@ -463,16 +566,21 @@ void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi) {
   if (b < c) { *nlo = nbv; *nhi = ncv; } else { *nlo = ncv; *nhi = nbv; }
 }
 
-size_t line_factor_doubles(const int* n, int a, int colour) {
-  int nlo, nhi;
-  line_colour_counts(n, a, colour, &nlo, &nhi);
-  return (size_t)nlo * nhi * (size_t)n[a] * FREC;
+static size_t chunks_of(int values, size_t esize) {
+  const size_t v = 16 / esize;
+  return (values + v - 1) / v;
 }
 
-size_t line_scratch_doubles(const int* n, int a, int colour) {
+size_t line_factor_bytes(const int* n, int a, int colour, size_t esize) {
   int nlo, nhi;
   line_colour_counts(n, a, colour, &nlo, &nhi);
-  return (size_t)nlo * nhi * (size_t)n[a] * SREC;
+  return (size_t)nlo * nhi * (size_t)n[a] * chunks_of(FREC, esize) * 16;
+}
+
+size_t line_scratch_bytes(const int* n, int a, int colour, size_t esize) {
+  int nlo, nhi;
+  line_colour_counts(n, a, colour, &nlo, &nhi);
+  return (size_t)nlo * nhi * (size_t)n[a] * chunks_of(SREC, esize) * 16;
 }
 
 template <typename T>
@@ -489,9 +597,9 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     const double rows = (double)nl * (5.0 * na - 4.0);
     // right-hand sides: the colour's rows read f and sigma-bar once, u is
     // read once over the level, the records are written once
-    rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * 8.0;
+    rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * sizeof(Rec<T>);
     // substitution: records and factors read once, the rows of u read and written
-    solve_bytes += (double)nl * na * (SREC + FREC) * 8.0 + rows * 2.0 * es;
+    solve_bytes += (double)nl * na * (SREC + FREC) * sizeof(Rec<T>) + rows * 2.0 * es;
   }
   if (maxl == 0 || P.njobs == 0) return;
   if (factor) {

@@ -70,7 +70,7 @@ struct Hier {
 
 // factored line systems of one (level, axis), per colour, for one frequency
 struct LineFactors {
-  double* fac[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* fac[4] = {nullptr, nullptr, nullptr, nullptr};  // values of the cycle precision
   double omu = -1.0;   // frequency the records were built for
 };
 
@@ -82,7 +82,7 @@ struct Slot {
   void* f64 = nullptr;
   void* r64 = nullptr;
   std::vector<LineFactors> facs;     // [(pattern*lmax + level)*3 + axis]; level 0 under pattern 0
-  double* lscr = nullptr;            // substitution scratch of the line solves
+  void* lscr = nullptr;              // right-hand side / substitution records of the line solves
   double* partial = nullptr;         // residual-norm partial sums
   double rnorm0 = 0;
   SolveStats* st = nullptr;
@@ -165,7 +165,7 @@ class Solver {
   Hier<float> h32_;
   DevLevel<double> l0_64_;    // level-0 FP64 operator of the refinement
   std::vector<Slot> slots_;
-  size_t lscr_doubles_ = 0;
+  size_t lscr_bytes_ = 0;
   double* dnorm_ = nullptr;   // one per slot
   double* hnorm_ = nullptr;   // pinned host, one per slot
   std::vector<void*> owned_;

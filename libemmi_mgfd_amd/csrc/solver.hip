@@ -193,12 +193,13 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
     l0_64_ = one.pat[0][0];
   }
 
-  // scratch of the line solves: max over levels, patterns, axes and colours
+  // records of the line solves: max over levels, patterns, axes and colours
+  const size_t esize = p_.precision == 0 ? sizeof(LineRec<double>::type) : sizeof(LineRec<float>::type);
   for (int c = 0; c < npat_; c++)
     for (int lev = 0; lev < lmax_; lev++)
       for (int a = 0; a < 3; a++)
         for (int col = 0; col < 4; col++)
-          lscr_doubles_ = std::max(lscr_doubles_, line_scratch_doubles(dims_[c][lev].data(), a, col));
+          lscr_bytes_ = std::max(lscr_bytes_, line_scratch_bytes(dims_[c][lev].data(), a, col, esize));
 
   slots_.resize(nslots);
   for (int k = 0; k < nslots; k++) {
@@ -217,7 +218,7 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
       S.r64 = dalloc(b, owned_);
     }
     S.facs.assign((size_t)npat_ * lmax_ * 3, LineFactors());
-    S.lscr = (double*)dalloc(sizeof(double) * std::max<size_t>(1, lscr_doubles_), owned_);
+    S.lscr = dalloc(std::max<size_t>(16, lscr_bytes_), owned_);
     S.partial = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
   }
   dnorm_ = (double*)dalloc(sizeof(double) * nslots, owned_);
@@ -330,6 +331,9 @@ bool Solver::exact_coarsest(int pat) const {
 // and rebuilt when the slot's frequency changed since they were made
 template <typename T>
 LineFactors& Solver::factors(Slot& S, int pat, int lev, int axis) {
+  // records are sized for the cycle precision of this solver
+  if ((sizeof(T) == sizeof(double)) != (p_.precision == 0))
+    throw std::runtime_error("line solve precision differs from the solver's cycle precision");
   const int pe = lev == 0 ? 0 : pat;
   LineFactors& F = S.facs[((size_t)pe * lmax_ + lev) * 3 + axis];
   if (F.omu == S.omu) return F;
@@ -337,12 +341,12 @@ LineFactors& Solver::factors(Slot& S, int pat, int lev, int axis) {
   LinePack<T> P;
   P.njobs = 4;
   for (int col = 0; col < 4; col++) {
-    if (!F.fac[col]) F.fac[col] = (double*)dalloc(sizeof(double) * std::max<size_t>(1, line_factor_doubles(L.n, axis, col)), owned_);
+    if (!F.fac[col]) F.fac[col] = dalloc(std::max<size_t>(16, line_factor_bytes(L.n, axis, col, sizeof(typename LineRec<T>::type))), owned_);
     LineJob<T>& J = P.job[col];
     J.L = L.view(S.omu);
     J.u = nullptr;
     J.f = nullptr;
-    J.fac = F.fac[col];
+    J.fac = (typename LineRec<T>::type*)F.fac[col];
     J.scr = nullptr;
     line_job_colour(J, axis, col);
   }
@@ -362,10 +366,10 @@ void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const 
   J.L = L.view(S.omu);
   J.u = (typename cvec<T>::type*)u;
   J.f = (const typename cvec<T>::type*)f;
-  J.scr = S.lscr;
+  J.scr = (typename LineRec<T>::type*)S.lscr;
   for (int k = 0; k < 4; k++) {
     const int colour = (iter % 2 == 1) ? 3 - k : k;
-    J.fac = F.fac[colour];
+    J.fac = (typename LineRec<T>::type*)F.fac[colour];
     line_job_colour(J, axis, colour);
     launch_line_solve<T>(P, axis, s_);
   }
@@ -392,8 +396,8 @@ void Solver::line_sweep(int pat, int lev, int axis, int iter) {
         J.L = L.view(S.omu);
         J.u = (typename cvec<T>::type*)S.u[lev];
         J.f = (const typename cvec<T>::type*)S.f[lev];
-        J.fac = F[i0 + j]->fac[colour];
-        J.scr = S.lscr;
+        J.fac = (typename LineRec<T>::type*)F[i0 + j]->fac[colour];
+        J.scr = (typename LineRec<T>::type*)S.lscr;
         line_job_colour(J, axis, colour);
       }
       launch_line_solve<T>(P, axis, s_);
