@@ -105,7 +105,9 @@ struct LineGeo {
   double Hb, Hc, Wb0, Wb1, Wc0, Wc1, sIb, sIc, dre_a;
   double v[4];
 
-  __device__ __forceinline__ LineGeo(const Level<T>& L, int pb_, int pc_) {
+  __device__ __forceinline__ LineGeo() {}
+  __device__ __forceinline__ LineGeo(const Level<T>& L, int pb_, int pc_) { init(L, pb_, pc_); }
+  __device__ __forceinline__ void init(const Level<T>& L, int pb_, int pc_) {
     pb = pb_;
     pc = pc_;
     N = L.n[A];
@@ -149,93 +151,148 @@ __device__ __forceinline__ bool line_of(const LineJob<T>& J, int tid, int& pb, i
   return true;
 }
 
+// D_p: the diagonal block of node p after the along-line edges p-1 and p
+// (with 1/d = rdm, rdp) are eliminated
+template <typename T, int A>
+__device__ __forceinline__ void diag_block(const Level<T>& L, const LineGeo<T, A>& G, int p, cxd rdm, cxd rdp,
+                                           cxd (&D)[4][4]) {
+  const double Ha = L.H[A][p];
+  const double iWa0 = L.IW[A][p - 1], iWa1 = L.IW[A][p];
+  const double Gb = Ha * G.sIc + G.Hc * (iWa1 + iWa0);
+  const double Gc = G.Hb * (iWa1 + iWa0) + Ha * G.sIb;
+  const long off = p * G.sa;
+  D[0][0] = cxd(G.Wb0 * Gb, -L.omu * (double)L.sbar[G.ob0 + off]);
+  D[1][1] = cxd(G.Wb1 * Gb, -L.omu * (double)L.sbar[G.ob1 + off]);
+  D[2][2] = cxd(G.Wc0 * Gc, -L.omu * (double)L.sbar[G.oc0 + off]);
+  D[3][3] = cxd(G.Wc1 * Gc, -L.omu * (double)L.sbar[G.oc1 + off]);
+  D[0][1] = D[1][0] = cxd(0, 0);
+  D[2][3] = D[3][2] = cxd(0, 0);
+  D[0][2] = D[2][0] = cxd(-Ha, 0);
+  D[0][3] = D[3][0] = cxd(Ha, 0);
+  D[1][2] = D[2][1] = cxd(Ha, 0);
+  D[1][3] = D[3][1] = cxd(-Ha, 0);
+  const cxd s = rdm + rdp;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) D[i][j] -= s * (G.v[i] * G.v[j]);
+}
+
+// D -= B Wi B for the coupling B = diag(l) + v v^T rd of two neighbouring
+// nodes (Wi: the inverse factor of the neighbour)
+__device__ __forceinline__ void sub_bwb(cxd (&D)[4][4], const cxd (&Wi)[4][4], const double (&l)[4], cxd rd,
+                                        const double (&v)[4]) {
+  cxd wv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) wv[i] = Wi[i][0] * v[0] + Wi[i][1] * v[1] + Wi[i][2] * v[2] + Wi[i][3] * v[3];
+  cxd Pm[4][4];   // Wi B
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) Pm[i][j] = Wi[i][j] * l[j] + (wv[i] * rd) * v[j];
+  cxd vP[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) vP[j] = Pm[0][j] * v[0] + Pm[1][j] * v[1] + Pm[2][j] * v[2] + Pm[3][j] * v[3];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) D[i][j] -= Pm[i][j] * l[i] + (vP[j] * rd) * v[i];
+}
+
+// inverts D in place and stores the symmetric part with 1/d of node p
+template <typename R>
+__device__ __forceinline__ void store_factor(Ch<R>* rec, long stride, int p, cxd (&D)[4][4], cxd rd) {
+  constexpr int FC = fchunks<R>();
+  inv4(D);
+  Ch<R> c[FC] = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = i; j < 4; j++) {
+      const int sidx = symidx(i, j);
+      const cxd w = (i == j) ? D[i][j] : (D[i][j] + D[j][i]) * 0.5;
+      put(c, 2 * sidx, w.re);
+      put(c, 2 * sidx + 1, w.im);
+    }
+  put(c, 20, rd.re);
+  put(c, 21, rd.im);
+  Ch<R>* r = rec + (long)p * FC * stride;
+#pragma unroll
+  for (int k = 0; k < FC; k++) r[k * stride] = c[k];
+}
+
+// Twisted factorization of each line, meeting at node m = N/2 of the nodes
+// 1..M (M = N-1):
+//   top     W_1 = D_1,  W_p = D_p - B_p W_{p-1}^-1 B_p        p = 2..m-1
+//   bottom  U_M = D_M,  U_p = D_p - B_{p+1} U_{p+1}^-1 B_{p+1}  p = M-1..m+1
+//   middle  S_m = D_m - B_m W_{m-1}^-1 B_m - B_{m+1} U_{m+1}^-1 B_{m+1}
+// Node p stores the inverse of its factor (W, S or U) and 1/d_p; node 0
+// stores 1/d_0.
 template <typename T, int A>
 __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
+  using R = Rec<T>;
+  constexpr int FC = fchunks<R>(), V = Ch<R>::V;
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int pb, pc;
   if (!line_of<T, A>(J, tid, pb, pc)) return;
   const LineGeo<T, A> G(L, pb, pc);
-  using R = Rec<T>;
-  constexpr int FC = fchunks<R>(), V = Ch<R>::V;
+  const int N = G.N;
+  if (N < 2) return;
+  const int M = N - 1, m = N / 2;
   const long stride = (long)J.nlo * J.nhi;
   Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.fac) + tid;
-  cxd rd_prev = cinv(G.d_edge(L, 0));
+  const cxd rd0 = cinv(G.d_edge(L, 0));
   {
     // node 0 holds only 1/d_0 (values 20, 21: one chunk)
     Ch<R> c[FC] = {};
-    put(c, 20, rd_prev.re);
-    put(c, 21, rd_prev.im);
+    put(c, 20, rd0.re);
+    put(c, 21, rd0.im);
     rec[(20 / V) * stride] = c[20 / V];
   }
-  cxd Wi[4][4];
-  for (int p = 1; p <= G.N - 1; p++) {
+  cxd Wt[4][4], Wb[4][4], Dm[4][4];
+  double l[4];
+  cxd rd_prev = rd0;
+  for (int p = 1; p <= m - 1; p++) {
     const cxd rd = cinv(G.d_edge(L, p));
-    const double Ha = L.H[A][p];
-    const double iWa0 = L.IW[A][p - 1], iWa1 = L.IW[A][p];
-    const double Gb = Ha * G.sIc + G.Hc * (iWa1 + iWa0);
-    const double Gc = G.Hb * (iWa1 + iWa0) + Ha * G.sIb;
-    const long off = p * G.sa;
-    cxd D[4][4];
-    D[0][0] = cxd(G.Wb0 * Gb, -L.omu * (double)L.sbar[G.ob0 + off]);
-    D[1][1] = cxd(G.Wb1 * Gb, -L.omu * (double)L.sbar[G.ob1 + off]);
-    D[2][2] = cxd(G.Wc0 * Gc, -L.omu * (double)L.sbar[G.oc0 + off]);
-    D[3][3] = cxd(G.Wc1 * Gc, -L.omu * (double)L.sbar[G.oc1 + off]);
-    D[0][1] = D[1][0] = cxd(0, 0);
-    D[2][3] = D[3][2] = cxd(0, 0);
-    D[0][2] = D[2][0] = cxd(-Ha, 0);
-    D[0][3] = D[3][0] = cxd(Ha, 0);
-    D[1][2] = D[2][1] = cxd(Ha, 0);
-    D[1][3] = D[3][1] = cxd(-Ha, 0);
-    {
-      const cxd s = rd_prev + rd;
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) D[i][j] -= s * (G.v[i] * G.v[j]);
-    }
+    diag_block(L, G, p, rd_prev, rd, Dm);
     if (p >= 2) {
-      double l[4];
       G.lcoef(L, p - 1, l);
-      // P = W_{p-1}^{-1} B_p ; D -= B_p P
-      cxd wv[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) wv[i] = Wi[i][0] * G.v[0] + Wi[i][1] * G.v[1] + Wi[i][2] * G.v[2] + Wi[i][3] * G.v[3];
-      cxd Pm[4][4];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) Pm[i][j] = Wi[i][j] * l[j] + (wv[i] * rd_prev) * G.v[j];
-      cxd vP[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) vP[j] = Pm[0][j] * G.v[0] + Pm[1][j] * G.v[1] + Pm[2][j] * G.v[2] + Pm[3][j] * G.v[3];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) D[i][j] -= Pm[i][j] * l[i] + (vP[j] * rd_prev) * G.v[i];
+      sub_bwb(Dm, Wt, l, rd_prev, G.v);
     }
+    store_factor(rec, stride, p, Dm, rd);
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) Wi[i][j] = D[i][j];
-    inv4(Wi);
-    Ch<R> c[FC] = {};
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = i; j < 4; j++) {
-        const int sidx = symidx(i, j);
-        const cxd w = (i == j) ? Wi[i][j] : (Wi[i][j] + Wi[j][i]) * 0.5;
-        put(c, 2 * sidx, w.re);
-        put(c, 2 * sidx + 1, w.im);
-      }
-    put(c, 20, rd.re);
-    put(c, 21, rd.im);
-    Ch<R>* r = rec + (long)p * FC * stride;
-#pragma unroll
-    for (int k = 0; k < FC; k++) r[k * stride] = c[k];
+      for (int j = 0; j < 4; j++) Wt[i][j] = Dm[i][j];
     rd_prev = rd;
+  }
+  for (int p = M; p >= m + 1; p--) {
+    const cxd rdm = cinv(G.d_edge(L, p - 1)), rdp = cinv(G.d_edge(L, p));
+    diag_block(L, G, p, rdm, rdp, Dm);
+    if (p <= M - 1) {
+      G.lcoef(L, p, l);
+      sub_bwb(Dm, Wb, l, rdp, G.v);
+    }
+    store_factor(rec, stride, p, Dm, rdp);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) Wb[i][j] = Dm[i][j];
+  }
+  {
+    const cxd rdm = cinv(G.d_edge(L, m - 1)), rdp = cinv(G.d_edge(L, m));
+    diag_block(L, G, m, rdm, rdp, Dm);
+    if (m >= 2) {
+      G.lcoef(L, m - 1, l);
+      sub_bwb(Dm, Wt, l, rdm, G.v);
+    }
+    if (m <= M - 1) {
+      G.lcoef(L, m, l);
+      sub_bwb(Dm, Wb, l, rdp, G.v);
+    }
+    store_factor(rec, stride, m, Dm, rdp);
   }
 }
 
@@ -390,168 +447,351 @@ __global__ void __launch_bounds__(256) k_line_rhs_x(LinePack<T> P, int nlb) {
   }
 }
 
-// Block substitution of the lines of one colour, one thread per line.  The
-// chunks of the next D nodes stay in flight in a register ring: node p's ring
-// entry is refilled with node p+D (forward) or p-D (backward) as soon as node
-// p has been taken from it, so D nodes of memory latency overlap the
-// arithmetic.  Ring loads past the end of the line are clamped to its last
-// node and never used.  The forward pass overwrites sigma_p with z_p and keeps
-// rho_p, from which the backward pass recomputes q_p = rho_p / d_p.
+// t = B z for the coupling B = diag(l) + v v^T rd of two neighbouring nodes
 template <typename T, int A>
-__global__ void __launch_bounds__(64) k_line_solve(LinePack<T> P) {
+__device__ __forceinline__ void bmul(const LineGeo<T, A>& G, const cxd (&z)[4], double iWe, cxd rd, cxd (&t)[4]) {
+  const double l[4] = {-G.Wb0 * G.Hc * iWe, -G.Wb1 * G.Hc * iWe, -G.Wc0 * G.Hb * iWe, -G.Wc1 * G.Hb * iWe};
+  const cxd vz = z[0] * G.v[0] + z[1] * G.v[1] + z[2] * G.v[2] + z[3] * G.v[3];
+#pragma unroll
+  for (int i = 0; i < 4; i++) t[i] = z[i] * l[i] + (vz * rd) * G.v[i];
+}
+
+template <typename R, int NC>
+__device__ __forceinline__ cxd cval(const Ch<R> (&c)[NC], int k) { return cxd(val(c, k), val(c, k + 1)); }
+
+// Block substitution of the lines of one colour on the twisted factors
+// (k_line_factor).  A block of 128 threads takes 64 lines: wave 0 ("top")
+// eliminates nodes 1..m-1 downwards and wave 1 ("bottom") nodes M..m+1
+// upwards at the same time,
+//   top     z_p = W_p^-1 (g_p - B_p z_{p-1})
+//   bottom  z_p = U_p^-1 (g_p - B_{p+1} z_{p+1}),
+// the waves exchange B_m z_{m-1} and B_{m+1} z_{m+1} through LDS, both solve
+// the meeting node x_m = S_m^-1 (g_m - B_m z_{m-1} - B_{m+1} z_{m+1}), and
+// each substitutes back over its half,
+//   top     x_p = z_p - W_p^-1 B_{p+1} x_{p+1}     p = m-1..1
+//   bottom  x_p = z_p - U_p^-1 B_p x_{p-1}         p = m+1..M,
+// updating the along-line edges dE_a(e) = q_e + v.(x_{e+1} - x_e) / d_e
+// (top: edges 0..m-1, bottom: m..N-1; q_e = rho_e / d_e) and the node
+// edges.  g_p = sigma_p + v (q_{p-1} - q_p).  Each wave keeps the chunks of
+// its next D nodes in flight in a register ring; ring loads past the end of
+// a half are clamped into the line and never used.  The forward passes
+// overwrite sigma_p with z_p.
+template <typename T, int A>
+__global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   using E = typename cvec<T>::type;
   using R = Rec<T>;
   constexpr int FC = fchunks<R>(), SC = schunks<R>(), V = Ch<R>::V, D = ring_depth<T>();
-  constexpr int ZC = 8 / V;   // chunks holding values 0..7 (sigma, then z)
+  constexpr int ZC = 8 / V;    // chunks holding values 0..7 (sigma, then z)
+  constexpr int CD = 20 / V;   // factor chunk holding 1/d (values 20, 21)
+  constexpr int CQ = 8 / V;    // record chunk holding rho (values 8, 9)
+  __shared__ double xch[2][8][64];
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  int pb, pc;
-  if (!line_of<T, A>(J, tid, pb, pc)) return;
-  const LineGeo<T, A> G(L, pb, pc);
+  const int N = L.n[A];
+  if (N < 2) return;   // uniform over the block: no interior node
+  const int M = N - 1, m = N / 2;
+  const int lane = threadIdx.x & 63, side = threadIdx.x >> 6;
+  const int line = blockIdx.x * 64 + lane;
+  int pb = 0, pc = 0;
+  const bool live = line_of<T, A>(J, line, pb, pc);
+  LineGeo<T, A> G;
+  if (live) G.init(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
-  const Ch<R>* __restrict__ fac = reinterpret_cast<const Ch<R>*>(J.fac) + tid;
-  Ch<R>* __restrict__ scr = reinterpret_cast<Ch<R>*>(J.scr) + tid;
+  const Ch<R>* __restrict__ fac = reinterpret_cast<const Ch<R>*>(J.fac) + line;
+  Ch<R>* __restrict__ scr = reinterpret_cast<Ch<R>*>(J.scr) + line;
   E* __restrict__ u = J.u;
   const T* __restrict__ iw = L.IW[A];
-  const int N = G.N;
-  if (N < 2) return;   // no interior node (levels keep at least 2 cells per axis)
   const long fstep = (long)FC * stride, sstep = (long)SC * stride;
-
-  // node 0: 1/d_0 and rho_0
-  const Ch<R> f0 = fac[(20 / V) * stride];
-  const Ch<R> s0 = scr[(8 / V) * stride];
-  const cxd rd0((double)f0.v[20 % V], (double)f0.v[21 % V]);
-  const cxd q0 = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd0;
+  const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
 
   Ch<R> rf[D][FC], rs[D][SC];
   T riw[D];
-
-  // ---- forward substitution: z_p = W_p^-1 (g_p - B_p z_{p-1}) ----
+  cxd tm[4];   // this half's coupling term at the meeting node
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int pn = min(1 + d, N - 1);
-    load_node(rf[d], fac + pn * fstep, stride);
-    load_node(rs[d], scr + pn * sstep, stride);
-    riw[d] = iw[pn - 1];
-  }
-  cxd rd_prev = rd0, q_prev = q0;
-  cxd z[4];
-  for (int p0 = 1; p0 <= N - 1; p0 += D) {
+  for (int i = 0; i < 4; i++) tm[i] = cxd(0, 0);
+
+  // ---------------- forward elimination ----------------
+  if (live && side == 0) {
+    const int last = m - 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const int p = p0 + d;
-      if (p <= N - 1) {
-        Ch<R> cf[FC], cr[SC];
+      const int pn = max(min(1 + d, last), 1);
+      load_node(rf[d], fac + pn * fstep, stride);
+      load_node(rs[d], scr + pn * sstep, stride);
+      riw[d] = iw[pn - 1];
+    }
+    const Ch<R> f0 = fac[CD * stride], s0 = scr[CQ * stride];
+    cxd rd_prev((double)f0.v[20 % V], (double)f0.v[21 % V]);
+    cxd q_prev = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd_prev;
+    cxd z[4];
+    for (int p0 = 1; p0 <= last; p0 += D) {
 #pragma unroll
-        for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+      for (int d = 0; d < D; d++) {
+        const int p = p0 + d;
+        if (p <= last) {
+          Ch<R> cf[FC], cr[SC];
 #pragma unroll
-        for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
-        const double iWa = (double)riw[d];
-        {
-          const int pn = min(p + D, N - 1);
-          load_node(rf[d], fac + pn * fstep, stride);
-          load_node(rs[d], scr + pn * sstep, stride);
-          riw[d] = iw[pn - 1];
+          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+#pragma unroll
+          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          const double iWe = (double)riw[d];
+          {
+            const int pn = min(p + D, last);
+            load_node(rf[d], fac + pn * fstep, stride);
+            load_node(rs[d], scr + pn * sstep, stride);
+            riw[d] = iw[pn - 1];
+          }
+          const cxd rd = cval(cf, 20);
+          const cxd q = cval(cr, 8) * rd;
+          const cxd dq = q_prev - q;
+          cxd g[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
+          if (p >= 2) {
+            cxd t[4];
+            bmul(G, z, iWe, rd_prev, t);
+#pragma unroll
+            for (int i = 0; i < 4; i++) g[i] -= t[i];
+          }
+          symv(cf, g, z);
+          Ch<R> zc[SC] = {};
+#pragma unroll
+          for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
+          Ch<R>* sp = scr + p * sstep;
+#pragma unroll
+          for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
+          rd_prev = rd;
+          q_prev = q;
         }
-        const cxd rd(val(cf, 20), val(cf, 21));
-        const cxd q = cxd(val(cr, 8), val(cr, 9)) * rd;
-        const cxd dq = q_prev - q;
-        cxd g[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) g[i] = cxd(val(cr, 2 * i), val(cr, 2 * i + 1)) + dq * G.v[i];
-        if (p >= 2) {
-          // B_p z_{p-1} with l = lcoef(p - 1)
-          const double l[4] = {-G.Wb0 * G.Hc * iWa, -G.Wb1 * G.Hc * iWa, -G.Wc0 * G.Hb * iWa, -G.Wc1 * G.Hb * iWa};
-          const cxd vz = z[0] * G.v[0] + z[1] * G.v[1] + z[2] * G.v[2] + z[3] * G.v[3];
-#pragma unroll
-          for (int i = 0; i < 4; i++) g[i] -= z[i] * l[i] + (vz * rd_prev) * G.v[i];
-        }
-        symv(cf, g, z);
-        Ch<R> zc[SC] = {};
-#pragma unroll
-        for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
-        Ch<R>* sp = scr + p * sstep;
-#pragma unroll
-        for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
-        rd_prev = rd;
-        q_prev = q;
       }
     }
-  }
-
-  // ---- back substitution: da_p = z_p - W_p^-1 B_{p+1} da_{p+1}; update ----
-  const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
-  E ru[D][5];
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    const int pn = max(N - 1 - d, 1);
-    load_node(rf[d], fac + pn * fstep, stride);
-    load_node(rs[d], scr + pn * sstep, stride);
-    riw[d] = iw[pn];
-#pragma unroll
-    for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
-  }
-  cxd an[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) an[i] = cxd(0, 0);
-  for (int p0 = N - 1; p0 >= 1; p0 -= D) {
+    if (m >= 2) bmul(G, z, (double)iw[m - 1], rd_prev, tm);   // B_m z_{m-1}
+  } else if (live) {
+    // ring entries also carry rho_{p-1} and 1/d_{p-1} of the node above
+    Ch<R> rdn[D], rqn[D];
+    const int last = m + 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      const int p = p0 - d;
-      if (p >= 1) {
-        Ch<R> cf[FC], cr[SC];
-        E cu[5];
+      const int pn = min(max(M - d, last), M);
+      load_node(rf[d], fac + pn * fstep, stride);
+      load_node(rs[d], scr + pn * sstep, stride);
+      rdn[d] = fac[(pn - 1) * fstep + CD * stride];
+      rqn[d] = scr[(pn - 1) * sstep + CQ * stride];
+      riw[d] = iw[pn];
+    }
+    cxd zn[4], rd_last(0, 0);
+    for (int p0 = M; p0 >= last; p0 -= D) {
 #pragma unroll
-        for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+      for (int d = 0; d < D; d++) {
+        const int p = p0 - d;
+        if (p >= last) {
+          Ch<R> cf[FC], cr[SC];
 #pragma unroll
-        for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
-        for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
-        const double iWa = (double)riw[d];
-        {
-          const int pn = max(p - D, 1);
-          load_node(rf[d], fac + pn * fstep, stride);
-          load_node(rs[d], scr + pn * sstep, stride);
-          riw[d] = iw[pn];
+          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          const Ch<R> cdm = rdn[d], cqm = rqn[d];
+          const double iWe = (double)riw[d];
+          {
+            const int pn = max(p - D, last);
+            load_node(rf[d], fac + pn * fstep, stride);
+            load_node(rs[d], scr + pn * sstep, stride);
+            rdn[d] = fac[(pn - 1) * fstep + CD * stride];
+            rqn[d] = scr[(pn - 1) * sstep + CQ * stride];
+            riw[d] = iw[pn];
+          }
+          const cxd rd = cval(cf, 20);
+          const cxd q = cval(cr, 8) * rd;
+          const cxd rdm((double)cdm.v[20 % V], (double)cdm.v[21 % V]);
+          const cxd qm = cxd((double)cqm.v[8 % V], (double)cqm.v[9 % V]) * rdm;
+          const cxd dq = qm - q;
+          cxd g[4];
 #pragma unroll
-          for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+          for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
+          if (p <= M - 1) {
+            cxd t[4];
+            bmul(G, zn, iWe, rd, t);
+#pragma unroll
+            for (int i = 0; i < 4; i++) g[i] -= t[i];
+          }
+          symv(cf, g, zn);
+          Ch<R> zc[SC] = {};
+#pragma unroll
+          for (int i = 0; i < 4; i++) { put(zc, 2 * i, zn[i].re); put(zc, 2 * i + 1, zn[i].im); }
+          Ch<R>* sp = scr + p * sstep;
+#pragma unroll
+          for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
+          rd_last = rdm;   // 1/d_{p-1}: after the loop, 1/d_m
         }
-        const cxd rd(val(cf, 20), val(cf, 21));
-        const cxd q = cxd(val(cr, 8), val(cr, 9)) * rd;
-        cxd ap[4];
+      }
+    }
+    if (m + 1 <= M) bmul(G, zn, (double)iw[m], rd_last, tm);   // B_{m+1} z_{m+1}
+  }
+
+  // ---------------- meeting node ----------------
 #pragma unroll
-        for (int i = 0; i < 4; i++) ap[i] = cxd(val(cr, 2 * i), val(cr, 2 * i + 1));
-        if (p < N - 1) {
-          // W_p^-1 B_{p+1} da_{p+1} with l = lcoef(p)
-          const double l[4] = {-G.Wb0 * G.Hc * iWa, -G.Wb1 * G.Hc * iWa, -G.Wc0 * G.Hb * iWa, -G.Wc1 * G.Hb * iWa};
-          const cxd va = an[0] * G.v[0] + an[1] * G.v[1] + an[2] * G.v[2] + an[3] * G.v[3];
-          cxd t[4], wt[4];
+  for (int i = 0; i < 4; i++) {
+    xch[side][2 * i][lane] = tm[i].re;
+    xch[side][2 * i + 1][lane] = tm[i].im;
+  }
+  __syncthreads();
+  if (!live) return;
+  cxd xm[4], qmid, rdmid;
+  {
+    Ch<R> cf[FC], cr[SC];
+    load_node(cf, fac + m * fstep, stride);
+    load_node(cr, scr + m * sstep, stride);
+    const Ch<R> cdm = fac[(m - 1) * fstep + CD * stride], cqm = scr[(m - 1) * sstep + CQ * stride];
+    rdmid = cval(cf, 20);
+    qmid = cval(cr, 8) * rdmid;
+    const cxd rdm((double)cdm.v[20 % V], (double)cdm.v[21 % V]);
+    const cxd dq = cxd((double)cqm.v[8 % V], (double)cqm.v[9 % V]) * rdm - qmid;
+    cxd g[4];
 #pragma unroll
-          for (int i = 0; i < 4; i++) t[i] = an[i] * l[i] + (va * rd) * G.v[i];
+    for (int i = 0; i < 4; i++) {
+      const cxd ttop(xch[0][2 * i][lane], xch[0][2 * i + 1][lane]);
+      const cxd tbot(xch[1][2 * i][lane], xch[1][2 * i + 1][lane]);
+      g[i] = ((cval(cr, 2 * i) + dq * G.v[i]) - ttop) - tbot;
+    }
+    symv(cf, g, xm);
+  }
+
+  // ---------------- back substitution ----------------
+  E ru[D][5];
+  if (side == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const long o = off_u[i + 1] + (long)m * G.sa;
+      st<T>(u, o, ld<T>(u, o) + xm[i]);
+    }
+    const int last = m - 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = max(last - d, 1);
+      load_node(rf[d], fac + pn * fstep, stride);
+      load_node(rs[d], scr + pn * sstep, stride);
+      riw[d] = iw[pn];
+#pragma unroll
+      for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+    }
+    cxd an[4];   // x_{p+1}
+#pragma unroll
+    for (int i = 0; i < 4; i++) an[i] = xm[i];
+    for (int p0 = last; p0 >= 1; p0 -= D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 - d;
+        if (p >= 1) {
+          Ch<R> cf[FC], cr[SC];
+          E cu[5];
+#pragma unroll
+          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+#pragma unroll
+          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+#pragma unroll
+          for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+          const double iWe = (double)riw[d];
+          {
+            const int pn = max(p - D, 1);
+            load_node(rf[d], fac + pn * fstep, stride);
+            load_node(rs[d], scr + pn * sstep, stride);
+            riw[d] = iw[pn];
+#pragma unroll
+            for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+          }
+          const cxd rd = cval(cf, 20);
+          const cxd q = cval(cr, 8) * rd;
+          cxd t[4], wt[4], ap[4];
+          bmul(G, an, iWe, rd, t);
           symv(cf, t, wt);
 #pragma unroll
-          for (int i = 0; i < 4; i++) ap[i] -= wt[i];
+          for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
+          cxd dv(0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
+          const long o = (long)p * G.sa;
+          st<T>(u, G.oa + o, cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd));
+#pragma unroll
+          for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+#pragma unroll
+          for (int i = 0; i < 4; i++) an[i] = ap[i];
         }
-        cxd dv(0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
-        const long o = (long)p * G.sa;
-        const cxd e0 = cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd);
-        st<T>(u, G.oa + o, e0);
-#pragma unroll
-        for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
-#pragma unroll
-        for (int i = 0; i < 4; i++) an[i] = ap[i];
       }
     }
-  }
-  {
     // along-line edge 0, between the boundary node and node 1
+    const Ch<R> f0 = fac[CD * stride], s0 = scr[CQ * stride];
+    const cxd rd0((double)f0.v[20 % V], (double)f0.v[21 % V]);
+    const cxd q0 = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd0;
     cxd dv(0, 0);
 #pragma unroll
     for (int i = 0; i < 4; i++) dv += an[i] * G.v[i];
     st<T>(u, G.oa, ld<T>(u, G.oa) + (q0 + dv * rd0));
+  } else {
+    const int last = m + 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = min(last + d, M);
+      load_node(rf[d], fac + pn * fstep, stride);
+      load_node(rs[d], scr + pn * sstep, stride);
+      riw[d] = iw[pn - 1];
+      ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
+#pragma unroll
+      for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+    }
+    cxd xp[4];   // x_{p-1}
+#pragma unroll
+    for (int i = 0; i < 4; i++) xp[i] = xm[i];
+    cxd rdm = rdmid, qm = qmid;   // 1/d and q of edge p-1
+    for (int p0 = last; p0 <= M; p0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 + d;
+        if (p <= M) {
+          Ch<R> cf[FC], cr[SC];
+          E cu[5];
+#pragma unroll
+          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
+#pragma unroll
+          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+#pragma unroll
+          for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+          const double iWe = (double)riw[d];
+          {
+            const int pn = min(p + D, M);
+            load_node(rf[d], fac + pn * fstep, stride);
+            load_node(rs[d], scr + pn * sstep, stride);
+            riw[d] = iw[pn - 1];
+            ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
+#pragma unroll
+            for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+          }
+          const cxd rd = cval(cf, 20);
+          const cxd q = cval(cr, 8) * rd;
+          cxd t[4], wt[4], ap[4];
+          bmul(G, xp, iWe, rdm, t);
+          symv(cf, t, wt);
+#pragma unroll
+          for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
+          cxd dv(0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; i++) dv += (ap[i] - xp[i]) * G.v[i];
+          const long o = (long)p * G.sa;
+          st<T>(u, G.oa + o - G.sa, cxd((double)cu[0].x, (double)cu[0].y) + (qm + dv * rdm));
+#pragma unroll
+          for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+#pragma unroll
+          for (int i = 0; i < 4; i++) xp[i] = ap[i];
+          rdm = rd;
+          qm = q;
+        }
+      }
+    }
+    // along-line edge M = N-1, between node M and the boundary node
+    cxd dv(0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dv += (cxd(0, 0) - xp[i]) * G.v[i];
+    const long o = G.oa + (long)M * G.sa;
+    st<T>(u, o, ld<T>(u, o) + (qm + dv * rdm));
   }
 }
 
@@ -624,7 +864,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     else hipLaunchKernelGGL((k_line_rhs<T, 2>), g, b, 0, s, P);
   }
   {
-    const dim3 g((maxl + 63) / 64, P.njobs), b(64);
+    const dim3 g((maxl + 63) / 64, P.njobs), b(128);
     timing::Scope ts(timing::LINE, s, solve_bytes);
     if (axis == 0) hipLaunchKernelGGL((k_line_solve<T, 0>), g, b, 0, s, P);
     else if (axis == 1) hipLaunchKernelGGL((k_line_solve<T, 1>), g, b, 0, s, P);
