@@ -33,6 +33,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s at 128^3 to rel-res 1e-6: plain FP64 vs mixed+batched+learned x0"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_line_solve launch over its algorithmic bytes, measured on
+# the mixed+batched 128^3 step with two rocprofv3 --pmc passes (FETCH_SIZE,
+# then WRITE_SIZE, kernels k_line_solve and the runtime's copy kernel), each
+# scaled by the same counter on the 103 MB device-to-device copies of that run
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 46.5 MB fetched + 12.0 MB written per
+# launch against 29.8 MB algorithmic.  The two passes of the substitution read
+# the factors twice and write and re-read z once.
+LINE_SOLVE_TRAFFIC_RATIO = (46.51e6 + 11.97e6) / 29.80e6
 
 
 def parse():
@@ -243,7 +251,9 @@ def main():
                                f"{survey.freqs} Hz, {len(survey.receivers)} receivers, rel-res 1e-6",
                    "solves_per_step": len(survey.freqs), "variant": best},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": LINE_SOLVE_TRAFFIC_RATIO * by / max(cnt, 1) if dom == "line_solve" else None,
+                     "kernel": dom,
                      "avg_launch_us": 1e3 * ms / max(cnt, 1), "launches": cnt},
         "variants": {k: {"solves_per_s": v["solves_per_s"], "ms_per_step": v["ms_per_step"],
                          "cycles": v["cycles"], "status": v["status"], "max_rel_res": v["rel_res"],

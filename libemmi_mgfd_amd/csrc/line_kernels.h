@@ -5,8 +5,8 @@
 // systems are factored once (k_line_factor: FREC values per node, depending
 // on grid, medium and frequency only) and every sweep then computes the
 // right-hand sides (k_line_rhs, SREC values per node) and runs a forward/back
-// substitution (k_line_solve).  Records hold values of type LineRec<T>::type
-// in 16-byte chunks: value k of node p of line l is element k % V of chunk
+// substitution (k_line_solve).  Records hold values of type LineFac<T>::type
+// (factors) or LineScr<T>::type (line records) in 16-byte chunks: value k of node p of line l is element k % V of chunk
 // (p * nchunks + k / V) * nlines + l, V = 16 / sizeof(value), so a lane reads
 // its node with 16-byte loads and a wave's load covers contiguous memory.  A launch takes a pack of up to
 // LINE_PACK_MAX jobs (one per right-hand side or per level) that share the
@@ -21,23 +21,22 @@ namespace emmi {
 
 constexpr int LINE_PACK_MAX = 8;
 
-// Value type of the line records for cycles of storage precision T.  FP64
-// for both: in air the line blocks are close to singular, W^-1 and the
-// forward intermediates z are large and cancel in the back substitution, and
-// FP32 records lose the correction there (the FP32-storage cycle diverges).
-// EMMI_LINE_REC_F32 selects FP32 records for FP32 cycles (experiments only).
-template <typename T> struct LineRec { using type = double; };
-#ifdef EMMI_LINE_REC_F32
-template <> struct LineRec<float> { using type = float; };
-#endif
+// Value types of the line factor records (LineFac) and of the right-hand
+// side / substitution records (LineScr) for cycles of storage precision T.
+// Factors are FP64 for both precisions: in air the line blocks are close to
+// singular and entries of W^-1 exceed the FP32 range.  The line records
+// (residuals sigma, rho and the forward intermediates z) take the cycle's
+// precision.
+template <typename T> struct LineFac { using type = double; };
+template <typename T> struct LineScr { using type = T; };
 
 template <typename T>
 struct LineJob {
   Level<T> L;
   typename cvec<T>::type* u;
   const typename cvec<T>::type* f;
-  typename LineRec<T>::type* fac;  // factor records of this colour, chunked: [n_a][FREC/V][nlines][V]
-  typename LineRec<T>::type* scr;  // right-hand side / substitution records, chunked: [n_a][SREC/V][nlines][V]
+  typename LineFac<T>::type* fac;  // factor records of this colour, chunked: [n_a][FREC/V][nlines][V]
+  typename LineScr<T>::type* scr;  // right-hand side / substitution records, chunked: [n_a][SREC/V][nlines][V]
   int cb, cc;    // colour: parity of pb and pc (1 = odd)
   int nlo, nhi;  // lines of this colour along the low- and high-stride perpendicular axes
 };

@@ -23,10 +23,13 @@
 // with g_p = sigma_p + v (rho_{p-1}/d_{p-1} - rho_p/d_p) from the residual.
 // Lines of one colour (pb%2, pc%2) are not coupled, so a launch sweeps one
 // colour; every kernel computes in FP64 whatever the storage precision.
-// Records hold LineRec<T> values in 16-byte chunks (layout in line_kernels.h);
+// Records hold LineFac<T> / LineScr<T> values in 16-byte chunks (layout in
+// line_kernels.h);
 // the substitution keeps a ring of the next D nodes' chunks in registers, so
 // D nodes of loads are in flight while a node is computed.
 #include "line_kernels.h"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include "stencil.h"
@@ -45,7 +48,8 @@ struct alignas(16) Ch {
 };
 template <typename R> constexpr int fchunks() { return (FREC + Ch<R>::V - 1) / Ch<R>::V; }
 template <typename R> constexpr int schunks() { return (SREC + Ch<R>::V - 1) / Ch<R>::V; }
-template <typename T> using Rec = typename LineRec<T>::type;
+template <typename T> using FacRec = typename LineFac<T>::type;
+template <typename T> using ScrRec = typename LineScr<T>::type;
 
 // nodes of records in flight in the substitution (build-time tunable)
 // for FP32-storage and FP64 cycles
@@ -230,7 +234,7 @@ __device__ __forceinline__ void store_factor(Ch<R>* rec, long stride, int p, cxd
 // stores 1/d_0.
 template <typename T, int A>
 __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
-  using R = Rec<T>;
+  using R = FacRec<T>;
   constexpr int FC = fchunks<R>(), V = Ch<R>::V;
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
@@ -334,7 +338,7 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
   const typename cvec<T>::type* __restrict__ u = J.u;
   const typename cvec<T>::type* __restrict__ f = J.f;
   const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
-  using R = Rec<T>;
+  using R = ScrRec<T>;
   constexpr int SC = schunks<R>(), V = Ch<R>::V;
   Ch<R> c[SC] = {};
   Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(256) k_line_rhs_x(LinePack<T> P, int nlb) {
     const int p = p0 + pl;
     const long line = l0 + ll;
     if (p < N && line < nl) {
-      using R = Rec<T>;
+      using R = ScrRec<T>;
       constexpr int SC = schunks<R>(), V = Ch<R>::V;
       Ch<R> c[SC] = {};
 #pragma unroll
@@ -479,11 +483,12 @@ __device__ __forceinline__ cxd cval(const Ch<R> (&c)[NC], int k) { return cxd(va
 template <typename T, int A>
 __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   using E = typename cvec<T>::type;
-  using R = Rec<T>;
-  constexpr int FC = fchunks<R>(), SC = schunks<R>(), V = Ch<R>::V, D = ring_depth<T>();
-  constexpr int ZC = 8 / V;    // chunks holding values 0..7 (sigma, then z)
-  constexpr int CD = 20 / V;   // factor chunk holding 1/d (values 20, 21)
-  constexpr int CQ = 8 / V;    // record chunk holding rho (values 8, 9)
+  using RF = FacRec<T>;
+  using RS = ScrRec<T>;
+  constexpr int FC = fchunks<RF>(), SC = schunks<RS>(), VF = Ch<RF>::V, VS = Ch<RS>::V, D = ring_depth<T>();
+  constexpr int ZC = 8 / VS;    // chunks holding values 0..7 (sigma, then z)
+  constexpr int CD = 20 / VF;   // factor chunk holding 1/d (values 20, 21)
+  constexpr int CQ = 8 / VS;    // record chunk holding rho (values 8, 9)
   __shared__ double xch[2][8][64];
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
@@ -497,14 +502,15 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   LineGeo<T, A> G;
   if (live) G.init(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
-  const Ch<R>* __restrict__ fac = reinterpret_cast<const Ch<R>*>(J.fac) + line;
-  Ch<R>* __restrict__ scr = reinterpret_cast<Ch<R>*>(J.scr) + line;
+  const Ch<RF>* __restrict__ fac = reinterpret_cast<const Ch<RF>*>(J.fac) + line;
+  Ch<RS>* __restrict__ scr = reinterpret_cast<Ch<RS>*>(J.scr) + line;
   E* __restrict__ u = J.u;
   const T* __restrict__ iw = L.IW[A];
   const long fstep = (long)FC * stride, sstep = (long)SC * stride;
   const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
 
-  Ch<R> rf[D][FC], rs[D][SC];
+  Ch<RF> rf[D][FC];
+  Ch<RS> rs[D][SC];
   T riw[D];
   cxd tm[4];   // this half's coupling term at the meeting node
 #pragma unroll
@@ -520,16 +526,18 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       load_node(rs[d], scr + pn * sstep, stride);
       riw[d] = iw[pn - 1];
     }
-    const Ch<R> f0 = fac[CD * stride], s0 = scr[CQ * stride];
-    cxd rd_prev((double)f0.v[20 % V], (double)f0.v[21 % V]);
-    cxd q_prev = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd_prev;
+    const Ch<RF> f0 = fac[CD * stride];
+    const Ch<RS> s0 = scr[CQ * stride];
+    cxd rd_prev((double)f0.v[20 % VF], (double)f0.v[21 % VF]);
+    cxd q_prev = cxd((double)s0.v[8 % VS], (double)s0.v[9 % VS]) * rd_prev;
     cxd z[4];
     for (int p0 = 1; p0 <= last; p0 += D) {
 #pragma unroll
       for (int d = 0; d < D; d++) {
         const int p = p0 + d;
         if (p <= last) {
-          Ch<R> cf[FC], cr[SC];
+          Ch<RF> cf[FC];
+          Ch<RS> cr[SC];
 #pragma unroll
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
@@ -554,10 +562,10 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
             for (int i = 0; i < 4; i++) g[i] -= t[i];
           }
           symv(cf, g, z);
-          Ch<R> zc[SC] = {};
+          Ch<RS> zc[SC] = {};
 #pragma unroll
           for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
-          Ch<R>* sp = scr + p * sstep;
+          Ch<RS>* sp = scr + p * sstep;
 #pragma unroll
           for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
           rd_prev = rd;
@@ -568,7 +576,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     if (m >= 2) bmul(G, z, (double)iw[m - 1], rd_prev, tm);   // B_m z_{m-1}
   } else if (live) {
     // ring entries also carry rho_{p-1} and 1/d_{p-1} of the node above
-    Ch<R> rdn[D], rqn[D];
+    Ch<RF> rdn[D];
+    Ch<RS> rqn[D];
     const int last = m + 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -585,12 +594,14 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       for (int d = 0; d < D; d++) {
         const int p = p0 - d;
         if (p >= last) {
-          Ch<R> cf[FC], cr[SC];
+          Ch<RF> cf[FC];
+          Ch<RS> cr[SC];
 #pragma unroll
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
           for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
-          const Ch<R> cdm = rdn[d], cqm = rqn[d];
+          const Ch<RF> cdm = rdn[d];
+          const Ch<RS> cqm = rqn[d];
           const double iWe = (double)riw[d];
           {
             const int pn = max(p - D, last);
@@ -602,8 +613,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
           }
           const cxd rd = cval(cf, 20);
           const cxd q = cval(cr, 8) * rd;
-          const cxd rdm((double)cdm.v[20 % V], (double)cdm.v[21 % V]);
-          const cxd qm = cxd((double)cqm.v[8 % V], (double)cqm.v[9 % V]) * rdm;
+          const cxd rdm((double)cdm.v[20 % VF], (double)cdm.v[21 % VF]);
+          const cxd qm = cxd((double)cqm.v[8 % VS], (double)cqm.v[9 % VS]) * rdm;
           const cxd dq = qm - q;
           cxd g[4];
 #pragma unroll
@@ -615,10 +626,10 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
             for (int i = 0; i < 4; i++) g[i] -= t[i];
           }
           symv(cf, g, zn);
-          Ch<R> zc[SC] = {};
+          Ch<RS> zc[SC] = {};
 #pragma unroll
           for (int i = 0; i < 4; i++) { put(zc, 2 * i, zn[i].re); put(zc, 2 * i + 1, zn[i].im); }
-          Ch<R>* sp = scr + p * sstep;
+          Ch<RS>* sp = scr + p * sstep;
 #pragma unroll
           for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
           rd_last = rdm;   // 1/d_{p-1}: after the loop, 1/d_m
@@ -638,14 +649,16 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   if (!live) return;
   cxd xm[4], qmid, rdmid;
   {
-    Ch<R> cf[FC], cr[SC];
+    Ch<RF> cf[FC];
+          Ch<RS> cr[SC];
     load_node(cf, fac + m * fstep, stride);
     load_node(cr, scr + m * sstep, stride);
-    const Ch<R> cdm = fac[(m - 1) * fstep + CD * stride], cqm = scr[(m - 1) * sstep + CQ * stride];
+    const Ch<RF> cdm = fac[(m - 1) * fstep + CD * stride];
+    const Ch<RS> cqm = scr[(m - 1) * sstep + CQ * stride];
     rdmid = cval(cf, 20);
     qmid = cval(cr, 8) * rdmid;
-    const cxd rdm((double)cdm.v[20 % V], (double)cdm.v[21 % V]);
-    const cxd dq = cxd((double)cqm.v[8 % V], (double)cqm.v[9 % V]) * rdm - qmid;
+    const cxd rdm((double)cdm.v[20 % VF], (double)cdm.v[21 % VF]);
+    const cxd dq = cxd((double)cqm.v[8 % VS], (double)cqm.v[9 % VS]) * rdm - qmid;
     cxd g[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -682,7 +695,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       for (int d = 0; d < D; d++) {
         const int p = p0 - d;
         if (p >= 1) {
-          Ch<R> cf[FC], cr[SC];
+          Ch<RF> cf[FC];
+          Ch<RS> cr[SC];
           E cu[5];
 #pragma unroll
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
@@ -719,9 +733,10 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       }
     }
     // along-line edge 0, between the boundary node and node 1
-    const Ch<R> f0 = fac[CD * stride], s0 = scr[CQ * stride];
-    const cxd rd0((double)f0.v[20 % V], (double)f0.v[21 % V]);
-    const cxd q0 = cxd((double)s0.v[8 % V], (double)s0.v[9 % V]) * rd0;
+    const Ch<RF> f0 = fac[CD * stride];
+    const Ch<RS> s0 = scr[CQ * stride];
+    const cxd rd0((double)f0.v[20 % VF], (double)f0.v[21 % VF]);
+    const cxd q0 = cxd((double)s0.v[8 % VS], (double)s0.v[9 % VS]) * rd0;
     cxd dv(0, 0);
 #pragma unroll
     for (int i = 0; i < 4; i++) dv += an[i] * G.v[i];
@@ -747,7 +762,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       for (int d = 0; d < D; d++) {
         const int p = p0 + d;
         if (p <= M) {
-          Ch<R> cf[FC], cr[SC];
+          Ch<RF> cf[FC];
+          Ch<RS> cr[SC];
           E cu[5];
 #pragma unroll
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
@@ -823,6 +839,16 @@ size_t line_scratch_bytes(const int* n, int a, int colour, size_t esize) {
   return (size_t)nlo * nhi * (size_t)n[a] * chunks_of(SREC, esize) * 16;
 }
 
+// A launch; when timing is on, its events are recorded by the dispatch itself
+// (hipExtLaunchKernelGGL), so their elapsed time is the kernel's alone.
+template <typename K, typename... Args>
+static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, double bytes, Args... args) {
+  hipEvent_t e0, e1;
+  timing::kernel_events(cls, bytes, &e0, &e1);
+  if (e0) hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
+}
+
 template <typename T>
 static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
   int maxl = 0;
@@ -837,9 +863,9 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     const double rows = (double)nl * (5.0 * na - 4.0);
     // right-hand sides: the colour's rows read f and sigma-bar once, u is
     // read once over the level, the records are written once
-    rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * sizeof(Rec<T>);
+    rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * sizeof(ScrRec<T>);
     // substitution: records and factors read once, the rows of u read and written
-    solve_bytes += (double)nl * na * (SREC + FREC) * sizeof(Rec<T>) + rows * 2.0 * es;
+    solve_bytes += (double)nl * na * (SREC * sizeof(ScrRec<T>) + FREC * sizeof(FacRec<T>)) + rows * 2.0 * es;
   }
   if (maxl == 0 || P.njobs == 0) return;
   if (factor) {
@@ -855,20 +881,17 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
     const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
     const dim3 g((unsigned)(nlb * npb), P.njobs), b(XT * XT);
-    timing::Scope ts(timing::LINE_RHS, s, rhs_bytes);
-    hipLaunchKernelGGL((k_line_rhs_x<T>), g, b, 0, s, P, nlb);
+    launch_timed(k_line_rhs_x<T>, g, b, s, timing::LINE_RHS, rhs_bytes, P, nlb);
   } else {
     const dim3 g((unsigned)((maxnodes + 255) / 256), P.njobs), b(256);
-    timing::Scope ts(timing::LINE_RHS, s, rhs_bytes);
-    if (axis == 1) hipLaunchKernelGGL((k_line_rhs<T, 1>), g, b, 0, s, P);
-    else hipLaunchKernelGGL((k_line_rhs<T, 2>), g, b, 0, s, P);
+    if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
+    else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
   }
   {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
-    timing::Scope ts(timing::LINE, s, solve_bytes);
-    if (axis == 0) hipLaunchKernelGGL((k_line_solve<T, 0>), g, b, 0, s, P);
-    else if (axis == 1) hipLaunchKernelGGL((k_line_solve<T, 1>), g, b, 0, s, P);
-    else hipLaunchKernelGGL((k_line_solve<T, 2>), g, b, 0, s, P);
+    if (axis == 0) launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);
+    else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, solve_bytes, P);
+    else launch_timed(k_line_solve<T, 2>, g, b, s, timing::LINE, solve_bytes, P);
   }
 }
 

@@ -194,7 +194,7 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
   }
 
   // records of the line solves: max over levels, patterns, axes and colours
-  const size_t esize = p_.precision == 0 ? sizeof(LineRec<double>::type) : sizeof(LineRec<float>::type);
+  const size_t esize = p_.precision == 0 ? sizeof(LineScr<double>::type) : sizeof(LineScr<float>::type);
   for (int c = 0; c < npat_; c++)
     for (int lev = 0; lev < lmax_; lev++)
       for (int a = 0; a < 3; a++)
@@ -341,12 +341,12 @@ LineFactors& Solver::factors(Slot& S, int pat, int lev, int axis) {
   LinePack<T> P;
   P.njobs = 4;
   for (int col = 0; col < 4; col++) {
-    if (!F.fac[col]) F.fac[col] = dalloc(std::max<size_t>(16, line_factor_bytes(L.n, axis, col, sizeof(typename LineRec<T>::type))), owned_);
+    if (!F.fac[col]) F.fac[col] = dalloc(std::max<size_t>(16, line_factor_bytes(L.n, axis, col, sizeof(typename LineFac<T>::type))), owned_);
     LineJob<T>& J = P.job[col];
     J.L = L.view(S.omu);
     J.u = nullptr;
     J.f = nullptr;
-    J.fac = (typename LineRec<T>::type*)F.fac[col];
+    J.fac = (typename LineFac<T>::type*)F.fac[col];
     J.scr = nullptr;
     line_job_colour(J, axis, col);
   }
@@ -366,10 +366,10 @@ void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const 
   J.L = L.view(S.omu);
   J.u = (typename cvec<T>::type*)u;
   J.f = (const typename cvec<T>::type*)f;
-  J.scr = (typename LineRec<T>::type*)S.lscr;
+  J.scr = (typename LineScr<T>::type*)S.lscr;
   for (int k = 0; k < 4; k++) {
     const int colour = (iter % 2 == 1) ? 3 - k : k;
-    J.fac = (typename LineRec<T>::type*)F.fac[colour];
+    J.fac = (typename LineFac<T>::type*)F.fac[colour];
     line_job_colour(J, axis, colour);
     launch_line_solve<T>(P, axis, s_);
   }
@@ -396,8 +396,8 @@ void Solver::line_sweep(int pat, int lev, int axis, int iter) {
         J.L = L.view(S.omu);
         J.u = (typename cvec<T>::type*)S.u[lev];
         J.f = (const typename cvec<T>::type*)S.f[lev];
-        J.fac = (typename LineRec<T>::type*)F[i0 + j]->fac[colour];
-        J.scr = (typename LineRec<T>::type*)S.lscr;
+        J.fac = (typename LineFac<T>::type*)F[i0 + j]->fac[colour];
+        J.scr = (typename LineScr<T>::type*)S.lscr;
         line_job_colour(J, axis, colour);
       }
       launch_line_solve<T>(P, axis, s_);

@@ -56,6 +56,16 @@ void end(int cls, hipStream_t s, double bytes) {
   g_pending.push_back({cls, g_open[cls], e, bytes});
 }
 
+void kernel_events(int cls, double bytes, hipEvent_t* start, hipEvent_t* stop) {
+  *start = nullptr;
+  *stop = nullptr;
+  if (!g_on) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  *start = get_event();
+  *stop = get_event();
+  g_pending.push_back({cls, *start, *stop, bytes});
+}
+
 int read(double* ms, long* count, double* bytes, int n) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto& p : g_pending) {

@@ -14,6 +14,10 @@ bool enabled();
 // bracket one launch; no-ops when disabled
 void begin(int cls, hipStream_t s);
 void end(int cls, hipStream_t s, double bytes);
+// events for a launch that records them in its own dispatch
+// (hipExtLaunchKernelGGL), so the elapsed time is the kernel's alone; both
+// null when disabled
+void kernel_events(int cls, double bytes, hipEvent_t* start, hipEvent_t* stop);
 // synchronises the recorded events, returns accumulated ms, launch counts and
 // algorithmic bytes per class
 int read(double* ms, long* count, double* bytes, int n);
