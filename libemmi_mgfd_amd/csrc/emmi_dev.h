@@ -55,6 +55,18 @@ __device__ __forceinline__ void st(typename cvec<T>::type* p, long i, cxd v) {
   p[i] = w;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip (block b
+// runs on XCD b % 8), each XCD with its own L2.  xcd_block maps the dispatch
+// index b of nb blocks to a logical block so that XCD x takes the x-th
+// contiguous eighth of the logical blocks: a stencil's neighbours along the
+// slowest axis are then loaded by the same XCD and found in its L2.  It is a
+// bijection of [0, nb).
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  const long x = b & 7, q = b >> 3;
+  const long base = nb >> 3, extra = nb & 7;
+  return x * base + (x < extra ? x : extra) + q;
+}
+
 // Geometry of one multigrid level as the kernels see it.
 //   W[a][i]  cell width along axis a (i in [0, n_a])
 //   IW[a][i] 1/W[a][i]

@@ -57,9 +57,23 @@ template <typename T> using ScrRec = typename LineScr<T>::type;
 #define EMMI_LINE_RING_F32 2
 #endif
 #ifndef EMMI_LINE_RING_F64
-#define EMMI_LINE_RING_F64 2
+#define EMMI_LINE_RING_F64 1
 #endif
 template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_LINE_RING_F32 : EMMI_LINE_RING_F64; }
+// Launches of at most this many lines per job (FP32-storage / FP64 cycles)
+// use the quad-lane substitution, whose ring holds EMMI_LINE_QUAD_RING nodes.
+#ifndef EMMI_LINE_QUAD_MAX_LINES_F32
+#define EMMI_LINE_QUAD_MAX_LINES_F32 1024
+#endif
+#ifndef EMMI_LINE_QUAD_MAX_LINES_F64
+#define EMMI_LINE_QUAD_MAX_LINES_F64 4096
+#endif
+#ifndef EMMI_LINE_QUAD_RING
+#define EMMI_LINE_QUAD_RING 3
+#endif
+template <typename T> constexpr int quad_max_lines() {
+  return sizeof(T) == 4 ? EMMI_LINE_QUAD_MAX_LINES_F32 : EMMI_LINE_QUAD_MAX_LINES_F64;
+}
 
 // value k of a node's chunks, as double
 template <typename R, int NC>
@@ -329,7 +343,7 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
   const Level<T>& L = J.L;
   const long nl = (long)J.nlo * J.nhi;
   const int N = L.n[A];
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nl * N) return;
   const int line = (int)(t % nl);
   const int p = (int)(t / nl);
@@ -812,6 +826,338 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
 }
 
 // ---------------------------------------------------------------------------
+// quad-lane substitution for launches with few lines
+// ---------------------------------------------------------------------------
+// Lanes 4k..4k+3 of a wave hold the components 0..3 of one line's node
+// vector and the matching rows of W^-1, so the serial chain of a node is a
+// quarter of the arithmetic plus quad exchanges (DPP): on levels with few
+// lines the launch is bound by that chain, not by memory.
+template <int CTRL>
+__device__ __forceinline__ double qdpp(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over the quad (every lane gets the same value)
+__device__ __forceinline__ cxd qsum(cxd a) {
+  a += cxd(qdpp<0xB1>(a.re), qdpp<0xB1>(a.im));   // quad_perm [1,0,3,2]
+  a += cxd(qdpp<0x4E>(a.re), qdpp<0x4E>(a.im));   // quad_perm [2,3,0,1]
+  return a;
+}
+// component J of the quad's vector
+template <int J>
+__device__ __forceinline__ cxd qget(cxd a) {
+  return cxd(qdpp<J * 0x55>(a.re), qdpp<J * 0x55>(a.im));
+}
+// row q of W^-1 (w) times the quad's vector y
+__device__ __forceinline__ cxd qsymv(const cxd (&w)[4], cxd y) {
+  const cxd y0 = qget<0>(y), y1 = qget<1>(y), y2 = qget<2>(y), y3 = qget<3>(y);
+  cxd acc(0, 0);
+  acc += w[0] * y0;
+  acc += w[1] * y1;
+  acc += w[2] * y2;
+  acc += w[3] * y3;
+  return acc;
+}
+
+// one node of one lane's records: row q of W^-1, 1/d, component q of the line
+// record (sigma or z) and rho
+template <typename RS>
+struct QNode {
+  double2 w[4];
+  double2 rd;
+  typename cvec<RS>::type s, rho;
+  typename cvec<RS>::type pad;
+};
+
+template <typename RS>
+__device__ __forceinline__ void qload(QNode<RS>& n, const Ch<double>* __restrict__ fnode, const Ch<RS>* __restrict__ snode,
+                                      long stride, const int (&wch)[4], long soff, int sel) {
+  using P = typename cvec<RS>::type;
+#pragma unroll
+  for (int j = 0; j < 4; j++) n.w[j] = *reinterpret_cast<const double2*>(&fnode[wch[j] * stride]);
+  n.rd = *reinterpret_cast<const double2*>(&fnode[10 * stride]);
+  n.s = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + soff) + sel);
+  constexpr int VS = Ch<RS>::V;
+  n.rho = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + (8 / VS) * stride) + 8 % VS);
+}
+
+template <typename T, int A, int D>
+__global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
+  using E = typename cvec<T>::type;
+  using RS = ScrRec<T>;
+  using PS = typename cvec<RS>::type;
+  constexpr int FC = fchunks<double>(), SC = schunks<RS>(), VS = Ch<RS>::V;
+  __shared__ double xch[2][2][64];
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const int N = L.n[A];
+  if (N < 2) return;   // uniform over the block: no interior node
+  const int M = N - 1, m = N / 2;
+  const int lane = threadIdx.x & 63, side = threadIdx.x >> 6;
+  const int q = lane & 3;
+  const int line = blockIdx.x * 16 + (lane >> 2);
+  int pb = 0, pc = 0;
+  const bool live = line_of<T, A>(J, line, pb, pc);
+  LineGeo<T, A> G;
+  if (live) G.init(L, pb, pc);
+  const long stride = (long)J.nlo * J.nhi;
+  const Ch<double>* __restrict__ fac = reinterpret_cast<const Ch<double>*>(J.fac) + line;
+  Ch<RS>* __restrict__ scr = reinterpret_cast<Ch<RS>*>(J.scr) + line;
+  E* __restrict__ u = J.u;
+  const T* __restrict__ iw = L.IW[A];
+  const long fstep = (long)FC * stride, sstep = (long)SC * stride;
+  // this lane's component: v_q, the coupling coefficient (l_q = cq / W_a), the
+  // chunks of row q of W^-1, and where component q of a line record sits
+  const double vq = q == 0 ? G.Hc : q == 1 ? -G.Hc : q == 2 ? G.Hb : -G.Hb;
+  const double cq = q == 0 ? -G.Wb0 * G.Hc : q == 1 ? -G.Wb1 * G.Hc : q == 2 ? -G.Wc0 * G.Hb : -G.Wc1 * G.Hb;
+  const int wch[4] = {symidx(q, 0), symidx(q, 1), symidx(q, 2), symidx(q, 3)};
+  const long soff = (long)((2 * q) / VS) * stride;
+  const int sel = (2 * q) % VS;
+  const long off_q = q == 0 ? G.ob0 : q == 1 ? G.ob1 : q == 2 ? G.oc0 : G.oc1;
+
+  QNode<RS> ring[D];
+  T riw[D];
+  cxd tm(0, 0);   // component q of this half's coupling term at the meeting node
+  auto cw = [](const QNode<RS>& n, cxd (&w)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) w[j] = cxd(n.w[j].x, n.w[j].y);
+  };
+
+  // ---------------- forward elimination ----------------
+  if (live && side == 0) {
+    const int last = m - 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = max(min(1 + d, last), 1);
+      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      riw[d] = iw[pn - 1];
+    }
+    const cxd rd0(fac[10 * stride].v[0], fac[10 * stride].v[1]);
+    const PS r0 = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (8 / VS) * stride) + 8 % VS);
+    cxd rd_prev = rd0;
+    cxd q_prev = cxd((double)r0.x, (double)r0.y) * rd0;
+    cxd z(0, 0);
+    for (int p0 = 1; p0 <= last; p0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 + d;
+        if (p <= last) {
+          const QNode<RS> nd = ring[d];
+          const double iWe = (double)riw[d];
+          {
+            const int pn = min(p + D, last);
+            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[d] = iw[pn - 1];
+          }
+          cxd w[4];
+          cw(nd, w);
+          const cxd rd(nd.rd.x, nd.rd.y);
+          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+          cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (q_prev - qp) * vq;
+          if (p >= 2) {
+            const cxd vz = qsum(z * vq);
+            g -= z * (cq * iWe) + (vz * rd_prev) * vq;
+          }
+          z = qsymv(w, g);
+          PS zs;
+          zs.x = (RS)z.re;
+          zs.y = (RS)z.im;
+          *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+          rd_prev = rd;
+          q_prev = qp;
+        }
+      }
+    }
+    if (m >= 2) {
+      const cxd vz = qsum(z * vq);
+      tm = z * (cq * (double)iw[m - 1]) + (vz * rd_prev) * vq;   // B_m z_{m-1}
+    }
+  } else if (live) {
+    const int last = m + 1;
+    cxd rdn[D], qn[D];   // 1/d and q of the node above each ring entry
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = min(max(M - d, last), M);
+      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      riw[d] = iw[pn];
+      const Ch<double> c = fac[(pn - 1) * fstep + 10 * stride];
+      const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+      rdn[d] = cxd(c.v[0], c.v[1]);
+      qn[d] = cxd((double)r.x, (double)r.y);
+    }
+    cxd zn(0, 0), rd_last(0, 0);
+    for (int p0 = M; p0 >= last; p0 -= D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 - d;
+        if (p >= last) {
+          const QNode<RS> nd = ring[d];
+          const double iWe = (double)riw[d];
+          const cxd rdm = rdn[d], rhom = qn[d];
+          {
+            const int pn = max(p - D, last);
+            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[d] = iw[pn];
+            const Ch<double> c = fac[(pn - 1) * fstep + 10 * stride];
+            const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+            rdn[d] = cxd(c.v[0], c.v[1]);
+            qn[d] = cxd((double)r.x, (double)r.y);
+          }
+          cxd w[4];
+          cw(nd, w);
+          const cxd rd(nd.rd.x, nd.rd.y);
+          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+          const cxd qm = rhom * rdm;
+          cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (qm - qp) * vq;
+          if (p <= M - 1) {
+            const cxd vz = qsum(zn * vq);
+            g -= zn * (cq * iWe) + (vz * rd) * vq;
+          }
+          zn = qsymv(w, g);
+          PS zs;
+          zs.x = (RS)zn.re;
+          zs.y = (RS)zn.im;
+          *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+          rd_last = rdm;
+        }
+      }
+    }
+    if (m + 1 <= M) {
+      const cxd vz = qsum(zn * vq);
+      tm = zn * (cq * (double)iw[m]) + (vz * rd_last) * vq;   // B_{m+1} z_{m+1}
+    }
+  }
+
+  // ---------------- meeting node ----------------
+  xch[side][0][lane] = tm.re;
+  xch[side][1][lane] = tm.im;
+  __syncthreads();
+  if (!live) return;
+  cxd xm, qmid, rdmid;
+  {
+    QNode<RS> nd;
+    qload(nd, fac + m * fstep, scr + m * sstep, stride, wch, soff, sel);
+    const Ch<double> c = fac[(m - 1) * fstep + 10 * stride];
+    const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (m - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+    cxd w[4];
+    cw(nd, w);
+    rdmid = cxd(nd.rd.x, nd.rd.y);
+    qmid = cxd((double)nd.rho.x, (double)nd.rho.y) * rdmid;
+    const cxd rdm(c.v[0], c.v[1]);
+    const cxd dq = cxd((double)r.x, (double)r.y) * rdm - qmid;
+    const cxd ttop(xch[0][0][lane], xch[0][1][lane]);
+    const cxd tbot(xch[1][0][lane], xch[1][1][lane]);
+    const cxd g = ((cxd((double)nd.s.x, (double)nd.s.y) + dq * vq) - ttop) - tbot;
+    xm = qsymv(w, g);
+  }
+
+  // ---------------- back substitution ----------------
+  E ru[D], ra[D];
+  if (side == 0) {
+    {
+      const long o = off_q + (long)m * G.sa;
+      st<T>(u, o, ld<T>(u, o) + xm);
+    }
+    const int last = m - 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = max(last - d, 1);
+      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      riw[d] = iw[pn];
+      ru[d] = u[off_q + (long)pn * G.sa];
+      ra[d] = u[G.oa + (long)pn * G.sa];
+    }
+    cxd an = xm;   // component q of x_{p+1}
+    for (int p0 = last; p0 >= 1; p0 -= D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 - d;
+        if (p >= 1) {
+          const QNode<RS> nd = ring[d];
+          const double iWe = (double)riw[d];
+          const E cu = ru[d], ca = ra[d];
+          {
+            const int pn = max(p - D, 1);
+            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[d] = iw[pn];
+            ru[d] = u[off_q + (long)pn * G.sa];
+            ra[d] = u[G.oa + (long)pn * G.sa];
+          }
+          cxd w[4];
+          cw(nd, w);
+          const cxd rd(nd.rd.x, nd.rd.y);
+          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+          const cxd va = qsum(an * vq);
+          const cxd t = an * (cq * iWe) + (va * rd) * vq;
+          const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
+          const cxd dv = qsum((an - ap) * vq);
+          const long o = (long)p * G.sa;
+          if (q == 0) st<T>(u, G.oa + o, cxd((double)ca.x, (double)ca.y) + (qp + dv * rd));
+          st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
+          an = ap;
+        }
+      }
+    }
+    // along-line edge 0, between the boundary node and node 1
+    const cxd rd0(fac[10 * stride].v[0], fac[10 * stride].v[1]);
+    const PS r0 = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (8 / VS) * stride) + 8 % VS);
+    const cxd q0 = cxd((double)r0.x, (double)r0.y) * rd0;
+    const cxd dv = qsum(an * vq);
+    if (q == 0) st<T>(u, G.oa, ld<T>(u, G.oa) + (q0 + dv * rd0));
+  } else {
+    const int last = m + 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int pn = min(last + d, M);
+      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      riw[d] = iw[pn - 1];
+      ru[d] = u[off_q + (long)pn * G.sa];
+      ra[d] = u[G.oa + (long)(pn - 1) * G.sa];
+    }
+    cxd xp = xm;   // component q of x_{p-1}
+    cxd rdm = rdmid, qm = qmid;
+    for (int p0 = last; p0 <= M; p0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        const int p = p0 + d;
+        if (p <= M) {
+          const QNode<RS> nd = ring[d];
+          const double iWe = (double)riw[d];
+          const E cu = ru[d], ca = ra[d];
+          {
+            const int pn = min(p + D, M);
+            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[d] = iw[pn - 1];
+            ru[d] = u[off_q + (long)pn * G.sa];
+            ra[d] = u[G.oa + (long)(pn - 1) * G.sa];
+          }
+          cxd w[4];
+          cw(nd, w);
+          const cxd rd(nd.rd.x, nd.rd.y);
+          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+          const cxd vx = qsum(xp * vq);
+          const cxd t = xp * (cq * iWe) + (vx * rdm) * vq;
+          const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
+          const cxd dv = qsum((ap - xp) * vq);
+          const long o = (long)p * G.sa;
+          if (q == 0) st<T>(u, G.oa + o - G.sa, cxd((double)ca.x, (double)ca.y) + (qm + dv * rdm));
+          st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
+          xp = ap;
+          rdm = rd;
+          qm = qp;
+        }
+      }
+    }
+    // along-line edge M = N-1, between node M and the boundary node
+    const cxd dv = qsum((cxd(0, 0) - xp) * vq);
+    const long o = G.oa + (long)M * G.sa;
+    if (q == 0) st<T>(u, o, ld<T>(u, o) + (qm + dv * rdm));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi) {
@@ -887,7 +1233,14 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
     else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
   }
-  {
+  if (maxl <= quad_max_lines<T>()) {
+    // few lines: four lanes per line half shorten the serial chain
+    constexpr int QD = EMMI_LINE_QUAD_RING;
+    const dim3 g((maxl + 15) / 16, P.njobs), b(128);
+    if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD>, g, b, s, timing::LINE, solve_bytes, P);
+    else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD>, g, b, s, timing::LINE, solve_bytes, P);
+    else launch_timed(k_line_solve_q<T, 2, QD>, g, b, s, timing::LINE, solve_bytes, P);
+  } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
     if (axis == 0) launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);
     else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, solve_bytes, P);

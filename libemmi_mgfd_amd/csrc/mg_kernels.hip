@@ -21,8 +21,8 @@
 
 namespace emmi {
 
-// block-wide sum of doubles into partial[blockIdx]; blockDim multiple of 64
-__device__ __forceinline__ void block_sum_store(double v, double* partial) {
+// block-wide sum of doubles into partial[blk]; blockDim multiple of 64
+__device__ __forceinline__ void block_sum_store(double v, double* partial, long blk) {
   __shared__ double red[16];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -32,7 +32,7 @@ __device__ __forceinline__ void block_sum_store(double v, double* partial) {
     double s = 0;
     const int nw = (blockDim.x + 63) >> 6;
     for (int w = 0; w < nw; w++) s += red[w];
-    partial[blockIdx.x] = s;
+    partial[blk] = s;
   }
 }
 
@@ -42,7 +42,8 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, const typename cve
                                                   const typename cvec<T>::type* __restrict__ f,
                                                   typename cvec<T>::type* __restrict__ r, double* partial) {
   const long npts = L.nc;
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long t = blk * blockDim.x + threadIdx.x;
   double acc = 0;
   if (t < npts) {
     const int i = t % L.es[1];
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, const typename cve
     st<T>(r, 2 * L.nc + t, v);
     acc += (double)v.re * v.re + (double)v.im * v.im;
   }
-  if (partial) block_sum_store(acc, partial);
+  if (partial) block_sum_store(acc, partial, blk);
 }
 
 __global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
@@ -90,7 +91,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X,
                                                   const typename cvec<T>::type* __restrict__ rf,
                                                   typename cvec<T>::type* __restrict__ fc) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= Cl.nc) return;
   const int P[3] = {(int)(t % Cl.es[1]), (int)((t / Cl.es[1]) % (Cl.n[1] + 1)), (int)(t / Cl.es[2])};
 #pragma unroll
@@ -133,7 +134,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(Level<T> F, Level<T> Cl, Transfer<T> X,
                                                  const typename cvec<T>::type* __restrict__ uc,
                                                  typename cvec<T>::type* __restrict__ uf) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= F.nc) return;
   const int p[3] = {(int)(t % F.es[1]), (int)((t / F.es[1]) % (F.n[1] + 1)), (int)(t / F.es[2])};
   int ic[3];
@@ -173,7 +174,7 @@ __global__ void k_fill_zero(typename cvec<T>::type* __restrict__ v, long n) {
 template <typename T>
 __global__ void k_scale_rhs(Level<T> L, const T* __restrict__ D0, const T* __restrict__ D1, const T* __restrict__ D2,
                             const typename cvec<T>::type* __restrict__ b, typename cvec<T>::type* __restrict__ f) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= L.nc) return;
   const int i = t % L.es[1], j = (t / L.es[1]) % (L.n[1] + 1), k = t / L.es[2];
   st<T>(f, t, ld<T>(b, t) * ((double)L.W[0][i] * (double)D1[j] * (double)D2[k]));
@@ -185,7 +186,7 @@ __global__ void k_scale_rhs(Level<T> L, const T* __restrict__ D0, const T* __res
 // (src/gmg.c:1414-1460); zero elsewhere
 template <typename T>
 __global__ void k_curl_h(Level<T> L, const typename cvec<T>::type* __restrict__ u, typename cvec<T>::type* __restrict__ h) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= L.nc) return;
   const int p[3] = {(int)(t % L.es[1]), (int)((t / L.es[1]) % (L.n[1] + 1)), (int)(t / L.es[2])};
   const bool in = p[0] < L.n[0] && p[1] < L.n[1] && p[2] < L.n[2];
