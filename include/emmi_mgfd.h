@@ -285,11 +285,18 @@ int emmi_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int emmi_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int emmi_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int emmi_device_sync(void);
+/* Line-substitution kernel of the following launches (no counterpart in the
+ * reference; for tests and tuning): 0 by line count (default), 1 one lane per
+ * line half, 2 four lanes per line half.  Both solve the same line systems
+ * exactly.  Graphs a solver captured earlier keep the kernels they captured.
+ * Returns the previous mode, or -1 for an unknown mode. */
+int emmi_set_line_kernel(int mode);
 /* kernel timing: accumulated milliseconds, launch counts and algorithmic
  * bytes (each input read once, each output written once per launch) per
  * kernel class (0 residual, 1 line sweep, 2 restriction, 3 prolongation,
- * 4 other), measured with HIP events on the stream of each launch while
- * enabled; enabling clears the counters */
+ * 4 other, 5 line right-hand sides), measured with HIP events recorded on
+ * the stream of each launch (by the dispatch itself for the line kernels)
+ * while enabled; enabling clears the counters */
 void emmi_timing_enable(int on);
 int emmi_timing_read(double *ms, long *count, double *bytes, int n);
 

@@ -218,6 +218,8 @@ int emmi_memcpy_d2d(void* dst, const void* src, size_t bytes) {
 }
 int emmi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
 
+int emmi_set_line_kernel(int mode) { return emmi::set_line_kernel(mode); }
+
 void emmi_timing_enable(int on) { timing::enable(on != 0); }
 int emmi_timing_read(double* ms, long* count, double* bytes, int n) { return timing::read(ms, count, bytes, n); }
 

@@ -61,6 +61,10 @@ inline void line_job_colour(LineJob<T>& J, int a, int colour) {
   line_colour_counts(J.L.n, a, colour, &J.nlo, &J.nhi);
 }
 
+// substitution kernel of later launches: 0 by line count, 1 one lane per
+// line half, 2 four lanes per line half; returns the previous mode or -1
+int set_line_kernel(int mode);
+
 template <typename T>
 void launch_line_factor(const LinePack<T>& P, int axis, hipStream_t s);
 template <typename T>

@@ -1168,6 +1168,15 @@ void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi) {
   if (b < c) { *nlo = nbv; *nhi = ncv; } else { *nlo = ncv; *nhi = nbv; }
 }
 
+static int g_line_kernel = 0;
+
+int set_line_kernel(int mode) {
+  if (mode < 0 || mode > 2) return -1;
+  const int prev = g_line_kernel;
+  g_line_kernel = mode;
+  return prev;
+}
+
 static size_t chunks_of(int values, size_t esize) {
   const size_t v = 16 / esize;
   return (values + v - 1) / v;
@@ -1233,7 +1242,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
     else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
   }
-  if (maxl <= quad_max_lines<T>()) {
+  if (g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>())) {
     // few lines: four lanes per line half shorten the serial chain
     constexpr int QD = EMMI_LINE_QUAD_RING;
     const dim3 g((maxl + 15) / 16, P.njobs), b(128);

@@ -102,6 +102,8 @@ def lib() -> C.CDLL:
         L.emmi_memcpy_d2d.restype = C.c_int
         L.emmi_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.emmi_device_sync.restype = C.c_int
+        L.emmi_set_line_kernel.restype = C.c_int
+        L.emmi_set_line_kernel.argtypes = [C.c_int]
         L.emmi_timing_enable.argtypes = [C.c_int]
         L.emmi_timing_read.restype = C.c_int
         L.emmi_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double), C.c_int]

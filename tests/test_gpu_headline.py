@@ -18,8 +18,59 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config2.npz")
 
 
+def solve_config2(precision, tol):
+    """The product's path on config2: grid, device injection, batched solve
+    of the three frequencies, device extraction.  Returns the carriers (with
+    dcal filled), the survey and the solve stats."""
+    s = configs.config2()
+    sc = carriers.ShotCarriers(s)
+    L = native.lib()
+    L.regrid_init(sc.acq, sc.emf)
+    g = sc.grid()
+    xs = sc.x_arrays()
+    S = native.Solver((s.n1, s.n2, s.n3), xs[0], xs[1], xs[2], g["sigma11"], g["sigma22"], g["sigma33"],
+                      list(sc.omegas), precision=precision, tol=tol, ncycle=200)
+    nb = 16 * sc.nvec
+    bs = [native.DeviceBuffer(nb) for _ in s.freqs]
+    xo = [native.DeviceBuffer(nb) for _ in s.freqs]
+    for f, b in enumerate(bs):
+        assert L.emmi_inject_source_dev(sc.acq, sc.emf, f, b.ptr, None) == 0
+    sts = S.solve_batch_device([b.ptr for b in bs], [x.ptr for x in xo], [b.ptr for b in bs])
+    for f in range(len(s.freqs)):
+        assert L.emmi_extract_emf_data_dev(sc.acq, sc.emf, f, xo[f].ptr, bs[f].ptr, None) == 0
+    return sc, s, sts
+
+
+@pytest.fixture(params=[0, 1, 2], ids=["auto", "one_lane", "quad_lane"])
+def line_kernel(request):
+    L = native.lib()
+    prev = L.emmi_set_line_kernel(request.param)
+    yield request.param
+    L.emmi_set_line_kernel(prev)
+
+
+def test_config2_mixed_matches_fp64_at_bench_tolerance():
+    """FP32-storage cycles (one-lane substitution on the two finest levels,
+    quad-lane below) against the FP64 solve at the benchmark's tolerance:
+    the signal channels agree within 1e-5."""
+    print("config2 batched solves, precision 1 and 0, tol 1e-6", flush=True)
+    sc1, s, st1 = solve_config2(1, 1e-6)
+    d1 = sc1.dcal.copy()
+    sc0, _, st0 = solve_config2(0, 1e-6)
+    print("cycles mixed", [t.cycles for t in st1], "fp64", [t.cycles for t in st0])
+    assert all(t.status == 0 for t in st1 + st0)
+    for ch, name in enumerate(s.chrec):
+        if name in ("Ey", "Hx"):
+            continue
+        a = np.abs(sc0.dcal[ch])
+        m = a > 1e-15
+        err = np.abs(d1[ch] - sc0.dcal[ch])[m] / a[m]
+        print(f"{name}: mixed vs fp64 max rel {err.max():.2e}")
+        assert err.max() < 1e-5, name
+
+
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10)])
-def test_config2_receivers_match_converged_reference(precision, tol):
+def test_config2_receivers_match_converged_reference(precision, tol, line_kernel):
     s = configs.config2()
     sc = carriers.ShotCarriers(s)
     L = native.lib()
@@ -37,7 +88,7 @@ def test_config2_receivers_match_converged_reference(precision, tol):
     xo = [native.DeviceBuffer(nb) for _ in s.freqs]
     for f, b in enumerate(bs):
         assert L.emmi_inject_source_dev(sc.acq, sc.emf, f, b.ptr, None) == 0
-    print(f"config2 batched solve precision {precision} tol {tol}", flush=True)
+    print(f"config2 batched solve precision {precision} tol {tol} line kernel {line_kernel}", flush=True)
     sts = S.solve_batch_device([b.ptr for b in bs], [x.ptr for x in xo], [b.ptr for b in bs])
     print("cycles", [st.cycles for st in sts], "status", [st.status for st in sts])
     assert all(st.status == 0 for st in sts)

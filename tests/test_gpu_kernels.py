@@ -24,6 +24,18 @@ def tiny():
     return s, g, n, L0
 
 
+@pytest.fixture(params=[1, 2], ids=["one_lane", "quad_lane"])
+def line_kernel(request):
+    """Runs a test with each line-substitution kernel (one lane or four lanes
+    per line half), whatever the line count would select."""
+    from libemmi_mgfd_amd import native
+    L = native.lib()
+    prev = L.emmi_set_line_kernel(request.param)
+    assert prev >= 0
+    yield request.param
+    L.emmi_set_line_kernel(prev)
+
+
 def make_solver(s, g, precision, freq_index=0, **kw):
     from libemmi_mgfd_amd import native
     return native.Solver((s.n1, s.n2, s.n3), g["x1"], g["x2"], g["x3"], g["sigma11"], g["sigma22"], g["sigma33"],
@@ -50,7 +62,7 @@ def conductive_level(s, g, n):
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("it", [0, 1])
-def test_line_gs_fp64_conductive(tiny, axis, it):
+def test_line_gs_fp64_conductive(tiny, axis, it, line_kernel):
     """Exact line solves agree to rounding where the blocks are well conditioned."""
     from libemmi_mgfd_amd import native
     s, g, n, _ = tiny
@@ -65,7 +77,7 @@ def test_line_gs_fp64_conductive(tiny, axis, it):
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_line_gs_fp64_mild_air(axis):
+def test_line_gs_fp64_mild_air(axis, line_kernel):
     """With a mildly resistive 'air' (100 ohm-m) the blocks are well
     conditioned and the exact solves agree to rounding."""
     s = configs.tiny(rho_air=1e2)
@@ -84,7 +96,7 @@ def test_line_gs_fp64_mild_air(axis):
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("it", [0, 1])
-def test_line_gs_fp64(tiny, axis, it):
+def test_line_gs_fp64(tiny, axis, it, line_kernel):
     """With 1e8 ohm-m air the line blocks are near-singular along discrete
     gradients (omega mu0 sigma_air ~ 1e-10 of the curl-curl scale): a random
     right-hand side drives the solution to ~1e6 and two exact solvers differ
@@ -105,7 +117,7 @@ def test_line_gs_fp64(tiny, axis, it):
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_line_gs_fp32(axis):
+def test_line_gs_fp32(axis, line_kernel):
     """FP32 storage with FP64 arithmetic on better-conditioned blocks (100
     ohm-m air): agreement to the FP32 rounding of the stored vectors times
     the remaining block conditioning."""
@@ -123,7 +135,7 @@ def test_line_gs_fp32(axis):
     assert rel(out, ref) < 1e-4
 
 
-def test_line_gs_fp32_air_finite(tiny):
+def test_line_gs_fp32_air_finite(tiny, line_kernel):
     """In 1e8 ohm-m air the FP32-stored blocks are singular to working
     precision; the FP64 arithmetic must still return finite values."""
     s, g, n, L0 = tiny
@@ -131,6 +143,30 @@ def test_line_gs_fp32_air_finite(tiny):
     for axis in range(3):
         out = S.check_op(1, 0, 0, axis, 0, random_field(n, 5), random_field(n, 6))
         assert np.all(np.isfinite(out))
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_line_kernels_agree(tiny, axis, precision, tol):
+    """The one-lane and quad-lane substitutions solve the same line systems
+    exactly: on well-conditioned blocks they agree to rounding (FP64) or to
+    the FP32 rounding of the stored records and vectors."""
+    from libemmi_mgfd_amd import native
+    s, g, n, _ = tiny
+    one, _ = conductive_level(s, g, n)
+    L = native.lib()
+    outs = []
+    prev = L.emmi_set_line_kernel(1)
+    try:
+        for mode in (1, 2):
+            L.emmi_set_line_kernel(mode)
+            S = native.Solver(n, g["x1"], g["x2"], g["x3"], one, one, one, O.omega_f32(s.freqs[0]),
+                              precision=precision)
+            outs.append(S.check_op(1, 0, 0, axis, 1, random_field(n, 21), random_field(n, 22)))
+    finally:
+        L.emmi_set_line_kernel(prev)
+    print("one-lane vs quad-lane rel", rel(outs[1], outs[0]))
+    assert rel(outs[1], outs[0]) < tol
 
 
 @pytest.mark.parametrize("pattern", [0, 1, 2])
@@ -162,7 +198,7 @@ def test_curl_fp64(tiny):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 1e-11), (1, 1e-8)])
-def test_solve_matches_oracle(tiny, precision, tol):
+def test_solve_matches_oracle(tiny, precision, tol, line_kernel):
     """Converged fields agree: both solvers run to a tight relative residual
     (1e-11 in FP64; 1e-8 with FP32 cycles, whose refinement stalls near the
     FP32 resolution of the near-singular air blocks)."""
