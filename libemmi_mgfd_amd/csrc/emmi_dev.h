@@ -67,6 +67,18 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
   return x * base + (x < extra ? x : extra) + q;
 }
 
+// (i, j, k) of the flat index t of a [n3+1][n2+1][n1+1] array with row
+// length es1 = n1+1 and n2p1 = n2+1 rows per plane, in 32-bit arithmetic
+// (level arrays hold fewer than 2^32 entries; 64-bit division is emulated
+// with long instruction sequences).
+__device__ __forceinline__ void ijk_of(long t, long es1, int n2p1, int& i, int& j, int& k) {
+  const unsigned tu = (unsigned)t, e1 = (unsigned)es1, m = (unsigned)n2p1;
+  const unsigned q = tu / e1;
+  i = (int)(tu - q * e1);
+  k = (int)(q / m);
+  j = (int)(q - (unsigned)k * m);
+}
+
 // Geometry of one multigrid level as the kernels see it.
 //   W[a][i]  cell width along axis a (i in [0, n_a])
 //   IW[a][i] 1/W[a][i]

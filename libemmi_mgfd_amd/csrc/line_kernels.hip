@@ -71,6 +71,18 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 #ifndef EMMI_LINE_QUAD_RING
 #define EMMI_LINE_QUAD_RING 3
 #endif
+// quad-lane launches of at most this many lines per job (FP32-storage /
+// FP64 cycles) compute their right-hand sides themselves (no k_line_rhs
+// launch)
+#ifndef EMMI_LINE_FUSE_MAX_LINES_F32
+#define EMMI_LINE_FUSE_MAX_LINES_F32 1024
+#endif
+#ifndef EMMI_LINE_FUSE_MAX_LINES_F64
+#define EMMI_LINE_FUSE_MAX_LINES_F64 4096
+#endif
+template <typename T> constexpr int fuse_max_lines() {
+  return sizeof(T) == 4 ? EMMI_LINE_FUSE_MAX_LINES_F32 : EMMI_LINE_FUSE_MAX_LINES_F64;
+}
 template <typename T> constexpr int quad_max_lines() {
   return sizeof(T) == 4 ? EMMI_LINE_QUAD_MAX_LINES_F32 : EMMI_LINE_QUAD_MAX_LINES_F64;
 }
@@ -336,19 +348,11 @@ __device__ __forceinline__ void symv(const Ch<R> (&w)[NC], const cxd (&y)[4], cx
 //   record p, slots 8..9: rho_p, the residual of the along-line edge p
 // Lines of one colour are not coupled, so these are the residuals each line
 // solve of the colour starts from.
+// the records of node p of line `line` (at (pb, pc)) of job J
 template <typename T, int A>
-__global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
+__device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, int pc, int p, long nl) {
   constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
-  const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
-  const long nl = (long)J.nlo * J.nhi;
-  const int N = L.n[A];
-  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= nl * N) return;
-  const int line = (int)(t % nl);
-  const int p = (int)(t / nl);
-  int pb, pc;
-  line_of<T, A>(J, line, pb, pc);
   const typename cvec<T>::type* __restrict__ u = J.u;
   const typename cvec<T>::type* __restrict__ f = J.f;
   const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
@@ -384,6 +388,21 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
   } else {
     rec[(8 / V) * nl] = c[8 / V];   // node 0: rho_0 only (values 8, 9: one chunk)
   }
+}
+
+template <typename T, int A>
+__global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const long nl = (long)J.nlo * J.nhi;
+  const int N = J.L.n[A];
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nl * N) return;
+  const unsigned pu = (unsigned)t / (unsigned)nl;   // fewer than 2^32 nodes per launch
+  const int p = (int)pu;
+  const int line = (int)((unsigned)t - pu * (unsigned)nl);
+  int pb, pc;
+  line_of<T, A>(J, line, pb, pc);
+  node_rhs<T, A>(J, line, pb, pc, p, nl);
 }
 
 // Right-hand sides for lines along x (axis 0): consecutive lines lie in
@@ -883,7 +902,10 @@ __device__ __forceinline__ void qload(QNode<RS>& n, const Ch<double>* __restrict
   n.rho = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + (8 / VS) * stride) + 8 % VS);
 }
 
-template <typename T, int A, int D>
+// With RHS the block first computes the records of its own 16 lines (the
+// work of k_line_rhs, for launches too small to pay for a launch of their
+// own) and then substitutes.
+template <typename T, int A, int D, bool RHS>
 __global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
   using E = typename cvec<T>::type;
   using RS = ScrRec<T>;
@@ -895,6 +917,15 @@ __global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
   const int N = L.n[A];
   if (N < 2) return;   // uniform over the block: no interior node
   const int M = N - 1, m = N / 2;
+  if (RHS) {
+    const long nl = (long)J.nlo * J.nhi;
+    for (int idx = threadIdx.x; idx < 16 * N; idx += blockDim.x) {
+      const int ln = blockIdx.x * 16 + (idx & 15), p = idx >> 4;
+      int b2, c2;
+      if (line_of<T, A>(J, ln, b2, c2)) node_rhs<T, A>(J, ln, b2, c2, p, nl);
+    }
+    __syncthreads();   // the block's records are complete (workgroup scope)
+  }
   const int lane = threadIdx.x & 63, side = threadIdx.x >> 6;
   const int q = lane & 3;
   const int line = blockIdx.x * 16 + (lane >> 2);
@@ -1231,7 +1262,11 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     else hipLaunchKernelGGL((k_line_factor<T, 2>), g, b, 0, s, P);
     return;
   }
-  if (axis == 0) {
+  const bool quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>());
+  const bool fuse = quad && maxl <= fuse_max_lines<T>();
+  if (fuse) {
+    // the substitution kernel computes its own right-hand sides
+  } else if (axis == 0) {
     int maxn = 0;
     for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
     const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
@@ -1242,13 +1277,20 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
     else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
   }
-  if (g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>())) {
+  if (quad) {
     // few lines: four lanes per line half shorten the serial chain
     constexpr int QD = EMMI_LINE_QUAD_RING;
     const dim3 g((maxl + 15) / 16, P.njobs), b(128);
-    if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD>, g, b, s, timing::LINE, solve_bytes, P);
-    else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD>, g, b, s, timing::LINE, solve_bytes, P);
-    else launch_timed(k_line_solve_q<T, 2, QD>, g, b, s, timing::LINE, solve_bytes, P);
+    if (fuse) {
+      const double by = rhs_bytes + solve_bytes;
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true>, g, b, s, timing::LINE, by, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true>, g, b, s, timing::LINE, by, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, true>, g, b, s, timing::LINE, by, P);
+    } else {
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
+    }
   } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
     if (axis == 0) launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);

@@ -46,9 +46,8 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, const typename cve
   const long t = blk * blockDim.x + threadIdx.x;
   double acc = 0;
   if (t < npts) {
-    const int i = t % L.es[1];
-    const int j = (t / L.es[1]) % (L.n[1] + 1);
-    const int k = t / L.es[2];
+    int i, j, k;
+    ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
     cxd v;
     v = ld<T>(f, t);
     if (interior<0>(L.n, i, j, k)) v -= apply_row<T, 0>(L, u, i, j, k, t);
@@ -93,7 +92,8 @@ __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Trans
                                                   typename cvec<T>::type* __restrict__ fc) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= Cl.nc) return;
-  const int P[3] = {(int)(t % Cl.es[1]), (int)((t / Cl.es[1]) % (Cl.n[1] + 1)), (int)(t / Cl.es[2])};
+  int P[3];
+  ijk_of(t, Cl.es[1], Cl.n[1] + 1, P[0], P[1], P[2]);
 #pragma unroll
   for (int A = 0; A < 3; A++) {
     const int Bx = (A + 1) % 3, Cx = (A + 2) % 3;
@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(256) k_prolong(Level<T> F, Level<T> Cl, Transf
                                                  typename cvec<T>::type* __restrict__ uf) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= F.nc) return;
-  const int p[3] = {(int)(t % F.es[1]), (int)((t / F.es[1]) % (F.n[1] + 1)), (int)(t / F.es[2])};
+  int p[3];
+  ijk_of(t, F.es[1], F.n[1] + 1, p[0], p[1], p[2]);
   int ic[3];
   double wl[3], wr[3];
 #pragma unroll
@@ -176,7 +177,8 @@ __global__ void k_scale_rhs(Level<T> L, const T* __restrict__ D0, const T* __res
                             const typename cvec<T>::type* __restrict__ b, typename cvec<T>::type* __restrict__ f) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= L.nc) return;
-  const int i = t % L.es[1], j = (t / L.es[1]) % (L.n[1] + 1), k = t / L.es[2];
+  int i, j, k;
+  ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
   st<T>(f, t, ld<T>(b, t) * ((double)L.W[0][i] * (double)D1[j] * (double)D2[k]));
   st<T>(f, L.nc + t, ld<T>(b, L.nc + t) * ((double)D0[i] * (double)L.W[1][j] * (double)D2[k]));
   st<T>(f, 2 * L.nc + t, ld<T>(b, 2 * L.nc + t) * ((double)D0[i] * (double)D1[j] * (double)L.W[2][k]));
@@ -188,7 +190,8 @@ template <typename T>
 __global__ void k_curl_h(Level<T> L, const typename cvec<T>::type* __restrict__ u, typename cvec<T>::type* __restrict__ h) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= L.nc) return;
-  const int p[3] = {(int)(t % L.es[1]), (int)((t / L.es[1]) % (L.n[1] + 1)), (int)(t / L.es[2])};
+  int p[3];
+  ijk_of(t, L.es[1], L.n[1] + 1, p[0], p[1], p[2]);
   const bool in = p[0] < L.n[0] && p[1] < L.n[1] && p[2] < L.n[2];
   const double io = 1.0 / L.omu;
 #pragma unroll
@@ -255,7 +258,8 @@ __global__ void k_edge_sigma(int n1, int n2, int n3, const double* __restrict__ 
   const long es1 = n1 + 1, es2 = (long)(n1 + 1) * (n2 + 1), nc = es2 * (n3 + 1);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= nc) return;
-  const int p[3] = {(int)(t % es1), (int)((t / es1) % (n2 + 1)), (int)(t / es2)};
+  int p[3];
+  ijk_of(t, es1, n2 + 1, p[0], p[1], p[2]);
   const int n[3] = {n1, n2, n3};
   const double* s[3] = {s11, s22, s33};
 #pragma unroll
