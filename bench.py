@@ -37,10 +37,10 @@ HBM_PEAK_GBS = 8000.0
 # the mixed+batched 128^3 step with two rocprofv3 --pmc passes (FETCH_SIZE,
 # then WRITE_SIZE, kernels k_line_solve and the runtime's copy kernel), each
 # scaled by the same counter on the 103 MB device-to-device copies of that run
-# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 46.9 MB fetched + 12.1 MB written per
-# launch against 29.8 MB algorithmic.  The two passes of the substitution read
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 50.8 MB fetched + 12.6 MB written per
+# launch against 33.0 MB algorithmic.  The two passes of the substitution read
 # the factors twice and write and re-read z once.
-LINE_SOLVE_TRAFFIC_RATIO = (46.94e6 + 12.14e6) / 29.80e6
+LINE_SOLVE_TRAFFIC_RATIO = (50.80e6 + 12.62e6) / 33.00e6
 
 
 def parse():
