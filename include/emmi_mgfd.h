@@ -223,6 +223,23 @@ int emmi_solver_check_op(emmi_solver_t *s, int op, int pattern, int lev, int axi
 int emmi_inject_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *b_dev, void *stream);
 int emmi_extract_emf_data_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_dev, const void *h_dev,
                               void *stream);
+/* extract_electric_field (src/inject_extract.c:477-635) on the device: E of
+ * the device field vector x_dev (complex double, 3(n1+1)(n2+1)(n3+1)) on the
+ * model's cell centres, written to e_dev as complex float [3][nz][ny][nx]
+ * (E1, E2, E3) instead of emf->E1..E3[ifreq]; the medium is read from emf
+ * (sigma33 of the multigrid, rho33 of the model).  Same arithmetic as the
+ * host entry point, bit for bit. */
+int emmi_extract_electric_field_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_dev, void *e_dev,
+                                    void *stream);
+/* build_gradient (src/build_gradient.c:13-111) on the device: ef_dev[f] and
+ * ea_dev[f] (f < emf->nfreq <= 32) are the forward and adjoint fields of
+ * frequency f as emmi_extract_electric_field_dev writes them; grad_dev and,
+ * when fwi->preco == 1, phess_dev receive [npar][nz][ny][nx] float
+ * (npar <= 8) instead of fwi->grad / fwi->phess, mutes included.  The
+ * gradient is bit-identical to the host entry point; the pseudo-Hessian uses
+ * the device hypot and agrees to float rounding. */
+int emmi_build_gradient_dev(acq_t *acq, emf_t *emf, fwi_t *fwi, const void *const *ef_dev, const void *const *ea_dev,
+                            float *grad_dev, float *phess_dev, void *stream);
 const char *emmi_path_last_error(void);
 
 /* ------------------------------------------------------------------------- */

@@ -740,19 +740,27 @@ inline float creal_mul_f(emmi_c64 a, emmi_c64 b) {
 inline double cabs_d(emmi_c64 a) { return hypot((double)a.re, (double)a.im); }
 
 void mute_box(const emf_t* emf, const fwi_t* fwi, float px, float py, float pz, float r) {
+  int b[6];
+  emmi::mute_bounds(emf, px, py, pz, r, b);
+  for (int k = 0; k < fwi->npar; k++)
+    for (int i3 = b[4]; i3 <= b[5]; i3++)
+      for (int i2 = b[2]; i2 <= b[3]; i2++)
+        for (int i1 = b[0]; i1 <= b[1]; i1++) fwi->grad[k][i3][i2][i1] = 0.f;
+}
+}  // namespace
+
+namespace emmi {
+// cells within r of (px, py, pz), clipped to the model (src/build_gradient.c:64-111)
+void mute_bounds(const emf_t* emf, float px, float py, float pz, float r, int* b) {
   int i1min = (int)((px - r - emf->ox) / emf->dx), i1max = (int)((px + r - emf->ox) / emf->dx);
   int i2min = (int)((py - r - emf->oy) / emf->dy), i2max = (int)((py + r - emf->oy) / emf->dy);
   int i3min = (int)((pz - r - emf->oz) / emf->dz), i3max = (int)((pz + r - emf->oz) / emf->dz);
   auto clip = [](int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); };
-  i1min = clip(i1min, emf->nx); i1max = clip(i1max, emf->nx);
-  i2min = clip(i2min, emf->ny); i2max = clip(i2max, emf->ny);
-  i3min = clip(i3min, emf->nz); i3max = clip(i3max, emf->nz);
-  for (int k = 0; k < fwi->npar; k++)
-    for (int i3 = i3min; i3 <= i3max; i3++)
-      for (int i2 = i2min; i2 <= i2max; i2++)
-        for (int i1 = i1min; i1 <= i1max; i1++) fwi->grad[k][i3][i2][i1] = 0.f;
+  b[0] = clip(i1min, emf->nx); b[1] = clip(i1max, emf->nx);
+  b[2] = clip(i2min, emf->ny); b[3] = clip(i2max, emf->ny);
+  b[4] = clip(i3min, emf->nz); b[5] = clip(i3max, emf->nz);
 }
-}  // namespace
+}  // namespace emmi
 
 extern "C" void build_gradient(acq_t* acq, emf_t* emf, fwi_t* fwi, emmi_c64**** E1f, emmi_c64**** E2f,
                                emmi_c64**** E3f, emmi_c64**** E1a, emmi_c64**** E2a, emmi_c64**** E3a) {

@@ -36,4 +36,6 @@ void receiver_indices(acq_t* acq, emf_t* emf, std::vector<long>& ix, std::vector
 // extract_emf_data from previously gathered entries (same order as receiver_indices)
 void receiver_data_from(acq_t* acq, emf_t* emf, int ifreq, const std::vector<emmi_c128>& vx,
                         const std::vector<emmi_c128>& vb);
+// gradient mute box around a point: b = i1min, i1max, i2min, i2max, i3min, i3max
+void mute_bounds(const emf_t* emf, float px, float py, float pz, float r, int* b);
 }  // namespace emmi

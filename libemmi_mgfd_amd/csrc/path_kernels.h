@@ -1,4 +1,5 @@
-// path_kernels.h -- launchers of the source-scatter and receiver-gather kernels.
+// path_kernels.h -- launchers of the kernels around the solve: source scatter,
+// receiver gathers, E on the model grid and the gradient.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,4 +8,34 @@ namespace emmi {
 void launch_zero_c128(void* v, long n, hipStream_t s);
 void launch_scatter_set(void* v, const long* idx, const void* val, int n, hipStream_t s);
 void launch_gather(const void* v, const long* idx, void* out, int n, hipStream_t s);
+
+// per-axis cell-centre interpolation tables (device pointers), as
+// CentreTables holds them on the host
+struct CentreDev {
+  const int *i1, *i1s, *i2, *i2s, *i3, *i3s;
+  const float *w1, *w1s, *w2, *w2s, *w3, *w3s;
+  int nx, ny, nz;
+};
+// E1, E2, E3 of one frequency on the model's cell centres: e = [3][nz][ny][nx]
+// complex float, from the field vector x (complex double) of a grid n1 x n2 x n3;
+// s33: MG-grid sigma33 [n3][n2][n1]; r33: model rho33 [nz][ny][nx]
+void launch_efield(const CentreDev& T, const void* x, int n1, int n2, int n3, const float* s33, const float* r33,
+                   void* e, hipStream_t s);
+
+constexpr int GRAD_MAX_FREQ = 32;
+constexpr int GRAD_MAX_PAR = 8;
+struct GradArgs {
+  const void* ef[GRAD_MAX_FREQ];   // forward E per frequency, [3][nz][ny][nx] complex float
+  const void* ea[GRAD_MAX_FREQ];   // adjoint E per frequency
+  int nfreq, npar, preco;
+  int idxpar[GRAD_MAX_PAR];
+  int nx, ny, nz;
+  float s, dzf;                    // dx*dy*dz; dz
+  double oz, dz;
+  const float *rho11, *rho33, *bathy;
+  float *grad, *phess;             // [npar][nz][ny][nx]
+};
+void launch_gradient(const GradArgs& a, hipStream_t s);
+// zero grad[*][box] for each box (i1min, i1max, i2min, i2max, i3min, i3max)
+void launch_mute(float* grad, int npar, int nx, int ny, int nz, const int* boxes, int nbox, int maxvol, hipStream_t s);
 }  // namespace emmi
