@@ -223,6 +223,13 @@ int emmi_solver_check_op(emmi_solver_t *s, int op, int pattern, int lev, int axi
 int emmi_inject_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *b_dev, void *stream);
 int emmi_extract_emf_data_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_dev, const void *h_dev,
                               void *stream);
+/* inject_adj_source (src/inject_extract.c:124-314) on the device: the
+ * adjoint source of the residuals emf->dres_fd[.][ifreq][.] into b_dev, the
+ * magnetic dipole M of the H channels into x_dev (both complex double,
+ * 3(n1+1)(n2+1)(n3+1)).  The scatters are accumulated on the host in the
+ * reference's order, b += curl M runs on the device; bit-identical to the
+ * host entry point. */
+int emmi_inject_adj_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *x_dev, void *b_dev, void *stream);
 /* extract_electric_field (src/inject_extract.c:477-635) on the device: E of
  * the device field vector x_dev (complex double, 3(n1+1)(n2+1)(n3+1)) on the
  * model's cell centres, written to e_dev as complex float [3][nz][ny][nx]

@@ -469,12 +469,13 @@ extern "C" void inject_source(acq_t* acq, emf_t* emf, emmi_c128* b, int ifreq) {
   emmi::source_terms(acq, emf, ifreq, sk);
 }
 
-// src/inject_extract.c:124-314
-extern "C" void inject_adj_source(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128* b, int ifreq) {
+namespace emmi {
+// the scatters of src/inject_extract.c:133-302: E-channel residuals into b,
+// H-channel residuals into the magnetic dipole M (x), in the reference's order
+template <typename SinkB, typename SinkX>
+void adj_terms(acq_t* acq, emf_t* emf, int ifreq, SinkB& sb, SinkX& sx) {
   const long n123 = n123_of(emf);
-  const int n1 = emf->n1, n2 = emf->n2, n3 = emf->n3;
-  memset(b, 0, sizeof(emmi_c128) * 3 * n123);
-  memset(x, 0, sizeof(emmi_c128) * 3 * n123);
+  const int n1 = emf->n1, n2 = emf->n2;
   C* amp = (C*)xmalloc(sizeof(C) * (emf->nchrec + 1));
   const C iwm = i_omega_mu0(emf->omegas[ifreq]);
   for (int ir = 0; ir < acq->nrec; ir++) {
@@ -486,17 +487,35 @@ extern "C" void inject_adj_source(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c12
     moment(emf->nchrec, emf->chrec, 0, amp, acq->rec_azimuth[ir], acq->rec_dip[ir], F);
     for (int comp = 0; comp < 3; comp++) {
       const Tri t = emmi::locate(emf, comp, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
-      DenseSink sb{b};
       scatter8(sb, comp * n123, t, n1, n2, cdivr(cmul(iwm, F[comp]), (double)t.vol));
     }
     moment(emf->nchrec, emf->chrec, 3, amp, acq->rec_azimuth[ir], acq->rec_dip[ir], F);
     for (int comp = 0; comp < 3; comp++) {
       const Tri t = emmi::locate(emf, 3 + comp, acq->rec_x1[ir], acq->rec_x2[ir], acq->rec_x3[ir]);
-      DenseSink sx{x};
       scatter8(sx, comp * n123, t, n1, n2, cdivr(F[comp], (double)t.vol));
     }
   }
   free(amp);
+}
+
+void adj_entries(acq_t* acq, emf_t* emf, int ifreq, std::vector<long>& bidx, std::vector<emmi_c128>& bval,
+                 std::vector<long>& xidx, std::vector<emmi_c128>& xval) {
+  SparseSink sb, sx;
+  adj_terms(acq, emf, ifreq, sb, sx);
+  bidx.clear(); bval.clear(); xidx.clear(); xval.clear();
+  for (auto& kv : sb.m) { bidx.push_back(kv.first); bval.push_back(emmi_c128{kv.second.re, kv.second.im}); }
+  for (auto& kv : sx.m) { xidx.push_back(kv.first); xval.push_back(emmi_c128{kv.second.re, kv.second.im}); }
+}
+}  // namespace emmi
+
+// src/inject_extract.c:124-314
+extern "C" void inject_adj_source(acq_t* acq, emf_t* emf, emmi_c128* x, emmi_c128* b, int ifreq) {
+  const long n123 = n123_of(emf);
+  const int n1 = emf->n1, n2 = emf->n2, n3 = emf->n3;
+  memset(b, 0, sizeof(emmi_c128) * 3 * n123);
+  memset(x, 0, sizeof(emmi_c128) * 3 * n123);
+  DenseSink sb{b}, sx{x};
+  emmi::adj_terms(acq, emf, ifreq, sb, sx);
   // b += curl M with the dual widths (mu_r = 1)
   const long s1 = n1 + 1, s2 = (long)(n1 + 1) * (n2 + 1);
   for (int k = 1; k < n3 + 1; k++)

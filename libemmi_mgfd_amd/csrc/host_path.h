@@ -36,6 +36,11 @@ void receiver_indices(acq_t* acq, emf_t* emf, std::vector<long>& ix, std::vector
 // extract_emf_data from previously gathered entries (same order as receiver_indices)
 void receiver_data_from(acq_t* acq, emf_t* emf, int ifreq, const std::vector<emmi_c128>& vx,
                         const std::vector<emmi_c128>& vb);
+// the nonzero entries of the adjoint source's scatters (src/inject_extract.c:
+// 133-302): b (E channels) and the magnetic dipole M (H channels), accumulated
+// in the reference's order; b += curl M follows on the device
+void adj_entries(acq_t* acq, emf_t* emf, int ifreq, std::vector<long>& bidx, std::vector<emmi_c128>& bval,
+                 std::vector<long>& xidx, std::vector<emmi_c128>& xval);
 // gradient mute box around a point: b = i1min, i1max, i2min, i2max, i3min, i3max
 void mute_bounds(const emf_t* emf, float px, float py, float pz, float r, int* b);
 }  // namespace emmi

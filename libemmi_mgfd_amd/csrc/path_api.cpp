@@ -92,6 +92,46 @@ extern "C" int emmi_extract_emf_data_dev(acq_t* acq, emf_t* emf, int ifreq, cons
   }
 }
 
+// src/inject_extract.c:124-314
+extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, void* x_dev, void* b_dev, void* stream) {
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<long> bi, xi;
+    std::vector<emmi_c128> bv, xv;
+    emmi::adj_entries(acq, emf, ifreq, bi, bv, xi, xv);
+    const int n1 = emf->n1, n2 = emf->n2, n3 = emf->n3;
+    const long n = 3L * (n1 + 1) * (n2 + 1) * (n3 + 1);
+    for (long i : bi) if (i < 0 || i >= n) throw std::runtime_error("adjoint source entry outside the vector");
+    for (long i : xi) if (i < 0 || i >= n) throw std::runtime_error("adjoint source entry outside the vector");
+    const size_t nb = bi.size(), nx = xi.size();
+    long* di = (long*)g_idx.get(sizeof(long) * (nb + nx + 1));
+    emmi_c128* dv = (emmi_c128*)g_val.get(sizeof(emmi_c128) * (nb + nx + 1));
+    std::vector<long> idx(bi);
+    idx.insert(idx.end(), xi.begin(), xi.end());
+    std::vector<emmi_c128> val(bv);
+    val.insert(val.end(), xv.begin(), xv.end());
+    check(hipMemcpyAsync(di, idx.data(), sizeof(long) * idx.size(), hipMemcpyHostToDevice, s), "h2d");
+    check(hipMemcpyAsync(dv, val.data(), sizeof(emmi_c128) * val.size(), hipMemcpyHostToDevice, s), "h2d");
+    std::vector<float> dd;
+    dd.insert(dd.end(), emf->d1, emf->d1 + n1 + 1);
+    dd.insert(dd.end(), emf->d2, emf->d2 + n2 + 1);
+    dd.insert(dd.end(), emf->d3, emf->d3 + n3 + 1);
+    float* dw = (float*)g_tab.get(sizeof(float) * dd.size());
+    check(hipMemcpyAsync(dw, dd.data(), sizeof(float) * dd.size(), hipMemcpyHostToDevice, s), "h2d");
+    emmi::launch_zero_c128(b_dev, n, s);
+    emmi::launch_zero_c128(x_dev, n, s);
+    emmi::launch_scatter_set(b_dev, di, dv, (int)nb, s);
+    emmi::launch_scatter_set(x_dev, di + nb, dv + nb, (int)nx, s);
+    emmi::launch_add_curl_m(b_dev, x_dev, n1, n2, n3, dw, dw + n1 + 1, dw + n1 + 1 + n2 + 1, s);
+    check(hipStreamSynchronize(s), "sync");
+    check(hipGetLastError(), "adjoint source");
+    return 0;
+  } catch (const std::exception& e) {
+    g_path_err = e.what();
+    return -1;
+  }
+}
+
 namespace {
 template <typename V>
 void put(char* dev, size_t& off, const std::vector<V>& h, const V*& out, hipStream_t s) {

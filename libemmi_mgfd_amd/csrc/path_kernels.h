@@ -22,6 +22,11 @@ struct CentreDev {
 void launch_efield(const CentreDev& T, const void* x, int n1, int n2, int n3, const float* s33, const float* r33,
                    void* e, hipStream_t s);
 
+// b += curl M over the nodes 1..n of each axis (src/inject_extract.c:303-312);
+// m = the dipole vector, d1..d3 the dual widths (n+1 floats each)
+void launch_add_curl_m(void* b, const void* m, int n1, int n2, int n3, const float* d1, const float* d2,
+                       const float* d3, hipStream_t s);
+
 constexpr int GRAD_MAX_FREQ = 32;
 constexpr int GRAD_MAX_PAR = 8;
 struct GradArgs {

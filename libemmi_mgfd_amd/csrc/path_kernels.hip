@@ -87,6 +87,27 @@ __global__ void k_efield(CentreDev T, const double2* __restrict__ x, long n123, 
   e[2 * nxyz + o] = make_float2((float)ez.re, (float)ez.im);
 }
 
+// b += curl M with the dual widths (mu_r = 1): one thread per node, which
+// owns the three entries of b at that node
+__global__ void k_add_curl_m(double2* __restrict__ b, const double2* __restrict__ m, int n1, int n2, int n3,
+                             const float* __restrict__ d1, const float* __restrict__ d2, const float* __restrict__ d3) {
+#pragma clang fp contract(off)
+  const long nn = (long)n1 * n2 * n3;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nn) return;
+  const int i = 1 + (int)(t % n1), j = 1 + (int)((t / n1) % n2), k = 1 + (int)(t / ((long)n1 * n2));
+  const long s1 = n1 + 1, s2 = (long)(n1 + 1) * (n2 + 1), n123 = s2 * (n3 + 1);
+  const long o = i + s1 * j + s2 * k;
+  const double w1 = d1[i], w2 = d2[j], w3 = d3[k];
+  const cd mx = ldz(m, o), my = ldz(m, n123 + o), mz = ldz(m, 2 * n123 + o);
+  auto sub = [](cd a, cd c) { return cd{a.re - c.re, a.im - c.im}; };
+  auto divr = [](cd a, double r) { return cd{a.re / r, a.im / r}; };
+  auto add = [&](long at, cd v) { const double2 h = b[at]; b[at] = make_double2(h.x + v.re, h.y + v.im); };
+  add(o, sub(divr(sub(mz, ldz(m, 2 * n123 + o - s1)), w2), divr(sub(my, ldz(m, n123 + o - s2)), w3)));
+  add(n123 + o, sub(divr(sub(mx, ldz(m, o - s2)), w3), divr(sub(mz, ldz(m, 2 * n123 + o - 1)), w1)));
+  add(2 * n123 + o, sub(divr(sub(my, ldz(m, n123 + o - 1)), w1), divr(sub(mx, ldz(m, o - s1)), w2)));
+}
+
 // real part of a float complex product, in float
 __device__ __forceinline__ float re_mul(float2 a, float2 b) {
 #pragma clang fp contract(off)
@@ -156,6 +177,12 @@ void launch_efield(const CentreDev& T, const void* x, int n1, int n2, int n3, co
   const long n123 = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
   hipLaunchKernelGGL(k_efield, dim3(nb(nxyz)), dim3(256), 0, s, T, (const double2*)x, n123, n1, n2, s33, r33,
                      (float2*)e);
+}
+
+void launch_add_curl_m(void* b, const void* m, int n1, int n2, int n3, const float* d1, const float* d2,
+                       const float* d3, hipStream_t s) {
+  const long nn = (long)n1 * n2 * n3;
+  hipLaunchKernelGGL(k_add_curl_m, dim3(nb(nn)), dim3(256), 0, s, (double2*)b, (const double2*)m, n1, n2, n3, d1, d2, d3);
 }
 
 void launch_gradient(const GradArgs& a, hipStream_t s) {

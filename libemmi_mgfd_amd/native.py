@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         L.emmi_extract_emf_data_dev.restype = C.c_int
         L.emmi_extract_emf_data_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_path_last_error.restype = C.c_char_p
+        L.emmi_inject_adj_source_dev.restype = C.c_int
+        L.emmi_inject_adj_source_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_extract_electric_field_dev.restype = C.c_int
         L.emmi_extract_electric_field_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_build_gradient_dev.restype = C.c_int

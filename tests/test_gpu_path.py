@@ -1,10 +1,10 @@
-"""The device-resident steps after the solve against the product's host entry
-points, which are themselves bit-exact with the reference's fixtures
-(test_host_path.py): E on the model's cell centres
-(src/inject_extract.c:477-635) and the gradient with its pseudo-Hessian and
-mutes (src/build_gradient.c:13-111).  E and the gradient must match bit for
-bit; the pseudo-Hessian goes through the device hypot and is held to float
-rounding (2 ulp)."""
+"""The device-resident steps of the gradient path against the product's host
+entry points, which are themselves bit-exact with the reference's fixtures
+(test_host_path.py): the adjoint source (src/inject_extract.c:124-314), E on
+the model's cell centres (src/inject_extract.c:477-635) and the gradient with
+its pseudo-Hessian and mutes (src/build_gradient.c:13-111).  The adjoint
+source, E and the gradient must match bit for bit; the pseudo-Hessian goes
+through the device hypot and is held to float rounding (2 ulp)."""
 import ctypes as C
 import os
 
@@ -20,6 +20,22 @@ pytestmark = pytest.mark.gpu
 def _check(rc):
     if rc != 0:
         raise native.NativeError(native.lib().emmi_path_last_error().decode())
+
+
+def test_inject_adj_source_dev_matches_reference_fixture():
+    s, gd = _grad_survey()
+    sc = regridded(s)
+    nch, nf = len(s.chrec), len(s.freqs)
+    sc.dres[...] = gd["dres"].reshape(nch, nf, -1)
+    b = np.zeros(sc.nvec, np.complex128)
+    x = np.zeros(sc.nvec, np.complex128)
+    native.lib().inject_adj_source(sc.acq, sc.emf, x.ctypes.data, b.ctypes.data, 0)
+    bd, xd = native.DeviceBuffer(b.nbytes), native.DeviceBuffer(x.nbytes)
+    print(f"adjoint source, tiny survey, {len(s.receivers)} receivers", flush=True)
+    _check(native.lib().emmi_inject_adj_source_dev(sc.acq, sc.emf, 0, xd.ptr, bd.ptr, None))
+    np.testing.assert_array_equal(bd.download(np.complex128), b)
+    np.testing.assert_array_equal(bd.download(np.complex128), gd["badj_f0"])
+    np.testing.assert_array_equal(xd.download(np.complex128), x)
 
 
 def test_extract_electric_field_dev_matches_host():
