@@ -277,7 +277,7 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
     DevVec<T> fine(nv), coarse(3 * C.nc);
     if (op == 2) {
       fine.up(in1);
-      launch_restrict<T>(V, VC, C.transfer(), fine.p, coarse.p, s);
+      launch_restrict<T>(V, VC, C.transfer(), fine.p, coarse.p, nullptr, s);
       check_hip(hipStreamSynchronize(s), "sync");
       coarse.down(out);
     } else {

@@ -18,7 +18,9 @@ void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, d
 int residual_blocks(long nc);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 template <typename T>
-void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s);
+// fc = R rf; uc (may be null) is zeroed on the coarse level in the same pass
+void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, void* uc,
+                     hipStream_t s);
 template <typename T>
 void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s);
 template <typename T>

@@ -85,11 +85,14 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // wr[a][q]: weight of the fine node q towards coarse node q/sc + 1 on axis a
 // (the reference's w?r); its complement goes to coarse node q/sc.
 // ---------------------------------------------------------------------------
-// restriction, gather form: one thread per coarse index, three components
+// restriction, gather form: one thread per coarse index, three components;
+// uc, when given, is the coarse correction, zeroed at the same indices (the
+// coarse solve starts from zero)
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X,
                                                   const typename cvec<T>::type* __restrict__ rf,
-                                                  typename cvec<T>::type* __restrict__ fc) {
+                                                  typename cvec<T>::type* __restrict__ fc,
+                                                  typename cvec<T>::type* __restrict__ uc) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= Cl.nc) return;
   int P[3];
@@ -126,6 +129,7 @@ __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Trans
           }
     }
     st<T>(fc, A * Cl.nc + t, acc);
+    if (uc) st<T>(uc, A * Cl.nc + t, cxd(0, 0));
   }
 }
 
@@ -298,10 +302,11 @@ void launch_sum_partials(const double* partial, int n, double* out, hipStream_t 
 }
 
 template <typename T>
-void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, hipStream_t s) {
+void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, void* uc,
+                     hipStream_t s) {
   using V = typename cvec<T>::type;
   timing::Scope ts(timing::RESTRICT, s, 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)rf, (V*)fc);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)rf, (V*)fc, (V*)uc);
 }
 
 template <typename T>
@@ -354,7 +359,7 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 
 #define EMMI_INSTANTIATE(T)                                                                                    \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
-  template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, \
+  template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, void*, \
                                    hipStream_t);                                                               \
   template void launch_prolong<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*,  \
                                   hipStream_t);                                                                \

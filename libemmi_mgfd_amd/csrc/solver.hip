@@ -505,8 +505,8 @@ bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess
   }
   for (int k : act_) {
     Slot& S = slots_[k];
-    launch_restrict<T>(L.view(S.omu), C.view(S.omu), C.transfer(), from_f ? S.f[lev] : S.r[lev], S.f[lev + 1], s_);
-    launch_zero<T>(S.u[lev + 1], 3 * C.nc, s_);
+    launch_restrict<T>(L.view(S.omu), C.view(S.omu), C.transfer(), from_f ? S.f[lev] : S.r[lev], S.f[lev + 1],
+                       S.u[lev + 1], s_);
   }
   if (kind == 2) f_cycle<T>(pat, lev + 1, false, true);
   else if (kind == 3) { w_cycle<T>(pat, lev + 1, false, true); w_cycle<T>(pat, lev + 1, false, false); }
