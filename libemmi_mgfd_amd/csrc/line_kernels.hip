@@ -80,6 +80,11 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 #ifndef EMMI_LINE_FUSE_MAX_LINES_F64
 #define EMMI_LINE_FUSE_MAX_LINES_F64 4096
 #endif
+// threads of a block of the fused launches (a multiple of 128; the waves
+// beyond the first two only compute right-hand sides)
+#ifndef EMMI_LINE_FUSE_THREADS
+#define EMMI_LINE_FUSE_THREADS 512
+#endif
 template <typename T> constexpr int fuse_max_lines() {
   return sizeof(T) == 4 ? EMMI_LINE_FUSE_MAX_LINES_F32 : EMMI_LINE_FUSE_MAX_LINES_F64;
 }
@@ -904,9 +909,10 @@ __device__ __forceinline__ void qload(QNode<RS>& n, const Ch<double>* __restrict
 
 // With RHS the block first computes the records of its own 16 lines (the
 // work of k_line_rhs, for launches too small to pay for a launch of their
-// own) and then substitutes.
-template <typename T, int A, int D, bool RHS>
-__global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
+// own) with all its BT threads, and then its first two waves substitute;
+// the other waves only help with the right-hand sides.
+template <typename T, int A, int D, bool RHS, int BT>
+__global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   using E = typename cvec<T>::type;
   using RS = ScrRec<T>;
   using PS = typename cvec<RS>::type;
@@ -930,7 +936,7 @@ __global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
   const int q = lane & 3;
   const int line = blockIdx.x * 16 + (lane >> 2);
   int pb = 0, pc = 0;
-  const bool live = line_of<T, A>(J, line, pb, pc);
+  const bool live = side < 2 && line_of<T, A>(J, line, pb, pc);
   LineGeo<T, A> G;
   if (live) G.init(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
@@ -1062,8 +1068,10 @@ __global__ void __launch_bounds__(128) k_line_solve_q(LinePack<T> P) {
   }
 
   // ---------------- meeting node ----------------
-  xch[side][0][lane] = tm.re;
-  xch[side][1][lane] = tm.im;
+  if (side < 2) {
+    xch[side][0][lane] = tm.re;
+    xch[side][1][lane] = tm.im;
+  }
   __syncthreads();
   if (!live) return;
   cxd xm, qmid, rdmid;
@@ -1279,17 +1287,19 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
   }
   if (quad) {
     // few lines: four lanes per line half shorten the serial chain
-    constexpr int QD = EMMI_LINE_QUAD_RING;
-    const dim3 g((maxl + 15) / 16, P.njobs), b(128);
+    constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
+    const dim3 g((maxl + 15) / 16, P.njobs);
     if (fuse) {
       const double by = rhs_bytes + solve_bytes;
-      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true>, g, b, s, timing::LINE, by, P);
-      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true>, g, b, s, timing::LINE, by, P);
-      else launch_timed(k_line_solve_q<T, 2, QD, true>, g, b, s, timing::LINE, by, P);
+      const dim3 b(FT);
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT>, g, b, s, timing::LINE, by, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT>, g, b, s, timing::LINE, by, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, true, FT>, g, b, s, timing::LINE, by, P);
     } else {
-      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
-      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
-      else launch_timed(k_line_solve_q<T, 2, QD, false>, g, b, s, timing::LINE, solve_bytes, P);
+      const dim3 b(128);
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
     }
   } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
