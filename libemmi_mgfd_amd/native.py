@@ -87,6 +87,8 @@ def lib() -> C.CDLL:
         L.emmi_build_gradient_dev.argtypes = [PA, PE, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                               C.c_void_p, C.c_void_p, C.c_void_p]
         L.gmg_apply.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.gmg_free.restype = None
+        L.gmg_free.argtypes = []
         L.find_index.restype = C.c_int
         L.find_index.argtypes = [C.c_int, C.c_void_p, C.c_float]
         L.create_nugrid.restype = C.c_float
