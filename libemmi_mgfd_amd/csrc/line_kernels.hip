@@ -48,6 +48,12 @@ struct alignas(16) Ch {
 };
 template <typename R> constexpr int fchunks() { return (FREC + Ch<R>::V - 1) / Ch<R>::V; }
 template <typename R> constexpr int schunks() { return (SREC + Ch<R>::V - 1) / Ch<R>::V; }
+// minimum waves per SIMD the right-hand-side kernels are compiled for
+// (bounds their registers; more waves keep more stencil loads in flight)
+#ifndef EMMI_LINE_RHS_MIN_WAVES
+#define EMMI_LINE_RHS_MIN_WAVES 1
+#endif
+
 template <typename T> using FacRec = typename LineFac<T>::type;
 template <typename T> using ScrRec = typename LineScr<T>::type;
 
@@ -396,7 +402,7 @@ __device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, 
 }
 
 template <typename T, int A>
-__global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
+__global__ void __launch_bounds__(256, EMMI_LINE_RHS_MIN_WAVES) k_line_rhs(LinePack<T> P) {
   const LineJob<T>& J = P.job[blockIdx.y];
   const long nl = (long)J.nlo * J.nhi;
   const int N = J.L.n[A];
@@ -418,7 +424,7 @@ __global__ void __launch_bounds__(256) k_line_rhs(LinePack<T> P) {
 constexpr int XT = 16;
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_line_rhs_x(LinePack<T> P, int nlb) {
+__global__ void __launch_bounds__(256, EMMI_LINE_RHS_MIN_WAVES) k_line_rhs_x(LinePack<T> P, int nlb) {
   constexpr int A = 0, B = 1, C = 2;
   __shared__ double tile[SREC][XT][XT + 1];
   const LineJob<T>& J = P.job[blockIdx.y];
