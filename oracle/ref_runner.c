@@ -11,6 +11,13 @@
  * sequence of src/do_modelling.c:28-104, the gradient task the one of
  * src/fg_fwi.c:139-270 without the MPI sums (one shot per process).
  *
+ * Built with -DEMMI_DROPIN (oracle/Makefile target `dropin`), the same
+ * driver links the reference's host sources (getpar, emf_init, acq_init,
+ * write_data) against libemmi_mgfd.so in place of the reference's hot-path
+ * files: the link-time swap a libEMMI_MGFD host program makes to use the
+ * GPU library.  The unit task needs the reference's own src/gmg.c and is
+ * left out of that build.
+ *
  * Tasks (key=value on the command line, getpar conventions):
  *   task=model  outdir=DIR [dumpfields=1]   mode-0 pipeline
  *   task=units  outdir=DIR [seed=N]         single-kernel known answers
@@ -52,6 +59,7 @@ void build_gradient(acq_t *acq, emf_t *emf, fwi_t *fwi,
                     _Complex float ****E1a, _Complex float ****E2a, _Complex float ****E3a);
 int find_index(int n, float *x, float val);
 
+#ifndef EMMI_DROPIN
 /* Layout of one multigrid level as src/gmg.c:24-34 declares it; needed only
  * by the unit task to fill and read the reference's level arrays. */
 typedef struct {
@@ -77,6 +85,7 @@ void line_gauss_seidel_y(ref_level_t *g, int lev, int iter);
 void line_gauss_seidel_z(ref_level_t *g, int lev, int iter);
 void gauss_seidel(ref_level_t *g, int lev, int iter);
 void compute_H_from_E(ref_level_t *g);
+#endif
 
 static char *outdir;
 
@@ -147,6 +156,7 @@ static double urand(void)
   return (double)(rng_state >> 11) * (1.0/9007199254740992.0) - 0.5;
 }
 
+#ifndef EMMI_DROPIN
 /* fill a level vector with random values on interior edges, zero elsewhere */
 static void fill_interior(ref_level_t *g, double _Complex ****v)
 {
@@ -241,6 +251,8 @@ static void task_units(acq_t *acq, emf_t *emf)
   gmg_free();
   regrid_free(emf);
 }
+
+#endif
 
 static void task_model(acq_t *acq, emf_t *emf)
 {
@@ -432,7 +444,11 @@ int main(int argc, char **argv)
   if (!getparint("verb", &emf->verb)) emf->verb = 1;
   emf_init(emf);
   acq_init(acq, emf);
+#ifdef EMMI_DROPIN
+  if (strcmp(task, "units") == 0) { fprintf(stderr, "task=units needs the reference's src/gmg.c\n"); exit(1); }
+#else
   if (strcmp(task, "units") == 0) task_units(acq, emf);
+#endif
   else if (strcmp(task, "grad") == 0) task_grad(acq, emf);
   else task_model(acq, emf);
   printf("ref_runner done\n");
