@@ -1,4 +1,6 @@
-"""Build the in-tree shared library libemmi_mgfd.so for gfx950 with hipcc.
+"""Build the in-tree shared library libemmi_mgfd.so for gfx950 with hipcc,
+then the command-line program: libemmi_cli.so (parameters, survey and data
+files, data weighting) and the executable emmi_main, linked against both.
 
 Device code is compiled for gfx950 only; the host-side bit-exact path is
 compiled without floating-point contraction.
@@ -19,6 +21,10 @@ LIB = os.path.join(HERE, "libemmi_mgfd.so")
 
 HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "path_kernels.hip", "x0net_kernels.hip"]
 HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp"]
+
+CLI_LIB_SOURCES = ["params.cpp", "survey.cpp", "weighting.cpp"]
+CLI_LIB = os.path.join(HERE, "libemmi_cli.so")
+CLI_EXE = os.path.join(HERE, "emmi_main")
 
 ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
@@ -57,9 +63,35 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, CLI_LIB, CLI_EXE)
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_cli() -> None:
+    """libemmi_cli.so and emmi_main (host code only, g++)."""
+    cli = os.path.join(CSRC, "cli")
+    flags = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(ROOT, "include"),
+             "-I" + cli]
+    objs = []
+    for src in CLI_LIB_SOURCES:
+        obj = os.path.join(BUILD, "cli_" + src + ".o")
+        _run(["g++"] + flags + ["-c", os.path.join(cli, src), "-o", obj])
+        objs.append(obj)
+    _run(["g++", "-shared", "-o", CLI_LIB + ".tmp"] + objs + ["-lm"])
+    os.replace(CLI_LIB + ".tmp", CLI_LIB)
+    main_obj = os.path.join(BUILD, "cli_emmi_main.cpp.o")
+    _run(["g++"] + flags + ["-c", os.path.join(cli, "emmi_main.cpp"), "-o", main_obj])
+    _run(["g++", "-o", CLI_EXE + ".tmp", main_obj, "-L" + HERE, "-lemmi_cli", "-lemmi_mgfd",
+          "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(CLI_EXE + ".tmp", CLI_EXE)
 
 
 if __name__ == "__main__":

@@ -22,6 +22,8 @@
  *   task=model  outdir=DIR [dumpfields=1]   mode-0 pipeline
  *   task=units  outdir=DIR [seed=N]         single-kernel known answers
  *   task=grad   outdir=DIR                  mode-2 gradient pipeline
+ *   task=fwi    (in the run directory)      one rank of a mode-2 run, with
+ *                                            the reference's file outputs
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -54,6 +56,9 @@ void extract_emf_data(acq_t *acq, emf_t *emf, double _Complex *x, double _Comple
 void extract_electric_field(acq_t *acq, emf_t *emf, double _Complex *x, int ifreq);
 void extract_magnetic_field(acq_t *acq, emf_t *emf, double _Complex *b, int ifreq);
 void write_data(acq_t *acq, emf_t *emf, char *fname, _Complex float ***dcal_fd);
+void read_data(acq_t *acq, emf_t *emf);
+void write_rmse(acq_t *acq, emf_t *emf);
+float cal_uncertainty_noise(acq_t *acq, emf_t *emf);
 void build_gradient(acq_t *acq, emf_t *emf, fwi_t *fwi,
                     _Complex float ****E1f, _Complex float ****E2f, _Complex float ****E3f,
                     _Complex float ****E1a, _Complex float ****E2a, _Complex float ****E3a);
@@ -428,6 +433,151 @@ static void task_grad(acq_t *acq, emf_t *emf)
   dump("phess", &fwi.phess[0][0][0][0], sizeof(float)*fwi.n);
 }
 
+/* One gradient evaluation of one rank as src/fg_fwi.c runs it: fg_fwi_init
+ * (:43-137: bathymetry, read_data, cal_uncertainty_noise, obs_<shot>.txt)
+ * and fg_fwi (:139-285 up to the data gradient: the log/exp round trip of
+ * the model from src/do_fwi.c:110-121, forward solves, syn_<shot>.txt, the
+ * misfit, rmse_<shot>.txt, adjoint solves, build_gradient, and the files
+ * gradient_dat_Rv/_Rh), without MPI (one rank: the sums are copies) and
+ * without the regularisation that follows the file writes.  The muted
+ * residuals start at zero. */
+static void task_fwi(acq_t *acq, emf_t *emf)
+{
+  int ifreq, n, ic, ir, j, i, npar;
+  size_t nxyz = (size_t)emf->nx*emf->ny*emf->nz, nd;
+  double _Complex *x, *b;
+  _Complex float ****E1f, ****E2f, ****E3f, ****E1a, ****E2a, ****E3a;
+  fwi_t fwi;
+  char *fbathy, fname[64];
+  FILE *fp;
+  float tmp, fcost = 0, s, s1, s2, *xx;
+
+  memset(&fwi, 0, sizeof fwi);
+  if (!getparstring("fbathy", &fbathy)) err("Need fbathy= for CSEM inversion");
+  if (!getparint("addnoise", &emf->addnoise)) emf->addnoise = 1;
+  if (!getparfloat("amp_perc", &emf->amp_perc)) emf->amp_perc = 0.03;
+  if (!getparfloat("delta_phi", &emf->delta_phi)) emf->delta_phi = 0;
+  if (!getparfloat("delta_theta", &emf->delta_theta)) emf->delta_theta = 0;
+  if (!getparfloat("noisefloorE", &emf->noisefloorE)) emf->noisefloorE = 1e-15;
+  if (!getparfloat("noisefloorH", &emf->noisefloorH)) emf->noisefloorH = 1e-13;
+  if (!getparfloat("dsmute", &emf->dsmute)) emf->dsmute = 800;
+  if (!getparfloat("drmute", &emf->drmute)) emf->drmute = 200;
+  if (!getparint("preco", &fwi.preco)) fwi.preco = 1;
+  if (!getparint("npar", &npar)) npar = 2;
+  fwi.npar = npar;
+  fwi.idxpar = alloc1int(npar);
+  for (j = 0; j < npar; j++) fwi.idxpar[j] = j + 1;
+  getparint("idxpar", fwi.idxpar);
+  fwi.n = npar*nxyz;
+
+  emf->bathy = alloc2float(emf->nx, emf->ny);
+  fp = fopen(fbathy, "rb"); if (!fp) err("bathy");
+  if (fread(&emf->bathy[0][0], sizeof(float), emf->nx*emf->ny, fp) != (size_t)(emf->nx*emf->ny)) err("bathy size");
+  fclose(fp);
+  emf->depthmax = emf->bathy[0][0];
+  for (i = 0; i < emf->nx*emf->ny; i++) emf->depthmax = MAX(emf->bathy[0][i], emf->depthmax);
+  if (emf->depthmax < acq->src_x3[0]) err("coordinate error: source placed below seabed!");
+
+  nd = (size_t)emf->nchrec*emf->nfreq*acq->nrec;
+  emf->dobs_fd = alloc3complexf(acq->nrec, emf->nfreq, emf->nchrec);
+  emf->dcal_fd = alloc3complexf(acq->nrec, emf->nfreq, emf->nchrec);
+  emf->dres_fd = alloc3complexf(acq->nrec, emf->nfreq, emf->nchrec);
+  emf->delta_emf = alloc3float(acq->nrec, emf->nfreq, emf->nchrec);
+  emf->rmse = alloc3float(acq->nrec, emf->nfreq, emf->nchrec);
+  memset(&emf->dobs_fd[0][0][0], 0, nd*sizeof(_Complex float));
+  memset(&emf->dres_fd[0][0][0], 0, nd*sizeof(_Complex float));
+  read_data(acq, emf);
+  tmp = cal_uncertainty_noise(acq, emf);
+  printf("addnoise=%d, target rmse=%e\n", emf->addnoise, sqrt(tmp/nd));
+  sprintf(fname, "obs_%04d.txt", acq->shot_idx[iproc]);
+  write_data(acq, emf, fname, emf->dobs_fd);
+
+  fwi.grad = alloc4float(emf->nx, emf->ny, emf->nz, npar);
+  fwi.phess = alloc4float(emf->nx, emf->ny, emf->nz, npar);
+  xx = alloc1float(fwi.n);
+  for (j = 0; j < npar; j++)
+    for (i = 0; i < (int)nxyz; i++) {
+      if (fwi.idxpar[j] == 1) xx[i + nxyz*j] = log(emf->rho11[0][0][i]);
+      if (fwi.idxpar[j] == 2) xx[i + nxyz*j] = log(emf->rho33[0][0][i]);
+    }
+  for (j = 0; j < npar; j++)
+    for (i = 0; i < (int)nxyz; i++) {
+      s = exp(xx[i + nxyz*j]);
+      if (fwi.idxpar[j] == 1) { emf->rho11[0][0][i] = s; emf->rho22[0][0][i] = s; }
+      if (fwi.idxpar[j] == 2) emf->rho33[0][0][i] = s;
+    }
+
+  E1f = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  E2f = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  E3f = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  E1a = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  E2a = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  E3a = alloc4complexf(emf->nx, emf->ny, emf->nz, emf->nfreq);
+  n = 3*(emf->n1+1)*(emf->n2+1)*(emf->n3+1);
+  emf->E1 = E1f; emf->E2 = E2f; emf->E3 = E3f;
+  regrid_init(acq, emf);
+  for (ifreq = 0; ifreq < emf->nfreq; ifreq++) {
+    gmg_init(emf, ifreq);
+    x = alloc1complex(n); b = alloc1complex(n);
+    inject_source(acq, emf, b, ifreq);
+    memset(x, 0, n*sizeof(double _Complex));
+    gmg_apply(n, b, x);
+    extract_emf_data(acq, emf, x, b, ifreq);
+    extract_electric_field(acq, emf, x, ifreq);
+    free1complex(x); free1complex(b);
+    gmg_free();
+  }
+  regrid_free(emf);
+  sprintf(fname, "syn_%04d.txt", acq->shot_idx[iproc]);
+  write_data(acq, emf, fname, emf->dcal_fd);
+
+  memset(&emf->rmse[0][0][0], 0, nd*sizeof(float));
+  for (ic = 0; ic < emf->nchrec; ic++)
+    for (ifreq = 0; ifreq < emf->nfreq; ifreq++) {
+      s = 5.5*sqrt(2.*emf->rho_water/(emf->omegas[ifreq]*mu0));
+      for (ir = 0; ir < acq->nrec; ir++) {
+        s1 = acq->rec_x1[ir] - acq->src_x1[0];
+        s2 = acq->rec_x2[ir] - acq->src_x2[0];
+        if (sqrt(s1*s1 + s2*s2) > s) {
+          emf->dres_fd[ic][ifreq][ir] = emf->dobs_fd[ic][ifreq][ir] - emf->dcal_fd[ic][ifreq][ir];
+          emf->dres_fd[ic][ifreq][ir] /= emf->delta_emf[ic][ifreq][ir];
+          emf->rmse[ic][ifreq][ir] = cabs(emf->dres_fd[ic][ifreq][ir]);
+          fcost += emf->dres_fd[ic][ifreq][ir]*conj(emf->dres_fd[ic][ifreq][ir]);
+          emf->dres_fd[ic][ifreq][ir] /= emf->delta_emf[ic][ifreq][ir];
+          emf->dres_fd[ic][ifreq][ir] = conj(emf->dres_fd[ic][ifreq][ir]);
+        }
+      }
+    }
+  fcost *= 0.5;
+  write_rmse(acq, emf);
+  printf("isrc=%d, fcost=%g, rmse=%g\n", acq->shot_idx[iproc], fcost, sqrt(2.*fcost/nd));
+  dump("fcost", &fcost, sizeof fcost);
+
+  emf->E1 = E1a; emf->E2 = E2a; emf->E3 = E3a;
+  regrid_init(acq, emf);
+  for (ifreq = 0; ifreq < emf->nfreq; ifreq++) {
+    gmg_init(emf, ifreq);
+    x = alloc1complex(n); b = alloc1complex(n);
+    inject_adj_source(acq, emf, x, b, ifreq);
+    memset(x, 0, n*sizeof(double _Complex));
+    gmg_apply(n, b, x);
+    extract_emf_data(acq, emf, x, b, ifreq);
+    extract_electric_field(acq, emf, x, ifreq);
+    free1complex(x); free1complex(b);
+    gmg_free();
+  }
+  regrid_free(emf);
+  build_gradient(acq, emf, &fwi, E1f, E2f, E3f, E1a, E2a, E3a);
+  fp = fopen("gradient_dat_Rv", "wb");
+  fwrite(&fwi.grad[0][0][0][0], nxyz*sizeof(float), 1, fp);
+  fclose(fp);
+  if (npar > 1) {
+    fp = fopen("gradient_dat_Rh", "wb");
+    fwrite(&fwi.grad[0][0][0][0] + nxyz, nxyz*sizeof(float), 1, fp);
+    fclose(fp);
+  }
+}
+
 int main(int argc, char **argv)
 {
   emf_t *emf;
@@ -450,6 +600,7 @@ int main(int argc, char **argv)
   if (strcmp(task, "units") == 0) task_units(acq, emf);
 #endif
   else if (strcmp(task, "grad") == 0) task_grad(acq, emf);
+  else if (strcmp(task, "fwi") == 0) task_fwi(acq, emf);
   else task_model(acq, emf);
   printf("ref_runner done\n");
   return 0;
