@@ -10,11 +10,12 @@ pipeline on the same input files.
     reference ranks' gradients (oracle/_ref/ref_runner task=fwi, one run per
     shot, as the reference's MPI ranks would each run).
 
-Both sides solve to a relative residual of 1e-10, so their converged fields
-agree to far better than the 1e-5 bound used; emmi_main runs its default
-FP32 cycles inside FP64 refinement.  The runners are built from the
-reference tree by __graft_entry__.build(); without them the test is
-skipped.
+Both sides solve to a relative residual of 1e-10 (emmi_main with FP64
+cycles), so their converged fields agree to far better than the 1e-5 bound
+used.  The modelling run is repeated with emmi_main's defaults (FP32 cycles
+inside FP64 refinement, the reference's tol=1e-6).  The runners are built
+from the reference tree by __graft_entry__.build(); without them the test
+is skipped.
 """
 import os
 import shutil
@@ -33,7 +34,15 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(EXE)),
                                  reason="emmi_main or oracle/_ref/ref_runner not built")]
 
-TIGHT = ["tol=1e-10", "ncycle=200"]
+# FP64 cycles for the tight comparisons: FP32 cycles inside FP64 refinement
+# stall above 1e-10 on these air-padded grids (their default use is the
+# reference's tol=1e-6, checked separately below)
+TIGHT = ["tol=1e-10", "ncycle=200", "precision=0"]
+
+
+def converged(out):
+    bad = [ln for ln in out.splitlines() if "not converged" in ln]
+    assert not bad, bad[:6]
 
 
 def run(cmd, cwd, timeout=240):
@@ -68,14 +77,28 @@ def test_modelling_three_shots(tmp_path):
     args = cli_args(files) + TIGHT
     gpu = tmp_path / "gpu"
     gpu.mkdir()
-    out = run([EXE, "mode=0", "nproc=3"] + args, str(gpu))
-    assert "not converged" not in out
+    converged(run([EXE, "mode=0", "nproc=3"] + args, str(gpu)))
     for shot in (1, 2, 3):
         ref = tmp_path / f"ref{shot}"
         run([REF, "task=model", f"outdir={ref}", f"shots={shot}", "verb=0"] + args, str(tmp_path))
         name = f"emf_{shot:04d}.txt"
         worst = numbers_close(str(ref / name), str(gpu / name))
         print(f"{name}: max rel {worst:.2e}", flush=True)
+    # the default run: FP32 cycles in FP64 refinement at the reference's
+    # default tolerance, against the tight FP64 fields
+    mixed = tmp_path / "mixed"
+    mixed.mkdir()
+    out = run([EXE, "mode=0", "nproc=3"] + cli_args(files) + ["verb=1"], str(mixed))
+    converged(out)
+    for shot in (1, 2, 3):
+        name = f"emf_{shot:04d}.txt"
+        a = np.loadtxt(gpu / name, skiprows=1, usecols=(4, 5))
+        b = np.loadtxt(mixed / name, skiprows=1, usecols=(4, 5))
+        za, zb = a[:, 0] + 1j * a[:, 1], b[:, 0] + 1j * b[:, 1]
+        strong = np.abs(za) > 1e-3 * np.abs(za).max()
+        err = float(np.max(np.abs(zb - za)[strong] / np.abs(za)[strong]))
+        print(f"{name} default precision, tol 1e-6: max rel {err:.2e}", flush=True)
+        assert err < 1e-2, name
 
 
 def test_gradient_two_shots(tmp_path):
@@ -86,8 +109,7 @@ def test_gradient_two_shots(tmp_path):
     gpu.mkdir()
     for shot in (1, 2):
         shutil.copy(tmp_path / "in" / f"emf_{shot:04d}.txt", gpu)
-    out = run([EXE, "mode=2", "nproc=2"] + args, str(gpu))
-    assert "not converged" not in out
+    converged(run([EXE, "mode=2", "nproc=2"] + args, str(gpu)))
     nxyz = 24 * 20 * 10
     g_ref = {"gradient_dat_Rv": np.zeros(nxyz, np.float32), "gradient_dat_Rh": np.zeros(nxyz, np.float32)}
     for shot in (1, 2):
