@@ -77,6 +77,28 @@ float cal_uncertainty_noise(acq_t *acq, emf_t *emf);
  * weighted moduli into emf->rmse; returns 0.5 * sum |weighted residual|^2 */
 float emmi_data_misfit(acq_t *acq, emf_t *emf);
 
+/* ------------------------------------------------------------------------- */
+/* model regularisation and gradient preconditioning of the inversion        */
+/* ------------------------------------------------------------------------- */
+/* src/regularization.c:11-57: roughness c_h(|Dx x|^2 + |Dy x|^2) +
+ * c_v |Dz x|^2 of an n1 x n2 x n3 volume (unit spacings, c_h = 1,
+ * c_v = 0.03) and its gradient into g; d1..d3 are unused as there */
+float regularization_tikhonov(float *x, float *g, int n1, int n2, int n3, float d1, float d2, float d3);
+/* src/regularization.c:60-130: total variation sum sqrt(|grad x|^2 + 1e-14)
+ * and its gradient into g */
+float regularization_tv(float *x, float *g, int n1, int n2, int n3, float d1, float d2, float d3);
+/* src/triangle_smoothing.c:63-107: triangle filters of half-widths r1, r2,
+ * r3 along the three axes of mod[n3][n2][n1], repeated */
+void triangle_smoothing(float ***mod, int n1, int n2, int n3, int r1, int r2, int r3, int repeat);
+/* src/precondition.c:16-87: fwi->preco 1 divides g by the pseudo-Hessian
+ * fwi->phess, 2 scales it by exp(depth below seabed / skin depth), 3 smooths
+ * it (using fwi->grad as scratch) */
+void precondition(emf_t *emf, fwi_t *fwi, float *g);
+/* src/fg_mod_reg.c:19-94: adds the Tikhonov (fwi->gamma1) and TV
+ * (fwi->gamma2) terms of x - fwi->xref below the seabed, cooled by
+ * 0.8^fwi->iter, to g and to fwi->fcost_mod */
+void fg_mod_reg(acq_t *acq, emf_t *emf, fwi_t *fwi, float *x, float *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libemmi_mgfd.so")
 HIP_SOURCES = ["mg_kernels.hip", "line_kernels.hip", "solver.hip", "path_kernels.hip", "x0net_kernels.hip"]
 HOST_SOURCES = ["capi.cpp", "host_path.cpp", "timing.cpp", "path_api.cpp"]
 
-CLI_LIB_SOURCES = ["params.cpp", "survey.cpp", "weighting.cpp"]
+CLI_LIB_SOURCES = ["params.cpp", "survey.cpp", "weighting.cpp", "inversion.cpp"]
 CLI_LIB = os.path.join(HERE, "libemmi_cli.so")
 CLI_EXE = os.path.join(HERE, "emmi_main")
 

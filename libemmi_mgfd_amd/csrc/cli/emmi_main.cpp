@@ -4,11 +4,16 @@
 //
 //   mode=0  modelling (src/do_modelling.c:28-104): emf_<shot>.txt, and with
 //           icheck=1 the fields Ex.bin .. Hz.bin
-//   mode=2  one gradient of the data misfit (src/do_fwi.c:43-150 and
-//           src/fg_fwi.c:43-137, 139-300 up to the data gradient):
-//           obs_/syn_/rmse_<shot>.txt and gradient_dat_Rv, gradient_dat_Rh
-//   mode=1  (the inversion loop) is not provided: the program says so and
-//           exits with status 1.
+//   mode=1  inversion (src/do_fwi.c:43-240): l-BFGS with the bisection
+//           line search, regularisation and preconditioning around the
+//           misfit and gradient evaluations of src/fg_fwi.c:139-305;
+//           iterate.txt, rmse_misfit.txt, Rv/Rh_iteration_NN and the files
+//           of each evaluation
+//   mode=2  one misfit and gradient evaluation (src/do_fwi.c:43-150,
+//           src/fg_fwi.c:43-305): obs_/syn_/rmse_<shot>.txt,
+//           gradient_dat_Rv/_Rh and the regularisation gradients
+//   method=1 (Gauss-Newton) is refused: its Hessian-vector product is empty
+//           in the reference (src/fg_fwi.c:307-311).
 //
 // The reference runs one MPI rank per shot.  This program runs the ranks of
 // such a job one after another in one process on one GPU: shots=i,j,k (as
@@ -29,6 +34,7 @@
 
 #include "arrays.h"
 #include "emmi_cli.h"
+#include "inversion.h"
 
 using namespace emmi_cli;
 
@@ -172,7 +178,8 @@ void do_modelling(acq_t* acq, emf_t* emf, const emmi_params_t& p) {
 }
 
 // ---------------------------------------------------------------------------
-// mode 2: the data part of one misfit/gradient evaluation
+// modes 1 and 2: misfit and gradient evaluations (src/fg_fwi.c) inside the
+// inversion loop (src/do_fwi.c)
 // ---------------------------------------------------------------------------
 struct Rank {
   acq_t acq{};
@@ -180,189 +187,390 @@ struct Rank {
   float ***delta = nullptr, ***rmse = nullptr;
 };
 
-// point the carrier at one rank's data arrays
-void bind(emf_t* emf, Rank& r) {
-  emf->dobs_fd = r.dobs;
-  emf->dcal_fd = r.dcal;
-  emf->dres_fd = r.dres;
-  emf->delta_emf = r.delta;
-  emf->rmse = r.rmse;
-}
-
-void do_gradient(emf_t* emf, const emmi_params_t& p, int verb_set, int verb_par) {
-  // src/fg_fwi.c:55-76
-  char* fbathy;
-  if (!getparstring(const_cast<char*>("fbathy"), &fbathy)) emmi_cli_err("Need fbathy= for CSEM inversion");
-  emf->addnoise = par_int("addnoise", 1);
-  emf->amp_perc = par_float("amp_perc", 0.03f);
-  emf->delta_phi = par_float("delta_phi", 0);
-  emf->delta_theta = par_float("delta_theta", 0);
-  emf->noisefloorE = par_float("noisefloorE", 1e-15f);
-  emf->noisefloorH = par_float("noisefloorH", 1e-13f);
-  emf->dsmute = par_float("dsmute", 800);
-  emf->drmute = par_float("drmute", 200);
-  // src/do_fwi.c:65-72: the parameters of the gradient
-  fwi_t fwi{};
-  fwi.preco = par_int("preco", 1);
-  fwi.npar = par_int("npar", 2);
-  if (fwi.npar < 1 || fwi.npar > 8) emmi_cli_err("npar=%d outside 1..8", fwi.npar);
-  std::vector<int> idxpar(fwi.npar);
-  // the reference leaves idxpar unset when idxpar= is absent; here it
-  // defaults to 1..npar (rho_h, then rho_v)
-  for (int j = 0; j < fwi.npar; ++j) idxpar[j] = j + 1;
-  if (countparval(const_cast<char*>("idxpar")) > fwi.npar) emmi_cli_err("idxpar= has more than npar=%d values", fwi.npar);
-  getparint(const_cast<char*>("idxpar"), idxpar.data());
-  fwi.idxpar = idxpar.data();
-  const size_t nx = emf->nx, ny = emf->ny, nz = emf->nz, nxyz = nx * ny * nz, nf = emf->nfreq;
-  fwi.n = static_cast<int>(nxyz * fwi.npar);
-
-  // bathymetry (src/fg_fwi.c:86-99)
-  emf->bathy = alloc2<float>(nx, ny);
-  {
+class Inversion {
+ public:
+  // src/fg_fwi.c:43-137 (fg_fwi_init) for every rank, with the parameters
+  // src/do_fwi.c:55-100 reads
+  Inversion(emf_t* emf, const emmi_params_t& p, int verb_set, int verb_par)
+      : emf_(emf), p_(p), verb_set_(verb_set), verb_par_(verb_par) {
+    char* fbathy;
+    if (!getparstring(const_cast<char*>("fbathy"), &fbathy)) emmi_cli_err("Need fbathy= for CSEM inversion");
+    emf->addnoise = par_int("addnoise", 1);
+    emf->amp_perc = par_float("amp_perc", 0.03f);
+    emf->delta_phi = par_float("delta_phi", 0);
+    emf->delta_theta = par_float("delta_theta", 0);
+    emf->noisefloorE = par_float("noisefloorE", 1e-15f);
+    emf->noisefloorH = par_float("noisefloorH", 1e-13f);
+    emf->dsmute = par_float("dsmute", 800);
+    emf->drmute = par_float("drmute", 200);
+    fwi_.gamma1 = par_float("gamma1", 10.);
+    fwi_.gamma2 = par_float("gamma2", 0.);
+    fwi_.r1 = par_int("r1", 3);
+    fwi_.r2 = par_int("r2", 3);
+    fwi_.r3 = par_int("r3", 1);
+    fwi_.repeat = par_int("repeat", 1);
+    fwi_.preco = par_int("preco", 1);
+    fwi_.npar = par_int("npar", 2);
+    if (fwi_.npar < 1 || fwi_.npar > 8) emmi_cli_err("npar=%d outside 1..8", fwi_.npar);
+    // the reference leaves idxpar unset when idxpar= is absent; here it
+    // defaults to 1..npar (rho_h, then rho_v)
+    idxpar_.resize(fwi_.npar);
+    for (int j = 0; j < fwi_.npar; ++j) idxpar_[j] = j + 1;
+    if (countparval(const_cast<char*>("idxpar")) > fwi_.npar) emmi_cli_err("idxpar= has more than npar=%d values", fwi_.npar);
+    getparint(const_cast<char*>("idxpar"), idxpar_.data());
+    fwi_.idxpar = idxpar_.data();
+    nxyz_ = static_cast<size_t>(emf->nx) * emf->ny * emf->nz;
+    fwi_.n = static_cast<int>(nxyz_ * fwi_.npar);
+    if (emf->verb) {
+      printf("preco=%d (0=off; 1=Shin's preconditioner; 2=depth weighting; 3=triangle smoothing)\n", fwi_.preco);
+      printf("Amplitude uncertainty: %g%%\n", emf->amp_perc * 100.);
+      printf("noisefloorE=%g, noisefloorH=%g\n", emf->noisefloorE, emf->noisefloorH);
+      printf("Tikhonov regularization: gamma1=%g\n", fwi_.gamma1);
+      printf("TV regularization:       gamma2=%g\n", fwi_.gamma2);
+      printf("----------- fwi init ------------\n");
+    }
+    // bathymetry (src/fg_fwi.c:86-99)
+    const size_t nx = emf->nx, ny = emf->ny, nz = emf->nz, nf = emf->nfreq;
+    emf->bathy = alloc2<float>(nx, ny);
     FILE* fp = fopen(fbathy, "rb");
     if (!fp) emmi_cli_err("cannot open file fbathy=%s\n", fbathy);
     const size_t got = fread(&emf->bathy[0][0], sizeof(float), nx * ny, fp);
     fclose(fp);
     if (got != nx * ny) emmi_cli_err("size parameter does not match the file-bathy!");
+    emf->depthmax = emf->bathy[0][0];
+    for (size_t i = 0; i < nx * ny; ++i) emf->depthmax = std::max(emf->bathy[0][i], emf->depthmax);
+    if (emf->verb) printf("maximum water depth: depthmax=%g\n", emf->depthmax);
+    // every rank: acquisition, observed data with noise, obs_<shot>.txt;
+    // the noise generator starts fresh per rank as in a new process
+    ranks_.resize(nproc);
+    float mse = 0;
+    for (iproc = 0; iproc < nproc; ++iproc) {
+      set_verb();
+      Rank& r = ranks_[iproc];
+      acq_init(&r.acq, emf);
+      if (emf->depthmax < r.acq.src_x3[0]) emmi_cli_err("coordinate error: source placed below seabed!");
+      r.dobs = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
+      r.dcal = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
+      r.dres = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
+      r.delta = alloc3<float>(r.acq.nrec, nf, emf->nchrec);
+      r.rmse = alloc3<float>(r.acq.nrec, nf, emf->nchrec);
+      bind(r);
+      read_data(&r.acq, emf);
+      emmi_rannor_reset();
+      mse = mse + cal_uncertainty_noise(&r.acq, emf);
+      char fname[32];
+      snprintf(fname, sizeof fname, "obs_%04d.txt", r.acq.shot_idx[iproc]);
+      write_data(&r.acq, emf, fname, emf->dobs_fd);
+    }
+    iproc = 0;
+    set_verb();
+    fwi_.mse = mse;
+    if (emf->verb) {
+      const int n = nproc * emf->nchrec * emf->nfreq * ranks_[0].acq.nrec;
+      printf("addnoise=%d, target rmse=%e\n", emf->addnoise, std::sqrt(mse / n));
+    }
+    xref_.resize(fwi_.n);
+    fwi_.xref = xref_.data();
+    fwi_.grad = alloc4<float>(nx, ny, nz, fwi_.npar);
+    if (fwi_.preco == 1) fwi_.phess = alloc4<float>(nx, ny, nz, fwi_.npar);
+    fwi_.iter = 0;
+    fwi_.firstgrad = 1;
+    fwi_.alpha = 1;
+    // device buffers of the gradient evaluations
+    const size_t ebytes = sizeof(emmi_c64) * 3 * nxyz_;
+    ef_.resize(nf);
+    ea_.resize(nf);
+    for (size_t f = 0; f < nf; ++f) {
+      ef_[f] = emmi_device_alloc(ebytes);
+      ea_[f] = emmi_device_alloc(ebytes);
+      if (!ef_[f] || !ea_[f]) emmi_cli_err("device allocation of the model-grid fields failed");
+    }
+    grad_d_ = static_cast<float*>(emmi_device_alloc(sizeof(float) * fwi_.n));
+    phess_d_ = fwi_.preco == 1 ? static_cast<float*>(emmi_device_alloc(sizeof(float) * fwi_.n)) : nullptr;
+    if (!grad_d_ || (fwi_.preco == 1 && !phess_d_)) emmi_cli_err("device allocation of the gradient failed");
   }
-  emf->depthmax = emf->bathy[0][0];
-  for (size_t i = 0; i < nx * ny; ++i) emf->depthmax = std::max(emf->bathy[0][i], emf->depthmax);
-  if (verb_par) printf("maximum water depth: depthmax=%g\n", emf->depthmax);
 
-  // every rank: its acquisition, observed data with noise, obs_<shot>.txt
-  // (src/fg_fwi.c:100-116)
-  std::vector<Rank> ranks(nproc);
-  float mse = 0;
-  for (iproc = 0; iproc < nproc; ++iproc) {
-    emf->verb = verb_set ? verb_par : (iproc == 0);
-    Rank& r = ranks[iproc];
-    acq_init(&r.acq, emf);
-    if (emf->depthmax < r.acq.src_x3[0]) emmi_cli_err("coordinate error: source placed below seabed!");
-    r.dobs = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
-    r.dcal = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
-    r.dres = alloc3<emmi_c64>(r.acq.nrec, nf, emf->nchrec);
-    r.delta = alloc3<float>(r.acq.nrec, nf, emf->nchrec);
-    r.rmse = alloc3<float>(r.acq.nrec, nf, emf->nchrec);
-    bind(emf, r);
-    read_data(&r.acq, emf);
-    emmi_rannor_reset();
-    mse = mse + cal_uncertainty_noise(&r.acq, emf);
-    char fname[32];
-    snprintf(fname, sizeof fname, "obs_%04d.txt", r.acq.shot_idx[iproc]);
-    write_data(&r.acq, emf, fname, emf->dobs_fd);
-  }
-  iproc = 0;
-  emf->verb = verb_set ? verb_par : 1;
-  if (emf->verb) {
-    const int n = nproc * emf->nchrec * emf->nfreq * ranks[0].acq.nrec;
-    printf("addnoise=%d, target rmse=%e\n", emf->addnoise, std::sqrt(mse / n));
+  ~Inversion() {
+    for (void* q : ef_) emmi_device_free(q);
+    for (void* q : ea_) emmi_device_free(q);
+    emmi_device_free(grad_d_);
+    if (phess_d_) emmi_device_free(phess_d_);
+    for (Rank& r : ranks_) {
+      free3(r.dobs);
+      free3(r.dcal);
+      free3(r.dres);
+      free3(r.delta);
+      free3(r.rmse);
+      acq_free(&r.acq);
+    }
+    emf_->dobs_fd = emf_->dcal_fd = emf_->dres_fd = nullptr;
+    emf_->delta_emf = emf_->rmse = nullptr;
+    free4(fwi_.grad);
+    free4(fwi_.phess);
+    free2(emf_->bathy);
+    emf_->bathy = nullptr;
   }
 
-  // the model as the optimiser hands it over: log of the resistivities, and
-  // back (src/do_fwi.c:110-121, src/fg_fwi.c:151-170)
-  {
-    std::vector<float> xx(nxyz * fwi.npar);
-    for (int j = 0; j < fwi.npar; ++j)
-      for (size_t i = 0; i < nxyz; ++i) {
-        if (idxpar[j] == 1) xx[i + nxyz * j] = static_cast<float>(std::log(static_cast<double>(emf->rho11[0][0][i])));
-        if (idxpar[j] == 2) xx[i + nxyz * j] = static_cast<float>(std::log(static_cast<double>(emf->rho33[0][0][i])));
+  // src/do_fwi.c:43-240 for mode 1 (l-BFGS) and mode 2 (one gradient)
+  void run() {
+    emf_t* emf = emf_;
+    LineSearch ls;
+    const int niter = par_int("niter", 30);
+    ls.nls = par_int("nls", 5);
+    float tol = 1e-6f;
+    getparfloat(const_cast<char*>("tol"), &tol);
+    const int npair = par_int("npair", 5);
+    ls.c1 = par_float("c1", 1e-4f);
+    ls.c2 = par_float("c2", 0.9f);
+    const float alpha0 = par_float("alpha", 1.);
+    const int bound = par_int("bound", 0);
+    const int method = par_int("method", 0);
+    ls.verb = emf->verb;
+    const int n = fwi_.n;
+    std::vector<float> minpar, maxpar;
+    if (bound) {
+      const int j1 = countparval(const_cast<char*>("minpar")), j2 = countparval(const_cast<char*>("maxpar"));
+      if (!j1) emmi_cli_err("Need lower bound of parameters minpar= vector");
+      if (j1 != fwi_.npar) emmi_cli_err("npar=%d, must have length[idxpar]=npar", fwi_.npar);
+      if (!j2) emmi_cli_err("Need upper bound of parameters maxpar= vector");
+      if (j2 != fwi_.npar) emmi_cli_err("npar=%d, must have length[idxpar]=npar", fwi_.npar);
+      minpar.resize(fwi_.npar);
+      maxpar.resize(fwi_.npar);
+      getparfloat(const_cast<char*>("minpar"), minpar.data());
+      getparfloat(const_cast<char*>("maxpar"), maxpar.data());
+    }
+    if (emf->verb) {
+      printf("---------------------------------------\n");
+      printf("npar=%d\n", fwi_.npar);
+      printf("**** method=%d (0=l-BFGS; 1=Gauss-Newton)\n", method);
+      printf("**** niter=%d\n**** nls=%d\n**** bound=%d\n**** preco=%d\n", niter, ls.nls, bound, fwi_.preco);
+    }
+    fwi_.niter = niter;
+    // the unknowns: log resistivities (src/do_fwi.c:110-121)
+    std::vector<float> x(n), g(n), d(n), xmin, xmax;
+    for (int j = 0; j < fwi_.npar; ++j)
+      for (size_t i = 0; i < nxyz_; ++i) {
+        if (idxpar_[j] == 1) x[i + nxyz_ * j] = static_cast<float>(std::log(static_cast<double>(emf->rho11[0][0][i])));
+        if (idxpar_[j] == 2) x[i + nxyz_ * j] = static_cast<float>(std::log(static_cast<double>(emf->rho33[0][0][i])));
       }
+    xref_ = x;
+    fwi_.xref = xref_.data();
+    if (bound) {
+      // bounds below the seabed; the water and air keep their values
+      xmin.resize(n);
+      xmax.resize(n);
+      for (int j = 0; j < fwi_.npar; ++j)
+        for (int i3 = 0; i3 < emf->nz; ++i3)
+          for (int i2 = 0; i2 < emf->ny; ++i2)
+            for (int i1 = 0; i1 < emf->nx; ++i1) {
+              const size_t i = i1 + emf->nx * (i2 + emf->ny * (i3 + static_cast<size_t>(emf->nz) * j));
+              if (emf->oz + (i3 + 0.5) * emf->dz > emf->bathy[i2][i1] + emf->dz) {
+                xmin[i] = static_cast<float>(std::log(static_cast<double>(minpar[j])));
+                xmax[i] = static_cast<float>(std::log(static_cast<double>(maxpar[j])));
+              } else {
+                xmin[i] = xmax[i] = x[i];
+              }
+            }
+    }
+    const auto fg = [this](float* xx, float* gg) { return evaluate(xx, gg); };
+    float fcost = evaluate(x.data(), g.data());
+    if (emf->mode != 1) return;
+
+    // l-BFGS with the bisection line search (src/do_fwi.c:151-225)
+    (void)tol;  // the reference reads tol= for the loop but stops only at niter
+    (void)alpha0;
+    Lbfgs lb(n, npair);
+    const float f0 = fcost;
+    ls.fk = fcost;
+    ls.igrad = 0;
+    ls.ils = 0;
+    ls.alpha = alpha0;
+    if (emf->verb) {
+      FILE* fp = fopen("iterate.txt", "w");
+      if (!fp) emmi_cli_err("cannot open iterate.txt");
+      fprintf(fp, "==========================================================\n");
+      fprintf(fp, "l-BFGS memory length: %d\n", npair);
+      fprintf(fp, "Maximum number of iterations: %d\n", niter);
+      fprintf(fp, "Convergence tolerance: %3.2e\n", tol);
+      fprintf(fp, "maximum number of line search: %d\n", ls.nls);
+      fprintf(fp, "initial step length: alpha=%g\n", alpha0);
+      fprintf(fp, "==========================================================\n");
+      fprintf(fp, "iter    fk       fk/f0      ||gk||    alpha     nls   ngrad\n");
+      fclose(fp);
+    }
+    int iter = 0;
+    for (; iter < niter; ++iter) {
+      fwi_.iter = iter;
+      if (emf->verb) {
+        printf("==========================================================\n");
+        printf("# iter=%d  fk/f0=%g\n", iter, ls.fk / f0);
+        const float gk = l2norm(n, g.data());
+        FILE* fp = fopen("iterate.txt", "a");
+        fprintf(fp, "%3d   %3.2e  %3.2e   %3.2e  %3.2e  %3d  %4d\n", iter, ls.fk, ls.fk / f0, gk, ls.alpha, ls.ils,
+                ls.igrad);
+        fclose(fp);
+        fp = fopen("rmse_misfit.txt", iter == 0 ? "w" : "a");
+        fprintf(fp, "%d \t %g\n", fwi_.iter, fwi_.mse);
+        fclose(fp);
+      }
+      std::vector<float>& q = lb.q;
+      if (iter == 0) {
+        memcpy(q.data(), g.data(), sizeof(float) * n);
+        if (fwi_.preco) precondition(emf, &fwi_, q.data());
+        for (int j = 0; j < n; ++j) d[j] = -q[j];
+      } else {
+        lb.update(x.data(), g.data());
+        if (fwi_.preco) {
+          lb.first_loop(g.data());
+          precondition(emf, &fwi_, q.data());
+          // without a usable pair the reference keeps the previous direction
+          if (lb.loop1) lb.second_loop(d.data());
+        } else {
+          lb.direction(g.data(), d.data());
+        }
+      }
+      lb.save(x.data(), g.data());
+      line_search(ls, n, x.data(), g.data(), d.data(), fg, bound ? xmin.data() : nullptr,
+                  bound ? xmax.data() : nullptr);
+      if (emf->verb) {
+        char name[32];
+        snprintf(name, sizeof name, "Rv_iteration_%02d", iter);
+        write_bin(name, &emf->rho33[0][0][0], sizeof(float) * nxyz_);
+        snprintf(name, sizeof name, "Rh_iteration_%02d", iter);
+        write_bin(name, &emf->rho11[0][0][0], sizeof(float) * nxyz_);
+      }
+    }
+    if (emf->verb && iter == niter) {
+      FILE* fp = fopen("iterate.txt", "a");
+      fprintf(fp, "==>Maximum iteration number reached!\n");
+      fclose(fp);
+    }
+  }
+
+ private:
+  emf_t* emf_;
+  emmi_params_t p_;
+  int verb_set_, verb_par_;
+  fwi_t fwi_{};
+  std::vector<int> idxpar_;
+  std::vector<float> xref_;
+  size_t nxyz_ = 0;
+  std::vector<Rank> ranks_;
+  std::vector<void*> ef_, ea_;
+  float *grad_d_ = nullptr, *phess_d_ = nullptr;
+
+  void set_verb() { emf_->verb = verb_set_ ? verb_par_ : (iproc == 0); }
+
+  void bind(Rank& r) {
+    emf_->dobs_fd = r.dobs;
+    emf_->dcal_fd = r.dcal;
+    emf_->dres_fd = r.dres;
+    emf_->delta_emf = r.delta;
+    emf_->rmse = r.rmse;
+  }
+
+  // one misfit and gradient evaluation at the log resistivities xx, all
+  // ranks (src/fg_fwi.c:139-300); returns the scaled misfit, g its gradient
+  float evaluate(float* xx, float* g) {
+    emf_t* emf = emf_;
+    const size_t nf = emf->nfreq;
+    const int npar = fwi_.npar;
     emf->rhomin = emf->rhomax = static_cast<float>(std::exp(static_cast<double>(xx[0])));
-    for (int j = 0; j < fwi.npar; ++j)
-      for (size_t i = 0; i < nxyz; ++i) {
-        const float s = static_cast<float>(std::exp(static_cast<double>(xx[i + nxyz * j])));
-        if (idxpar[j] == 1) emf->rho11[0][0][i] = emf->rho22[0][0][i] = s;
-        if (idxpar[j] == 2) emf->rho33[0][0][i] = s;
+    for (int j = 0; j < npar; ++j)
+      for (size_t i = 0; i < nxyz_; ++i) {
+        const float s = static_cast<float>(std::exp(static_cast<double>(xx[i + nxyz_ * j])));
+        if (idxpar_[j] == 1) emf->rho11[0][0][i] = emf->rho22[0][0][i] = s;
+        if (idxpar_[j] == 2) emf->rho33[0][0][i] = s;
         emf->rhomin = std::min(emf->rhomin, s);
         emf->rhomax = std::max(emf->rhomax, s);
       }
-    if (emf->verb) printf("fg_fwi iter=0: [rhomin,rhomax]=[%g,%g]\n", emf->rhomin, emf->rhomax);
-  }
+    if (emf->verb) printf("fg_fwi iter=%d: [rhomin,rhomax]=[%g,%g]\n", fwi_.iter, emf->rhomin, emf->rhomax);
+    std::vector<float> gsum(fwi_.n, 0.0f), hsum(fwi_.preco == 1 ? fwi_.n : 0, 0.0f), part(fwi_.n);
+    float fcost_dat = 0;
+    int ndata0 = 0;  // rank 0's data count, times nproc below, as rank 0 forms it
+    for (iproc = 0; iproc < nproc; ++iproc) {
+      set_verb();
+      Rank& r = ranks_[iproc];
+      acq_t* acq = &r.acq;
+      bind(r);
+      if (emf->verb) printf("--------step 1: forward modelling------------\n");
+      regrid_init(acq, emf);
+      {
+        // the forward and adjoint solves share grid, medium and frequencies
+        ShotSolve S(emf, p_);
+        for (size_t f = 0; f < nf; ++f) check(emmi_inject_source_dev(acq, emf, f, S.b[f], nullptr), "inject_source");
+        S.solve("forward");
+        for (size_t f = 0; f < nf; ++f) {
+          check(emmi_extract_emf_data_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "extract_emf_data");
+          check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ef_[f], nullptr), "extract_electric_field");
+        }
+        char fname[32];
+        snprintf(fname, sizeof fname, "syn_%04d.txt", acq->shot_idx[iproc]);
+        write_data(acq, emf, fname, emf->dcal_fd);
 
-  // every rank: forward, misfit, adjoint, gradient (src/fg_fwi.c:179-268)
-  std::vector<float> g(nxyz * fwi.npar, 0.0f), grad_h(nxyz * fwi.npar);
-  const size_t ebytes = sizeof(emmi_c64) * 3 * nxyz;
-  std::vector<void*> ef(nf), ea(nf);
-  for (size_t f = 0; f < nf; ++f) {
-    ef[f] = emmi_device_alloc(ebytes);
-    ea[f] = emmi_device_alloc(ebytes);
-    if (!ef[f] || !ea[f]) emmi_cli_err("device allocation of the model-grid fields failed");
-  }
-  float* grad_d = static_cast<float*>(emmi_device_alloc(sizeof(float) * nxyz * fwi.npar));
-  float* phess_d = fwi.preco == 1 ? static_cast<float*>(emmi_device_alloc(sizeof(float) * nxyz * fwi.npar)) : nullptr;
-  if (!grad_d || (fwi.preco == 1 && !phess_d)) emmi_cli_err("device allocation of the gradient failed");
-  float fcost_dat = 0;
-  int ndata = 0;
-  for (iproc = 0; iproc < nproc; ++iproc) {
-    emf->verb = verb_set ? verb_par : (iproc == 0);
-    Rank& r = ranks[iproc];
-    acq_t* acq = &r.acq;
-    bind(emf, r);
-    if (emf->verb) printf("--------step 1: forward modelling------------\n");
-    regrid_init(acq, emf);
-    {
-      // the forward and adjoint solves share the grid, the medium and the
-      // frequencies, so one solver serves both
-      ShotSolve S(emf, p);
-      for (int f = 0; f < emf->nfreq; ++f) check(emmi_inject_source_dev(acq, emf, f, S.b[f], nullptr), "inject_source");
-      S.solve("forward");
-      for (int f = 0; f < emf->nfreq; ++f) {
-        check(emmi_extract_emf_data_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "extract_emf_data");
-        check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ef[f], nullptr), "extract_electric_field");
+        if (emf->verb) printf("------step 2: compute fcost + adjoint source---------\n");
+        const float fcost = emmi_data_misfit(acq, emf);
+        write_rmse(acq, emf);
+        const int n = emf->nchrec * emf->nfreq * acq->nrec;
+        printf("isrc=%d, fcost=%g, rmse=%g\n", acq->shot_idx[iproc], fcost, std::sqrt(2. * fcost / n));
+        fcost_dat = fcost_dat + fcost;
+        if (iproc == 0) ndata0 = n;
+
+        if (emf->verb) printf("-----------step3: adjoint modelling------------\n");
+        for (size_t f = 0; f < nf; ++f)
+          check(emmi_inject_adj_source_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "inject_adj_source");
+        S.solve("adjoint");
+        for (size_t f = 0; f < nf; ++f)
+          check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ea_[f], nullptr), "extract_electric_field");
+
+        if (emf->verb) printf("----------step 4: compute gradient --------------\n");
+        check(emmi_build_gradient_dev(acq, emf, &fwi_, ef_.data(), ea_.data(), grad_d_, phess_d_, nullptr),
+              "build_gradient");
+        // the sums over ranks (MPI_Allreduce in the reference), rank order
+        if (emmi_memcpy_d2h(part.data(), grad_d_, sizeof(float) * fwi_.n)) emmi_cli_err("gradient copy failed");
+        for (int i = 0; i < fwi_.n; ++i) gsum[i] = gsum[i] + part[i];
+        if (fwi_.preco == 1) {
+          if (emmi_memcpy_d2h(part.data(), phess_d_, sizeof(float) * fwi_.n)) emmi_cli_err("pseudo-Hessian copy failed");
+          for (int i = 0; i < fwi_.n; ++i) hsum[i] = hsum[i] + part[i];
+        }
       }
-      char fname[32];
-      snprintf(fname, sizeof fname, "syn_%04d.txt", acq->shot_idx[iproc]);
-      write_data(acq, emf, fname, emf->dcal_fd);
-
-      if (emf->verb) printf("------step 2: compute fcost + adjoint source---------\n");
-      const float fcost = emmi_data_misfit(acq, emf);
-      write_rmse(acq, emf);
-      const int n = emf->nchrec * emf->nfreq * acq->nrec;
-      printf("isrc=%d, fcost=%g, rmse=%g\n", acq->shot_idx[iproc], fcost, std::sqrt(2. * fcost / n));
-      fcost_dat = fcost_dat + fcost;
-      ndata += n;
-
-      if (emf->verb) printf("-----------step3: adjoint modelling------------\n");
-      for (int f = 0; f < emf->nfreq; ++f)
-        check(emmi_inject_adj_source_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "inject_adj_source");
-      S.solve("adjoint");
-      for (int f = 0; f < emf->nfreq; ++f)
-        check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ea[f], nullptr), "extract_electric_field");
-
-      if (emf->verb) printf("----------step 4: compute gradient --------------\n");
-      check(emmi_build_gradient_dev(acq, emf, &fwi, ef.data(), ea.data(), grad_d, phess_d, nullptr), "build_gradient");
-      if (emmi_memcpy_d2h(grad_h.data(), grad_d, sizeof(float) * grad_h.size()))
-        emmi_cli_err("copy of the gradient to the host failed");
-      for (size_t i = 0; i < g.size(); ++i) g[i] = g[i] + grad_h[i];
+      regrid_free(emf);
     }
-    regrid_free(emf);
+    iproc = 0;
+    set_verb();
+    if (fwi_.preco == 1) memcpy(&fwi_.phess[0][0][0][0], hsum.data(), sizeof(float) * fwi_.n);
+    memcpy(g, gsum.data(), sizeof(float) * fwi_.n);
+    fwi_.fcost_dat = fcost_dat;
+    fwi_.mse = static_cast<float>(std::sqrt(2. * fcost_dat / (ndata0 * nproc)));
+    if (emf->verb) printf("fcost_dat=%g, rmse=%g\n", fwi_.fcost_dat, fwi_.mse);
+    if (emf->verb) {
+      // the data gradient before regularisation (src/fg_fwi.c:276-285)
+      write_bin("gradient_dat_Rv", g, sizeof(float) * nxyz_);
+      if (npar > 1) write_bin("gradient_dat_Rh", g + nxyz_, sizeof(float) * nxyz_);
+    }
+    fg_mod_reg(&ranks_[0].acq, emf, &fwi_, xx, g);
+    fwi_.fcost = fwi_.fcost_dat + fwi_.fcost_mod;
+    if (emf->verb) {
+      printf("fcost_dat=%g\n", fwi_.fcost_dat);
+      printf("fcost_mod=%g\n", fwi_.fcost_mod);
+      printf("fcost=%g\n", fwi_.fcost);
+    }
+    // l-BFGS is not scale invariant: the first gradient of an inversion fixes
+    // a scaling of misfit and gradient (src/fg_fwi.c:294-305)
+    if (emf->mode == 1 && fwi_.firstgrad) {
+      float s1 = 0, s2 = 0;
+      for (int k = 0; k < fwi_.n; ++k) {  // float sums of double moduli
+        s1 = static_cast<float>(s1 + std::fabs(static_cast<double>(xx[k])));
+        s2 = static_cast<float>(s2 + std::fabs(static_cast<double>(g[k])));
+      }
+      fwi_.alpha = static_cast<float>(2e-3 * s1 / s2);
+      if (emf->verb) printf("|x|_1=%g, |g|_1=%g, scaling=%g\n", s1, s2, fwi_.alpha);
+    }
+    for (int k = 0; k < fwi_.n; ++k) g[k] *= fwi_.alpha;
+    if (fwi_.firstgrad) fwi_.firstgrad = 0;
+    return fwi_.fcost * fwi_.alpha;
   }
-  iproc = 0;
-  emf->verb = verb_set ? verb_par : 1;
-  const float rms = static_cast<float>(std::sqrt(2. * fcost_dat / ndata));
-  if (emf->verb) {
-    printf("fcost_dat=%g, rmse=%g\n", fcost_dat, rms);
-    // the data gradient before regularisation (src/fg_fwi.c:276-285)
-    write_bin("gradient_dat_Rv", g.data(), sizeof(float) * nxyz);
-    if (fwi.npar > 1) write_bin("gradient_dat_Rh", g.data() + nxyz, sizeof(float) * nxyz);
-  }
-  for (void* q : ef) emmi_device_free(q);
-  for (void* q : ea) emmi_device_free(q);
-  emmi_device_free(grad_d);
-  if (phess_d) emmi_device_free(phess_d);
-  for (Rank& r : ranks) {
-    free3(r.dobs);
-    free3(r.dcal);
-    free3(r.dres);
-    free3(r.delta);
-    free3(r.rmse);
-    acq_free(&r.acq);
-  }
-  emf->dobs_fd = emf->dcal_fd = emf->dres_fd = nullptr;
-  emf->delta_emf = emf->rmse = nullptr;
-  free2(emf->bathy);
-  emf->bathy = nullptr;
-}
+};
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -392,8 +600,10 @@ int main(int argc, char** argv) {
                                          : "Task: Compute CSEM inversion gradient\n");
     printf("=====================================================\n");
   }
-  if (emf.mode == 1)
-    emmi_cli_err("mode=1 (the inversion loop) is not provided by this program; mode=0 and mode=2 are");
+  if (emf.mode != 0 && par_int("method", 0) != 0)
+    emmi_cli_err("method=%d: only l-BFGS (method=0) is provided; the Gauss-Newton option needs a Hessian-vector "
+                 "product that the reference does not implement either (src/fg_fwi.c:307-311)",
+                 par_int("method", 0));
   emf_init(&emf);
   const emmi_params_t p = solver_params(emf.verb);
   if (emf.mode == 0) {
@@ -406,7 +616,8 @@ int main(int argc, char** argv) {
     }
     iproc = 0;
   } else {
-    do_gradient(&emf, p, verb_set, emf.verb);
+    Inversion inv(&emf, p, verb_set, emf.verb);
+    inv.run();
   }
   emf_free(&emf);
   return 0;

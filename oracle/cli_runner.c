@@ -1,8 +1,10 @@
 /* cli_runner.c -- test-only driver over the reference's host sources
  * (src/cstd.c, src/emf_init_free.c, src/acq_init_free.c,
  * src/read_write_data.c, src/cal_uncertainty_noise.c, src/frannor.c,
- * compiled in place by oracle/Makefile target `cli`) that prints what they
- * compute, for the golden fixtures of the command-line program.
+ * src/regularization.c, src/triangle_smoothing.c, src/precondition.c,
+ * src/fg_mod_reg.c, compiled in place by oracle/Makefile target `cli`)
+ * that prints what they compute, for the golden fixtures of the
+ * command-line program.
  *
  * This file is test infrastructure: never linked into the product, its
  * binary lives in oracle/_ref/ (git-ignored).
@@ -12,6 +14,9 @@
  *   task=pars keys=k1,k2 types=i,f  getpar of each key, as the reference
  *                                reads it
  *   task=init rank=R nranks=P     emf_init + acq_init of rank R of P
+ *   task=reg|smooth|precond|modreg n1= n2= n3= fx= ...  regularisation,
+ *                                triangle smoothing, preconditioning and
+ *                                fg_mod_reg on volumes read from files
  *   task=noise rank=R nranks=P    emf_init, acq_init, read_data,
  *                                cal_uncertainty_noise, write_data(obs),
  *                                then the misfit of src/fg_fwi.c:207-225
@@ -26,6 +31,7 @@
 #include "cstd.h"
 #include "emf.h"
 #include "acq.h"
+#include "fwi.h"
 
 int iproc = 0, nproc = 1, ierr = 0;
 
@@ -37,6 +43,95 @@ void write_rmse(acq_t *acq, emf_t *emf);
 float cal_uncertainty_noise(acq_t *acq, emf_t *emf);
 float frannor(void);
 void srannor(int seed);
+float regularization_tikhonov(float *x, float *g, int n1, int n2, int n3, float d1, float d2, float d3);
+float regularization_tv(float *x, float *g, int n1, int n2, int n3, float d1, float d2, float d3);
+void triangle_smoothing(float ***mod, int n1, int n2, int n3, int r1, int r2, int r3, int repeat);
+void precondition(emf_t *emf, fwi_t *fwi, float *g);
+void fg_mod_reg(acq_t *acq, emf_t *emf, fwi_t *fwi, float *x, float *g);
+
+static void farr(const char *name, const float *a, int n);
+
+/* n floats from the file named by parameter key */
+static float *read_floats(const char *key, size_t n)
+{
+  char *name;
+  FILE *fp;
+  float *a = alloc1float(n);
+  if (!getparstring(key, &name)) err("need %s=", key);
+  fp = fopen(name, "rb");
+  if (!fp || fread(a, sizeof(float), n, fp) != n) err("cannot read %zu floats from %s", n, name);
+  fclose(fp);
+  return a;
+}
+
+/* the model-side functions of the inversion on volumes read from files:
+ * regularisation (task=reg), smoothing (task=smooth), preconditioning
+ * (task=precond preco=1|2|3) and fg_mod_reg (task=modreg) */
+static void task_model_side(const char *task)
+{
+  int n1, n2, n3, npar;
+  size_t nxyz, n;
+  float *x, *g, f;
+  if (!getparint("n1", &n1) || !getparint("n2", &n2) || !getparint("n3", &n3)) err("need n1 n2 n3");
+  if (!getparint("npar", &npar)) npar = 1;
+  nxyz = (size_t)n1*n2*n3;
+  n = nxyz*npar;
+  x = read_floats("fx", n);
+  if (strcmp(task, "reg") == 0) {
+    g = alloc1float(nxyz);
+    f = regularization_tikhonov(x, g, n1, n2, n3, 1, 1, 1);
+    printf("tik %a\n", f); farr("gtik", g, nxyz);
+    f = regularization_tv(x, g, n1, n2, n3, 1, 1, 1);
+    printf("tv %a\n", f); farr("gtv", g, nxyz);
+  } else if (strcmp(task, "smooth") == 0) {
+    int r1, r2, r3, rep;
+    float ***m = alloc3float(n1, n2, n3);
+    if (!getparint("r1", &r1)) r1 = 3;
+    if (!getparint("r2", &r2)) r2 = 3;
+    if (!getparint("r3", &r3)) r3 = 1;
+    if (!getparint("repeat", &rep)) rep = 1;
+    memcpy(&m[0][0][0], x, nxyz*sizeof(float));
+    triangle_smoothing(m, n1, n2, n3, r1, r2, r3, rep);
+    farr("smooth", &m[0][0][0], nxyz);
+  } else {
+    emf_t e;
+    fwi_t w;
+    acq_t a;
+    float freq;
+    memset(&e, 0, sizeof e); memset(&w, 0, sizeof w); memset(&a, 0, sizeof a);
+    e.nx = n1; e.ny = n2; e.nz = n3;
+    if (!getparfloat("dx", &e.dx)) e.dx = 100;
+    if (!getparfloat("dy", &e.dy)) e.dy = 100;
+    if (!getparfloat("dz", &e.dz)) e.dz = 50;
+    if (!getparfloat("oz", &e.oz)) e.oz = 0;
+    if (!getparfloat("freq", &freq)) freq = 0.5;
+    e.omegas = alloc1float(1);
+    e.omegas[0] = 2.*PI*freq;
+    e.bathy = alloc2float(n1, n2);
+    memcpy(&e.bathy[0][0], read_floats("fbathy", (size_t)n1*n2), (size_t)n1*n2*sizeof(float));
+    w.npar = npar; w.n = n;
+    if (!getparint("preco", &w.preco)) w.preco = 1;
+    if (!getparint("r1", &w.r1)) w.r1 = 3;
+    if (!getparint("r2", &w.r2)) w.r2 = 3;
+    if (!getparint("r3", &w.r3)) w.r3 = 1;
+    if (!getparint("repeat", &w.repeat)) w.repeat = 1;
+    if (!getparint("iter", &w.iter)) w.iter = 0;
+    if (!getparfloat("gamma1", &w.gamma1)) w.gamma1 = 10;
+    if (!getparfloat("gamma2", &w.gamma2)) w.gamma2 = 0;
+    w.grad = alloc4float(n1, n2, n3, npar);
+    w.phess = alloc4float(n1, n2, n3, npar);
+    memcpy(&w.phess[0][0][0][0], read_floats("fphess", n), n*sizeof(float));
+    w.xref = read_floats("fxref", n);
+    g = read_floats("fg", n);
+    if (strcmp(task, "precond") == 0) {
+      precondition(&e, &w, g);
+    } else {
+      fg_mod_reg(&a, &e, &w, x, g);
+      printf("fcost_mod %a\n", w.fcost_mod);
+    }
+    farr("g", g, n);
+  }
+}
 
 static void task_rannor(void)
 {
@@ -185,6 +280,10 @@ int main(int argc, char **argv)
   if (!getparstring("task", &task)) task = "init";
   if (strcmp(task, "rannor") == 0) { task_rannor(); return 0; }
   if (strcmp(task, "pars") == 0) { task_pars(); return 0; }
+  if (!strcmp(task, "reg") || !strcmp(task, "smooth") || !strcmp(task, "precond") || !strcmp(task, "modreg")) {
+    task_model_side(task);
+    return 0;
+  }
   if (!getparint("nranks", &nproc)) nproc = 1;
   if (!getparint("rank", &iproc)) iproc = 0;
   memset(&emf, 0, sizeof emf);

@@ -2,7 +2,9 @@
 
 Run in a tree that has the reference (it builds oracle/_ref/cli_runner from
 the reference's own src/cstd.c, src/emf_init_free.c, src/acq_init_free.c,
-src/read_write_data.c, src/cal_uncertainty_noise.c and src/frannor.c, compiled
+src/read_write_data.c, src/cal_uncertainty_noise.c, src/frannor.c,
+src/regularization.c, src/triangle_smoothing.c, src/precondition.c and
+src/fg_mod_reg.c, compiled
 where they lie):
 
     python tests/golden/make_cli_fixtures.py
@@ -10,8 +12,10 @@ where they lie):
 Outputs (all data, no reference source):
   cli_fixtures.json      what the reference's functions compute for the
                          inputs of tests/helpers.py: frannor sequences,
-                         getpar lookups, emf_init/acq_init results, and the
-                         uncertainty, noise and misfit of a 6-channel survey
+                         getpar lookups, emf_init/acq_init results, the
+                         uncertainty, noise and misfit of a 6-channel survey,
+                         and regularisation, smoothing, preconditioning and
+                         fg_mod_reg on seeded random volumes
   run_inversion_0001/    the reference's own example data of shot 1 of
                          run_inversion/ (its Ex lines of emf_0001.txt, which
                          are all that run's chrec=Ex reads, and obs_0001.txt,
@@ -58,6 +62,51 @@ def after_dump(text: str) -> list:
     return lines[lines.index("---- dump") + 1:]
 
 
+MODEL_SHAPE = (9, 7, 5)   # n1 (x), n2 (y), n3 (z)
+
+
+def model_side_cases(d: str) -> list:
+    """Regularisation, smoothing, preconditioning and fg_mod_reg of the
+    reference on seeded random volumes (inputs kept in the fixture)."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    n1, n2, n3 = MODEL_SHAPE
+    nxyz = n1 * n2 * n3
+
+    def put(name, a):
+        a = np.ascontiguousarray(a, np.float32)
+        a.tofile(os.path.join(d, name + ".bin"))
+        return [float(v).hex() for v in a.reshape(-1)]
+
+    x1 = rng.standard_normal(nxyz).astype(np.float32)
+    x2 = (rng.standard_normal(2 * nxyz) * 0.7 + 1.0).astype(np.float32)
+    xref = (x2 + rng.standard_normal(2 * nxyz).astype(np.float32) * np.float32(0.2)).astype(np.float32)
+    g2 = rng.standard_normal(2 * nxyz).astype(np.float32) * np.float32(1e-3)
+    phess = np.abs(rng.standard_normal(2 * nxyz)).astype(np.float32) * np.float32(1e-2)
+    phess[::7] = 0
+    bathy = rng.uniform(20.0, 180.0, n1 * n2).astype(np.float32)
+    inputs = {"x1": put("x1", x1), "x2": put("x2", x2), "xref": put("xref", xref), "g2": put("g2", g2),
+              "phess": put("phess", phess), "bathy": put("bathy", bathy)}
+    dims = [f"n1={n1}", f"n2={n2}", f"n3={n3}"]
+    files = {k: os.path.join(d, k + ".bin") for k in inputs}
+    cases = []
+    spec = [
+        ("reg", ["npar=1", "fx=" + files["x1"]]),
+        ("smooth", ["npar=1", "fx=" + files["x1"], "r1=3", "r2=2", "r3=1", "repeat=2"]),
+        ("smooth", ["npar=1", "fx=" + files["x1"], "r1=1", "r2=4", "r3=2", "repeat=1"]),
+    ]
+    common = ["npar=2", "fx=" + files["x2"], "fxref=" + files["xref"], "fg=" + files["g2"],
+              "fphess=" + files["phess"], "fbathy=" + files["bathy"], "dz=50", "oz=0", "freq=0.5"]
+    for preco in (1, 2, 3):
+        spec.append(("precond", common + [f"preco={preco}", "r1=2", "r2=3", "r3=2", "repeat=1"]))
+    spec.append(("modreg", common + ["iter=2", "gamma1=10", "gamma2=0.5"]))
+    spec.append(("modreg", common + ["iter=0", "gamma1=0", "gamma2=2"]))
+    for task, args in spec:
+        cases.append({"task": task, "args": [a for a in args if not a.startswith("f")] ,
+                      "lines": run([f"task={task}"] + dims + args).splitlines()})
+    return {"shape": list(MODEL_SHAPE), "inputs": inputs, "cases": cases}
+
+
 def main() -> None:
     from helpers import cli_args, cli_write_inputs
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "cli"])
@@ -99,6 +148,7 @@ def main() -> None:
                 rmse = fp.read()
             out["noise"][name] = {"extra": extra, "lines": lines, "obs": obs, "rmse": rmse}
         out["args"] = base
+        out["model_side"] = model_side_cases(d)
     with open(os.path.join(HERE, "cli_fixtures.json"), "w") as fp:
         json.dump(out, fp, indent=0)
 

@@ -293,9 +293,9 @@ def _exe(args, cwd):
     return subprocess.run([cli.EXE] + args, capture_output=True, text=True, cwd=cwd, timeout=60)
 
 
-def test_emmi_main_refuses_mode1(tmp_path):
-    r = _exe(["mode=1", "freqs=1"], str(tmp_path))
-    assert r.returncode == 1 and "mode=1" in r.stderr
+def test_emmi_main_refuses_gauss_newton(tmp_path):
+    r = _exe(["mode=1", "method=1", "freqs=1"], str(tmp_path))
+    assert r.returncode == 1 and "method=1" in r.stderr
 
 
 def test_emmi_main_reports_missing_parameters(tmp_path):
@@ -320,3 +320,85 @@ def test_every_declared_cli_function_is_exported():
     assert len(decl) >= 15
     assert not [n for n in decl if syms.get(n) != "T"], decl
     assert syms.get("iproc") in ("B", "D") and syms.get("nproc") in ("B", "D")
+
+
+# ---------------------------------------------------------------------------
+# regularisation, smoothing, preconditioning, fg_mod_reg
+# ---------------------------------------------------------------------------
+def _hexarr(tokens):
+    return np.array([f32(h) for h in tokens], np.float32)
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def _model_side_lib(L):
+    from libemmi_mgfd_amd import carriers
+    P = C.POINTER
+    fp = P(C.c_float)
+    L.regularization_tikhonov.argtypes = [fp, fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
+    L.regularization_tikhonov.restype = C.c_float
+    L.regularization_tv.argtypes = L.regularization_tikhonov.argtypes
+    L.regularization_tv.restype = C.c_float
+    L.triangle_smoothing.argtypes = [C.c_void_p] + [C.c_int] * 7
+    L.triangle_smoothing.restype = None
+    L.precondition.argtypes = [P(carriers.Emf), P(carriers.Fwi), fp]
+    L.precondition.restype = None
+    L.fg_mod_reg.argtypes = [P(carriers.Acq), P(carriers.Emf), P(carriers.Fwi), fp, fp]
+    L.fg_mod_reg.restype = None
+
+
+def test_model_side_matches_reference(L, fx):
+    from libemmi_mgfd_amd import carriers
+    _model_side_lib(L)
+    ms = fx["model_side"]
+    n1, n2, n3 = ms["shape"]
+    nxyz = n1 * n2 * n3
+    inp = {k: _hexarr(v) for k, v in ms["inputs"].items()}
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    for case in ms["cases"]:
+        pars = dict(a.split("=", 1) for a in case["args"])
+        want = {ln.split(" ", 1)[0]: ln.split(" ")[1:] for ln in case["lines"]}
+        task = case["task"]
+        if task == "reg":
+            x = inp["x1"].copy()
+            g = np.zeros(nxyz, np.float32)
+            f = np.float32(L.regularization_tikhonov(fp(x), fp(g), n1, n2, n3, 1, 1, 1))
+            assert _bits_equal(f, f32(want["tik"][0])) and _bits_equal(g, _hexarr(want["gtik"][1:]))
+            f = np.float32(L.regularization_tv(fp(x), fp(g), n1, n2, n3, 1, 1, 1))
+            assert _bits_equal(f, f32(want["tv"][0])) and _bits_equal(g, _hexarr(want["gtv"][1:]))
+        elif task == "smooth":
+            T = carriers.Tables()
+            m = inp["x1"].reshape(n3, n2, n1).copy()
+            L.triangle_smoothing(T.f3(m), n1, n2, n3, int(pars["r1"]), int(pars["r2"]), int(pars["r3"]),
+                                 int(pars["repeat"]))
+            assert _bits_equal(T.keep[0].reshape(-1), _hexarr(want["smooth"][1:])), pars
+        else:
+            T = carriers.Tables()
+            emf, fwi, acq = carriers.Emf(), carriers.Fwi(), carriers.Acq()
+            emf.nx, emf.ny, emf.nz = n1, n2, n3
+            emf.dx = emf.dy = 100.0
+            emf.dz, emf.oz = float(pars["dz"]), float(pars["oz"])
+            emf.omegas = T.f1(np.array([carriers.omega_f32(0.5)], np.float32))
+            emf.bathy = T.f2(inp["bathy"].reshape(n2, n1))
+            npar = int(pars["npar"])
+            fwi.npar, fwi.n = npar, npar * nxyz
+            fwi.preco = int(pars.get("preco", 1))
+            fwi.r1, fwi.r2, fwi.r3, fwi.repeat = (int(pars.get(k, d)) for k, d in
+                                                   (("r1", 3), ("r2", 3), ("r3", 1), ("repeat", 1)))
+            fwi.iter = int(pars.get("iter", 0))
+            fwi.gamma1, fwi.gamma2 = float(pars.get("gamma1", 10)), float(pars.get("gamma2", 0))
+            grad = np.zeros((npar, n3, n2, n1), np.float32)
+            phess = inp["phess"].reshape(npar, n3, n2, n1).copy()
+            fwi.grad, fwi.phess = T.f4(grad), T.f4(phess)
+            xref = inp["xref"].copy()
+            fwi.xref = fp(xref)
+            g = inp["g2"].copy()
+            x = inp["x2"].copy()
+            if task == "precond":
+                L.precondition(C.byref(emf), C.byref(fwi), fp(g))
+            else:
+                L.fg_mod_reg(C.byref(acq), C.byref(emf), C.byref(fwi), fp(x), fp(g))
+                assert _bits_equal(np.float32(fwi.fcost_mod), f32(want["fcost_mod"][0])), pars
+            assert _bits_equal(g, _hexarr(want["g"][1:])), (task, pars)

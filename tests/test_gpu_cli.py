@@ -130,3 +130,59 @@ def test_gradient_two_shots(tmp_path):
         err = float(np.max(np.abs(b - a)) / np.max(np.abs(a)))
         print(f"{k}: max |diff| / max |g| = {err:.2e}", flush=True)
         assert np.max(np.abs(a)) > 0 and err < 1e-5, k
+
+
+FWI = os.path.join(ROOT, "oracle", "_ref", "fwi_runner")
+
+
+def _tokens_close(path_ref, path_gpu, rtol):
+    """Same lines; integers equal, other numbers within rtol."""
+    lr = open(path_ref).read().split("\n")
+    lg = open(path_gpu).read().split("\n")
+    assert len(lr) == len(lg), path_gpu
+    for a, b in zip(lr, lg):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb), (a, b)
+        for x, y in zip(ta, tb):
+            try:
+                fx, fy = float(x), float(y)
+            except ValueError:
+                assert x == y, (a, b)
+                continue
+            if x.lstrip("-").isdigit():
+                assert x == y, (path_gpu, a, b)
+            else:
+                assert abs(fx - fy) <= rtol * abs(fx) + 1e-30, (path_gpu, a, b)
+
+
+@pytest.mark.skipif(not os.path.exists(FWI), reason="oracle/_ref/fwi_runner not built")
+def test_inversion_two_iterations(tmp_path):
+    """mode=1: two l-BFGS iterations (up to 3 line-search steps each, default
+    preconditioner and Tikhonov term) against the reference's own do_fwi.c,
+    opt.c, fg_mod_reg.c, regularization.c and precondition.c on one rank."""
+    files = cli_write_inputs(str(tmp_path / "in"))
+    args = cli_args(files, ["chrec=Ex,Ey,Hx,Hy", "idxpar=1,2", "dsmute=300", "drmute=150", "niter=2", "nls=3",
+                            "shots=1"]) + TIGHT
+    for side, exe in (("gpu", EXE), ("ref", FWI)):
+        d = tmp_path / side
+        d.mkdir()
+        shutil.copy(tmp_path / "in" / "emf_0001.txt", d)
+        out = run([exe, "mode=1"] + args, str(d), timeout=400)
+        if side == "gpu":
+            converged(out)
+    gpu, ref = tmp_path / "gpu", tmp_path / "ref"
+    assert (ref / "obs_0001.txt").read_text() == (gpu / "obs_0001.txt").read_text()
+    print((gpu / "iterate.txt").read_text(), flush=True)
+    _tokens_close(ref / "iterate.txt", gpu / "iterate.txt", 1.5e-2)   # 3 printed digits
+    _tokens_close(ref / "rmse_misfit.txt", gpu / "rmse_misfit.txt", 1e-4)
+    for name in ("syn_0001.txt", "rmse_0001.txt"):
+        worst = numbers_close(str(ref / name), str(gpu / name), rtol=1e-4)
+        print(f"{name}: max rel {worst:.2e}", flush=True)
+    for name in ("Rv_iteration_00", "Rh_iteration_00", "Rv_iteration_01", "Rh_iteration_01", "gradient_dat_Rv",
+                 "gradient_dat_Rh", "gradient_tikhonov_Rh", "gradient_tikhonov_Rv"):
+        a = np.fromfile(ref / name, np.float32)
+        b = np.fromfile(gpu / name, np.float32)
+        assert a.shape == b.shape, name
+        err = float(np.max(np.abs(b - a)) / max(np.max(np.abs(a)), 1e-30))
+        print(f"{name}: max |diff| / max = {err:.2e}", flush=True)
+        assert np.all(np.isfinite(b)) and err < 1e-4, name
