@@ -1,8 +1,9 @@
 /* emmi_cli.h -- host side of the libEMMI_MGFD command line, as the library
  * libemmi_cli.so exports it: parameter parsing, the survey and data files,
- * and the data weighting of the inversion.  The executable emmi_main runs
- * the reference's own command lines (mode=0 modelling, mode=2 gradient) with
- * these functions around the GPU solver of libemmi_mgfd.so.
+ * the data weighting, regularisation and preconditioning of the inversion.
+ * The executable emmi_main runs the reference's own command lines (mode=0
+ * modelling, mode=1 inversion, mode=2 gradient) with these functions around
+ * the GPU solver of libemmi_mgfd.so.
  *
  * Every function keeps the name, argument meaning and error behaviour of the
  * reference function it replaces (cited per declaration); errors print
