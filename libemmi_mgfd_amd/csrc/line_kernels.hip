@@ -507,6 +507,81 @@ __device__ __forceinline__ void bmul(const LineGeo<T, A>& G, const cxd (&z)[4], 
 template <typename R, int NC>
 __device__ __forceinline__ cxd cval(const Ch<R> (&c)[NC], int k) { return cxd(val(c, k), val(c, k + 1)); }
 
+// Lines along x (axis 0) lie in different x rows, so a wave updating u node
+// by node would write 64 scattered 8-byte values per store, and the L2
+// evicts those partly written lines: the substitution along x wrote 2.7x the
+// bytes of the other axes.  Instead it leaves each node's corrections in the
+// node's line record (its z values are dead by then; values 0,1: the
+// along-line edge, 2..9: the four node edges) and k_line_solve_x_apply adds
+// them to u with the lanes running along x.
+template <typename RS>
+__device__ __forceinline__ void put_corr(Ch<RS>* __restrict__ rec, long stride, cxd ea, const cxd (&ap)[4]) {
+  constexpr int SC = schunks<RS>();
+  Ch<RS> c[SC] = {};
+  put(c, 0, ea.re);
+  put(c, 1, ea.im);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    put(c, 2 + 2 * i, ap[i].re);
+    put(c, 3 + 2 * i, ap[i].im);
+  }
+#pragma unroll
+  for (int k = 0; k < SC; k++) rec[k * stride] = c[k];
+}
+
+// u += the corrections left in the line records by k_line_solve<T, 0>: a
+// block takes 16 nodes x 16 lines, reads the records with lines fastest and
+// updates u through an LDS transpose with nodes fastest.  Node m = N/2 (its
+// node edges are updated by the substitution itself) and node 0 hold no
+// correction; the along-line edge of node p is p below m and p-1 above.
+template <typename T>
+__global__ void __launch_bounds__(XT * XT) k_line_solve_x_apply(LinePack<T> P, int nlb) {
+  constexpr int A = 0;
+  using RS = ScrRec<T>;
+  constexpr int SC = schunks<RS>();
+  __shared__ double tile[SREC][XT][XT + 1];
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  const long nl = (long)J.nlo * J.nhi;
+  const int N = L.n[A], m = N / 2;
+  const int lb = blockIdx.x % nlb, pbk = blockIdx.x / nlb;
+  const long l0 = (long)lb * XT;
+  const int p0 = pbk * XT;
+  if (l0 >= nl || p0 >= N) return;   // uniform over the block
+  {
+    const int ll = threadIdx.x % XT, pl = threadIdx.x / XT;
+    const int p = p0 + pl;
+    const long line = l0 + ll;
+    if (p < N && line < nl && p >= 1 && p != m) {
+      const Ch<RS>* rec = reinterpret_cast<const Ch<RS>*>(J.scr) + line + (long)p * SC * nl;
+      Ch<RS> c[SC];
+#pragma unroll
+      for (int k = 0; k < SC; k++) c[k] = rec[k * nl];
+#pragma unroll
+      for (int k = 0; k < SREC; k++) tile[k][pl][ll] = val(c, k);
+    }
+  }
+  __syncthreads();
+  {
+    const int pl = threadIdx.x % XT, ll = threadIdx.x / XT;
+    const int p = p0 + pl;
+    const long line = l0 + ll;
+    if (p < N && line < nl && p >= 1 && p != m) {
+      int pb, pc;
+      line_of<T, A>(J, (int)line, pb, pc);
+      const LineGeo<T, A> G(L, pb, pc);
+      typename cvec<T>::type* __restrict__ u = J.u;
+      const long o = (long)p * G.sa;
+      const long ea = G.oa + (long)(p < m ? p : p - 1) * G.sa;
+      st<T>(u, ea, ld<T>(u, ea) + cxd(tile[0][pl][ll], tile[1][pl][ll]));
+      const long offs[4] = {G.ob0, G.ob1, G.oc0, G.oc1};
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        st<T>(u, offs[i] + o, ld<T>(u, offs[i] + o) + cxd(tile[2 + 2 * i][pl][ll], tile[3 + 2 * i][pl][ll]));
+    }
+  }
+}
+
 // Block substitution of the lines of one colour on the twisted factors
 // (k_line_factor).  A block of 128 threads takes 64 lines: wave 0 ("top")
 // eliminates nodes 1..m-1 downwards and wave 1 ("bottom") nodes M..m+1
@@ -728,8 +803,10 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       load_node(rf[d], fac + pn * fstep, stride);
       load_node(rs[d], scr + pn * sstep, stride);
       riw[d] = iw[pn];
+      if constexpr (A != 0) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+        for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+      }
     }
     cxd an[4];   // x_{p+1}
 #pragma unroll
@@ -746,16 +823,20 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
           for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          if constexpr (A != 0) {
 #pragma unroll
-          for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+            for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+          }
           const double iWe = (double)riw[d];
           {
             const int pn = max(p - D, 1);
             load_node(rf[d], fac + pn * fstep, stride);
             load_node(rs[d], scr + pn * sstep, stride);
             riw[d] = iw[pn];
+            if constexpr (A != 0) {
 #pragma unroll
-            for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+              for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+            }
           }
           const cxd rd = cval(cf, 20);
           const cxd q = cval(cr, 8) * rd;
@@ -767,10 +848,15 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
           cxd dv(0, 0);
 #pragma unroll
           for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
-          const long o = (long)p * G.sa;
-          st<T>(u, G.oa + o, cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd));
+          if constexpr (A == 0) {
+            put_corr(scr + p * sstep, stride, q + dv * rd, ap);
+          } else {
+            const long o = (long)p * G.sa;
+            st<T>(u, G.oa + o, cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd));
 #pragma unroll
-          for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+            for (int i = 0; i < 4; i++)
+              st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+          }
 #pragma unroll
           for (int i = 0; i < 4; i++) an[i] = ap[i];
         }
@@ -793,9 +879,11 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       load_node(rf[d], fac + pn * fstep, stride);
       load_node(rs[d], scr + pn * sstep, stride);
       riw[d] = iw[pn - 1];
-      ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
+      if constexpr (A != 0) {
+        ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
 #pragma unroll
-      for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+        for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+      }
     }
     cxd xp[4];   // x_{p-1}
 #pragma unroll
@@ -813,17 +901,21 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
           for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
 #pragma unroll
           for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          if constexpr (A != 0) {
 #pragma unroll
-          for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+            for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
+          }
           const double iWe = (double)riw[d];
           {
             const int pn = min(p + D, M);
             load_node(rf[d], fac + pn * fstep, stride);
             load_node(rs[d], scr + pn * sstep, stride);
             riw[d] = iw[pn - 1];
-            ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
+            if constexpr (A != 0) {
+              ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
 #pragma unroll
-            for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+              for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+            }
           }
           const cxd rd = cval(cf, 20);
           const cxd q = cval(cr, 8) * rd;
@@ -835,10 +927,15 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
           cxd dv(0, 0);
 #pragma unroll
           for (int i = 0; i < 4; i++) dv += (ap[i] - xp[i]) * G.v[i];
-          const long o = (long)p * G.sa;
-          st<T>(u, G.oa + o - G.sa, cxd((double)cu[0].x, (double)cu[0].y) + (qm + dv * rdm));
+          if constexpr (A == 0) {
+            put_corr(scr + p * sstep, stride, qm + dv * rdm, ap);
+          } else {
+            const long o = (long)p * G.sa;
+            st<T>(u, G.oa + o - G.sa, cxd((double)cu[0].x, (double)cu[0].y) + (qm + dv * rdm));
 #pragma unroll
-          for (int i = 0; i < 4; i++) st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+            for (int i = 0; i < 4; i++)
+              st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+          }
 #pragma unroll
           for (int i = 0; i < 4; i++) xp[i] = ap[i];
           rdm = rd;
@@ -1309,8 +1406,16 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     }
   } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
-    if (axis == 0) launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);
-    else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, solve_bytes, P);
+    if (axis == 0) {
+      // the substitution's algorithmic bytes include the update of u, which
+      // the second kernel makes (counted once, with the substitution)
+      int maxn = 0;
+      for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
+      const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
+      launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);
+      launch_timed(k_line_solve_x_apply<T>, dim3((unsigned)(nlb * npb), P.njobs), dim3(XT * XT), s, timing::LINE, 0.0,
+                   P, nlb);
+    } else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, solve_bytes, P);
     else launch_timed(k_line_solve<T, 2>, g, b, s, timing::LINE, solve_bytes, P);
   }
 }
