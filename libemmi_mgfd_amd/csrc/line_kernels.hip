@@ -337,6 +337,270 @@ __global__ void __launch_bounds__(64) k_line_factor(LinePack<T> P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// single-line launches: parallel cyclic reduction
+// ---------------------------------------------------------------------------
+// The coarsest level of a semi-coarsening pattern is one line along the
+// kept axis (the full grid length), so the substitution would be one serial
+// chain of N/2 nodes per half.  With one line per launch, the node system
+//     B_p x_{p-1} + D_p x_p + B_{p+1} x_{p+1} = g_p,   p = 1..M, x_0 = x_N = 0
+// is reduced instead: at stride s = 1, 2, 4, ... each equation eliminates its
+// neighbours at distance s,
+//     alpha_p = -A_p D_{p-s}^-1,  gamma_p = -C_p D_{p+s}^-1
+//     D_p <- D_p + alpha_p C_{p-s} + gamma_p A_{p+s},  A_p <- alpha_p A_{p-s},
+//     C_p <- gamma_p C_{p+s},  g_p <- g_p + alpha_p g_{p-s} + gamma_p g_{p+s}
+// (A_p couples p to p-s, C_p to p+s, both start as the line's B), until no
+// coupling is left and x_p = D_p^-1 g_p.  k_line_pcr_factor builds alpha,
+// gamma and D^-1 once per frequency; k_line_pcr_solve then takes log2(M)
+// block-wide steps.  Launches of one line of up to PCR_MAX_NODES nodes use it.
+constexpr int PCR_MAX_NODES = 256;
+
+__host__ __device__ constexpr int pcr_levels(int M) {
+  int k = 0;
+  for (int s = 1; s < M; s *= 2) k++;
+  return k;
+}
+// complex entries of the factor buffer: alpha and gamma per level and node,
+// D^-1 and 1/d per node, two work copies of (D, A, C) per node
+__host__ __device__ constexpr long pcr_entries(int N) {
+  return (long)N * (32L * pcr_levels(N - 1) + 16 + 1 + 2 * 48);
+}
+
+struct PcrLayout {
+  cxd *alpha, *dinv, *rd, *work;
+  int N, nlev;
+  __device__ PcrLayout(void* base, int N_) : N(N_), nlev(pcr_levels(N_ - 1)) {
+    cxd* b = reinterpret_cast<cxd*>(base);
+    alpha = b;                                   // [lev][p][2][16]: alpha, gamma
+    dinv = alpha + (long)nlev * N * 32;          // [p][16]
+    rd = dinv + (long)N * 16;                    // [e]
+    work = rd + N;                               // [2][p][48]: D, A, C
+  }
+  __device__ cxd* ag(int lev, int p, int which) const { return alpha + (((long)lev * N + p) * 2 + which) * 16; }
+  __device__ cxd* w(int buf, int p) const { return work + ((long)buf * N + p) * 48; }
+};
+
+__device__ __forceinline__ void ld16(const cxd* s, cxd (&M)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) M[i][j] = s[4 * i + j];
+}
+__device__ __forceinline__ void st16(cxd* d, const cxd (&M)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) d[4 * i + j] = M[i][j];
+}
+// C = A B
+__device__ __forceinline__ void mm4(const cxd (&A)[4][4], const cxd (&B)[4][4], cxd (&C)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      cxd acc(0, 0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc += A[i][k] * B[k][j];
+      C[i][j] = acc;
+    }
+}
+__device__ __forceinline__ void zero4(cxd (&M)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) M[i][j] = cxd(0, 0);
+}
+// the coupling B = diag(l) + v v^T rd of two neighbouring nodes
+template <typename T, int A>
+__device__ __forceinline__ void coupling(const Level<T>& L, const LineGeo<T, A>& G, int e, cxd rd, cxd (&Bm)[4][4]) {
+  double l[4];
+  G.lcoef(L, e, l);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) Bm[i][j] = (G.v[i] * G.v[j]) * rd + cxd(i == j ? l[i] : 0.0, 0);
+}
+
+template <typename T, int A>
+__global__ void __launch_bounds__(PCR_MAX_NODES) k_line_pcr_factor(LinePack<T> P) {
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  int pb, pc;
+  if (!line_of<T, A>(J, 0, pb, pc)) return;   // uniform over the block
+  const LineGeo<T, A> G(L, pb, pc);
+  const int N = G.N;
+  if (N < 2 || N > PCR_MAX_NODES) return;     // uniform
+  const int M = N - 1, p = threadIdx.x;
+  const PcrLayout F(J.fac, N);
+  for (int e = p; e <= M; e += blockDim.x) F.rd[e] = cinv(G.d_edge(L, e));
+  __syncthreads();
+  const bool own = p >= 1 && p <= M;
+  if (own) {
+    cxd Dm[4][4], Am[4][4], Cm[4][4];
+    diag_block(L, G, p, F.rd[p - 1], F.rd[p], Dm);
+    if (p >= 2) coupling(L, G, p - 1, F.rd[p - 1], Am);
+    else zero4(Am);
+    if (p <= M - 1) coupling(L, G, p, F.rd[p], Cm);
+    else zero4(Cm);
+    cxd* w = F.w(0, p);
+    st16(w, Dm);
+    st16(w + 16, Am);
+    st16(w + 32, Cm);
+  }
+  __syncthreads();
+  int cur = 0, lev = 0;
+  for (int s = 1; s < M; s *= 2, lev++) {
+    if (own) {
+      cxd Dm[4][4], Am[4][4], Cm[4][4], al[4][4], ga[4][4], t[4][4], u[4][4];
+      const cxd* w = F.w(cur, p);
+      ld16(w, Dm);
+      ld16(w + 16, Am);
+      ld16(w + 32, Cm);
+      cxd An[4][4], Cn[4][4];
+      zero4(An);
+      zero4(Cn);
+      if (p - s >= 1) {
+        const cxd* wm = F.w(cur, p - s);
+        ld16(wm, t);
+        inv4(t);
+        mm4(Am, t, al);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) al[i][j] = -al[i][j];
+        ld16(wm + 32, t);   // C_{p-s}
+        mm4(al, t, u);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) Dm[i][j] += u[i][j];
+        ld16(wm + 16, t);   // A_{p-s}
+        mm4(al, t, An);
+      } else {
+        zero4(al);
+      }
+      if (p + s <= M) {
+        const cxd* wp = F.w(cur, p + s);
+        ld16(wp, t);
+        inv4(t);
+        mm4(Cm, t, ga);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) ga[i][j] = -ga[i][j];
+        ld16(wp + 16, t);   // A_{p+s}
+        mm4(ga, t, u);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) Dm[i][j] += u[i][j];
+        ld16(wp + 32, t);   // C_{p+s}
+        mm4(ga, t, Cn);
+      } else {
+        zero4(ga);
+      }
+      st16(F.ag(lev, p, 0), al);
+      st16(F.ag(lev, p, 1), ga);
+      cxd* wn = F.w(cur ^ 1, p);
+      st16(wn, Dm);
+      st16(wn + 16, An);
+      st16(wn + 32, Cn);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (own) {
+    cxd Dm[4][4];
+    ld16(F.w(cur, p), Dm);
+    inv4(Dm);
+    st16(F.dinv + (long)p * 16, Dm);
+  }
+}
+
+template <typename T, int A>
+__global__ void __launch_bounds__(4 * PCR_MAX_NODES) k_line_pcr_solve(LinePack<T> P) {
+  using RS = ScrRec<T>;
+  constexpr int SC = schunks<RS>();
+  // LDS holds the complex values as double2 (cxd has a constructor)
+  __shared__ double2 g_[2][PCR_MAX_NODES][4];
+  __shared__ double2 xs_[PCR_MAX_NODES + 1][4];
+  __shared__ double2 qs_[PCR_MAX_NODES];
+  auto get = [](const double2& a) { return cxd(a.x, a.y); };
+  auto set = [](double2& a, cxd v) { a.x = v.re; a.y = v.im; };
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const Level<T>& L = J.L;
+  int pb, pc;
+  if (!line_of<T, A>(J, 0, pb, pc)) return;   // uniform over the block
+  const LineGeo<T, A> G(L, pb, pc);
+  const int N = G.N;
+  if (N < 2 || N > PCR_MAX_NODES) return;     // uniform
+  const int M = N - 1, tid = threadIdx.x;
+  const PcrLayout F(J.fac, N);
+  // residual records of the line's nodes (as k_line_rhs writes them)
+  for (int p = tid; p <= M; p += blockDim.x) node_rhs<T, A>(J, 0, pb, pc, p, 1);
+  __syncthreads();   // the records are complete (workgroup scope)
+  const Ch<RS>* rec = reinterpret_cast<const Ch<RS>*>(J.scr);
+  for (int e = tid; e <= M; e += blockDim.x) {
+    Ch<RS> c[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) c[k] = rec[(long)e * SC + k];
+    set(qs_[e], cval(c, 8) * F.rd[e]);
+  }
+  __syncthreads();
+  const int p = tid / 4 + 1, r = tid % 4;
+  const bool own = p <= M;
+  if (own) {
+    Ch<RS> c[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) c[k] = rec[(long)p * SC + k];
+    set(g_[0][p][r], cval(c, 2 * r) + (get(qs_[p - 1]) - get(qs_[p])) * G.v[r]);
+  }
+  __syncthreads();
+  int cur = 0, lev = 0;
+  for (int s = 1; s < M; s *= 2, lev++) {
+    if (own) {
+      cxd acc = get(g_[cur][p][r]);
+      if (p - s >= 1) {
+        const cxd* a = F.ag(lev, p, 0) + 4 * r;
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc += a[j] * get(g_[cur][p - s][j]);
+      }
+      if (p + s <= M) {
+        const cxd* a = F.ag(lev, p, 1) + 4 * r;
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc += a[j] * get(g_[cur][p + s][j]);
+      }
+      set(g_[cur ^ 1][p][r], acc);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (own) {
+    const cxd* di = F.dinv + (long)p * 16 + 4 * r;
+    cxd acc(0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc += di[j] * get(g_[cur][p][j]);
+    set(xs_[p][r], acc);
+  }
+  if (tid < 4) {
+    set(xs_[0][tid], cxd(0, 0));
+    set(xs_[N][tid], cxd(0, 0));
+  }
+  __syncthreads();
+  typename cvec<T>::type* __restrict__ u = J.u;
+  if (own) {
+    const long o = (r == 0 ? G.ob0 : r == 1 ? G.ob1 : r == 2 ? G.oc0 : G.oc1) + (long)p * G.sa;
+    st<T>(u, o, ld<T>(u, o) + get(xs_[p][r]));
+  }
+  for (int e = tid; e <= M; e += blockDim.x) {
+    cxd dv(0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dv += (get(xs_[e + 1][i]) - get(xs_[e][i])) * G.v[i];
+    const long o = G.oa + (long)e * G.sa;
+    st<T>(u, o, ld<T>(u, o) + (get(qs_[e]) + dv * F.rd[e]));
+  }
+}
+
 // z = W^{-1} y with the packed symmetric inverse held in values 0..19
 template <typename R, int NC>
 __device__ __forceinline__ void symv(const Ch<R> (&w)[NC], const cxd (&y)[4], cxd (&z)[4]) {
@@ -1327,7 +1591,10 @@ static size_t chunks_of(int values, size_t esize) {
 size_t line_factor_bytes(const int* n, int a, int colour, size_t esize) {
   int nlo, nhi;
   line_colour_counts(n, a, colour, &nlo, &nhi);
-  return (size_t)nlo * nhi * (size_t)n[a] * chunks_of(FREC, esize) * 16;
+  const size_t twisted = (size_t)nlo * nhi * (size_t)n[a] * chunks_of(FREC, esize) * 16;
+  if ((long)nlo * nhi == 1 && n[a] >= 2 && n[a] <= PCR_MAX_NODES)
+    return std::max(twisted, (size_t)pcr_entries(n[a]) * sizeof(cxd));
+  return twisted;
 }
 
 size_t line_scratch_bytes(const int* n, int a, int colour, size_t esize) {
@@ -1365,6 +1632,24 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     solve_bytes += (double)nl * na * (SREC * sizeof(ScrRec<T>) + FREC * sizeof(FacRec<T>)) + rows * 2.0 * es;
   }
   if (maxl == 0 || P.njobs == 0) return;
+  int maxna = 0;
+  for (int j = 0; j < P.njobs; j++) maxna = std::max(maxna, P.job[j].L.n[axis]);
+  if (maxl == 1 && maxna <= PCR_MAX_NODES) {
+    const dim3 g(1, P.njobs);
+    if (factor) {
+      timing::Scope ts(timing::OTHER, s, 0);
+      if (axis == 0) hipLaunchKernelGGL((k_line_pcr_factor<T, 0>), g, dim3(PCR_MAX_NODES), 0, s, P);
+      else if (axis == 1) hipLaunchKernelGGL((k_line_pcr_factor<T, 1>), g, dim3(PCR_MAX_NODES), 0, s, P);
+      else hipLaunchKernelGGL((k_line_pcr_factor<T, 2>), g, dim3(PCR_MAX_NODES), 0, s, P);
+      return;
+    }
+    const double by = rhs_bytes + solve_bytes;
+    const dim3 b(4 * PCR_MAX_NODES);
+    if (axis == 0) launch_timed(k_line_pcr_solve<T, 0>, g, b, s, timing::LINE, by, P);
+    else if (axis == 1) launch_timed(k_line_pcr_solve<T, 1>, g, b, s, timing::LINE, by, P);
+    else launch_timed(k_line_pcr_solve<T, 2>, g, b, s, timing::LINE, by, P);
+    return;
+  }
   if (factor) {
     const dim3 g((maxl + 63) / 64, P.njobs), b(64);
     timing::Scope ts(timing::OTHER, s, 0);
