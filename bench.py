@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 50.8 MB fetched + 12.6 MB written per
 # launch against 33.0 MB algorithmic.  The two passes of the substitution read
 # the factors twice and write and re-read z once.
-LINE_SOLVE_TRAFFIC_RATIO = (50.34e6 + 10.37e6) / 31.64e6
+LINE_SOLVE_TRAFFIC_RATIO = (50.38e6 + 10.38e6) / 31.64e6
 
 
 def parse():
