@@ -695,7 +695,12 @@ __global__ void __launch_bounds__(256, EMMI_LINE_RHS_MIN_WAVES) k_line_rhs_x(Lin
   const Level<T>& L = J.L;
   const long nl = (long)J.nlo * J.nhi;
   const int N = L.n[A];
-  const int lb = blockIdx.x % nlb, pbk = blockIdx.x / nlb;
+  // XCD-aware order: each XCD takes a contiguous range of tiles (a slab of
+  // nodes over all lines), so the rows that neighbouring line tiles share
+  // are fetched into one L2 (in dispatch order they land on different XCDs
+  // and the level-0 launch fetched 2.4x its bytes)
+  const long bl = xcd_block(blockIdx.x, gridDim.x);
+  const int lb = (int)(bl % nlb), pbk = (int)(bl / nlb);
   const long l0 = (long)lb * XT;
   const int p0 = pbk * XT;
   if (l0 >= nl || p0 >= N) return;   // uniform over the block
