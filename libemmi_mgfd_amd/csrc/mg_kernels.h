@@ -13,8 +13,33 @@ struct Transfer {
   const T* wr[3];
 };
 
+// The per-slot vectors of one launch over several right-hand sides that share
+// a level (grid y = slot): in0/in1 are read, out0/out1 written, omu is the
+// slot's omega mu0.
+constexpr int MG_BATCH_MAX = 8;
+template <typename T>
+struct SlotVecs {
+  const typename cvec<T>::type* in0[MG_BATCH_MAX];
+  const typename cvec<T>::type* in1[MG_BATCH_MAX];
+  typename cvec<T>::type* out0[MG_BATCH_MAX];
+  typename cvec<T>::type* out1[MG_BATCH_MAX];
+  double omu[MG_BATCH_MAX];
+};
+
 template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s);
+// r_k = f_k - A(omu_k) u_k for n slots of one level, one launch
+template <typename T>
+void launch_residual_batch(const Level<T>& L, int n, const void* const* u, const void* const* f, void* const* r,
+                           const double* omu, hipStream_t s);
+// fc_k = R rf_k, uc_k zeroed, for n slots, one launch
+template <typename T>
+void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,
+                           void* const* fc, void* const* uc, hipStream_t s);
+// uf_k += P uc_k for n slots, one launch
+template <typename T>
+void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* uc,
+                          void* const* uf, hipStream_t s);
 int residual_blocks(long nc);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 template <typename T>

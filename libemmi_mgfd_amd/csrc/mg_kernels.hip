@@ -14,6 +14,8 @@
 // the reference's m/(d d) coefficient form with invmur*vol = W_a W_b W_c.
 // Only interior edges carry an equation; boundary-tangential edges are PEC
 // zeros and are never written.
+#include <stdexcept>
+
 #include "emmi_dev.h"
 #include "mg_kernels.h"
 #include "stencil.h"
@@ -38,9 +40,12 @@ __device__ __forceinline__ void block_sum_store(double v, double* partial, long 
 
 // r = f - A u on interior edges, r = f elsewhere; optional |r|^2 partials
 template <typename T>
-__global__ void __launch_bounds__(256) k_residual(Level<T> L, const typename cvec<T>::type* __restrict__ u,
-                                                  const typename cvec<T>::type* __restrict__ f,
-                                                  typename cvec<T>::type* __restrict__ r, double* partial) {
+__global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, double* partial) {
+  const int sl = blockIdx.y;
+  L.omu = V.omu[sl];
+  const typename cvec<T>::type* __restrict__ u = V.in0[sl];
+  const typename cvec<T>::type* __restrict__ f = V.in1[sl];
+  typename cvec<T>::type* __restrict__ r = V.out0[sl];
   const long npts = L.nc;
   const long blk = xcd_block(blockIdx.x, gridDim.x);
   const long t = blk * blockDim.x + threadIdx.x;
@@ -89,10 +94,10 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // uc, when given, is the coarse correction, zeroed at the same indices (the
 // coarse solve starts from zero)
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X,
-                                                  const typename cvec<T>::type* __restrict__ rf,
-                                                  typename cvec<T>::type* __restrict__ fc,
-                                                  typename cvec<T>::type* __restrict__ uc) {
+__global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X, SlotVecs<T> V) {
+  const typename cvec<T>::type* __restrict__ rf = V.in0[blockIdx.y];
+  typename cvec<T>::type* __restrict__ fc = V.out0[blockIdx.y];
+  typename cvec<T>::type* __restrict__ uc = V.out1[blockIdx.y];
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= Cl.nc) return;
   int P[3];
@@ -135,9 +140,9 @@ __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Trans
 
 // prolongation: u_f += P u_c on interior fine edges
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong(Level<T> F, Level<T> Cl, Transfer<T> X,
-                                                 const typename cvec<T>::type* __restrict__ uc,
-                                                 typename cvec<T>::type* __restrict__ uf) {
+__global__ void __launch_bounds__(256) k_prolong(Level<T> F, Level<T> Cl, Transfer<T> X, SlotVecs<T> V) {
+  const typename cvec<T>::type* __restrict__ uc = V.in0[blockIdx.y];
+  typename cvec<T>::type* __restrict__ uf = V.out0[blockIdx.y];
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= F.nc) return;
   int p[3];
@@ -288,11 +293,33 @@ __global__ void k_edge_sigma(int n1, int n2, int n3, const double* __restrict__ 
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T>
+void launch_residual_batch(const Level<T>& L, int n, const void* const* u, const void* const* f, void* const* r,
+                           const double* omu, hipStream_t s) {
+  using V = typename cvec<T>::type;
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("residual batch of 1..8 slots");
+  SlotVecs<T> B = {};
+  for (int k = 0; k < n; k++) {
+    B.in0[k] = (const V*)u[k];
+    B.in1[k] = (const V*)f[k];
+    B.out0[k] = (V*)r[k];
+    B.omu[k] = omu[k];
+  }
+  const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
+  timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (3 * sz + sr));
+  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256), n), dim3(256), 0, s, L, B, (double*)nullptr);
+}
+
+template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s) {
   using V = typename cvec<T>::type;
+  SlotVecs<T> B = {};
+  B.in0[0] = (const V*)u;
+  B.in1[0] = (const V*)f;
+  B.out0[0] = (V*)r;
+  B.omu[0] = L.omu;
   const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
   timing::Scope ts(timing::RESIDUAL, s, 3.0 * L.nc * (3 * sz + sr));
-  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, (const V*)u, (const V*)f, (V*)r, partial);
+  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, B, partial);
 }
 
 int residual_blocks(long nc) { return (int)nblk(nc, 256); }
@@ -302,18 +329,44 @@ void launch_sum_partials(const double* partial, int n, double* out, hipStream_t 
 }
 
 template <typename T>
+void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,
+                           void* const* fc, void* const* uc, hipStream_t s) {
+  using V = typename cvec<T>::type;
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("restriction batch of 1..8 slots");
+  SlotVecs<T> B = {};
+  for (int k = 0; k < n; k++) {
+    B.in0[k] = (const V*)rf[k];
+    B.out0[k] = (V*)fc[k];
+    B.out1[k] = uc ? (V*)uc[k] : nullptr;
+  }
+  timing::Scope ts(timing::RESTRICT, s, n * 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256), n), dim3(256), 0, s, F, C, X, B);
+}
+
+template <typename T>
 void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, void* uc,
                      hipStream_t s) {
+  void* ucs[1] = {uc};
+  launch_restrict_batch<T>(F, C, X, 1, &rf, &fc, ucs, s);
+}
+
+template <typename T>
+void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* uc,
+                          void* const* uf, hipStream_t s) {
   using V = typename cvec<T>::type;
-  timing::Scope ts(timing::RESTRICT, s, 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)rf, (V*)fc, (V*)uc);
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("prolongation batch of 1..8 slots");
+  SlotVecs<T> B = {};
+  for (int k = 0; k < n; k++) {
+    B.in0[k] = (const V*)uc[k];
+    B.out0[k] = (V*)uf[k];
+  }
+  timing::Scope ts(timing::PROLONG, s, n * 3.0 * (2.0 * F.nc + C.nc) * sizeof(typename cvec<T>::type));
+  hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(F.nc, 256), n), dim3(256), 0, s, F, C, X, B);
 }
 
 template <typename T>
 void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s) {
-  using V = typename cvec<T>::type;
-  timing::Scope ts(timing::PROLONG, s, 3.0 * (2.0 * F.nc + C.nc) * sizeof(typename cvec<T>::type));
-  hipLaunchKernelGGL(k_prolong<T>, dim3(nblk(F.nc, 256)), dim3(256), 0, s, F, C, X, (const V*)uc, (V*)uf);
+  launch_prolong_batch<T>(F, C, X, 1, &uc, &uf, s);
 }
 
 template <typename T>
@@ -359,6 +412,12 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 
 #define EMMI_INSTANTIATE(T)                                                                                    \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
+  template void launch_residual_batch<T>(const Level<T>&, int, const void* const*, const void* const*,         \
+                                         void* const*, const double*, hipStream_t);                            \
+  template void launch_restrict_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, int,            \
+                                         const void* const*, void* const*, void* const*, hipStream_t);         \
+  template void launch_prolong_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, int,             \
+                                        const void* const*, void* const*, hipStream_t);                        \
   template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, void*, \
                                    hipStream_t);                                                               \
   template void launch_prolong<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*,  \

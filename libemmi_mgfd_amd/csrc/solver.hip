@@ -491,29 +491,55 @@ template <typename T>
 bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess, int kind) {
   const DevLevel<T>& L = hier<T>().pat[pat][lev];
   const DevLevel<T>& C = hier<T>().pat[pat][lev + 1];
-  bool from_f = false;
+  bool from_f = false, r0_was_ready = false;
   if (lev == 0 && norm_here) {
     // level-0 residual with its norm (FP64 direct form: r64 is the level's r)
     if (check_active(pat)) return true;
   } else if (lev == 0 && r0_ready_) {
     // the cycle's convergence test has just computed r[0] = f - A u
     r0_ready_ = false;
+    r0_was_ready = true;
   } else if (zero_guess) {
     from_f = true;
-  } else {
-    for (int k : act_) launch_residual<T>(L.view(slots_[k].omu), slots_[k].u[lev], slots_[k].f[lev], slots_[k].r[lev], nullptr, s_);
   }
-  for (int k : act_) {
-    Slot& S = slots_[k];
-    launch_restrict<T>(L.view(S.omu), C.view(S.omu), C.transfer(), from_f ? S.f[lev] : S.r[lev], S.f[lev + 1],
-                       S.u[lev + 1], s_);
+  // the active slots share the level: one launch per step for up to
+  // MG_BATCH_MAX of them (grid y = slot)
+  const bool resid = !(lev == 0 && (norm_here || r0_was_ready)) && !from_f;
+  for (size_t i0 = 0; i0 < act_.size(); i0 += MG_BATCH_MAX) {
+    const int n = (int)std::min<size_t>(MG_BATCH_MAX, act_.size() - i0);
+    const void* u[MG_BATCH_MAX];
+    const void* f[MG_BATCH_MAX];
+    const void* rf[MG_BATCH_MAX];
+    void* r[MG_BATCH_MAX];
+    void* fc[MG_BATCH_MAX];
+    void* uc[MG_BATCH_MAX];
+    double omu[MG_BATCH_MAX];
+    for (int j = 0; j < n; j++) {
+      Slot& S = slots_[act_[i0 + j]];
+      u[j] = S.u[lev];
+      f[j] = S.f[lev];
+      r[j] = S.r[lev];
+      rf[j] = from_f ? S.f[lev] : S.r[lev];
+      fc[j] = S.f[lev + 1];
+      uc[j] = S.u[lev + 1];
+      omu[j] = S.omu;
+    }
+    if (resid) launch_residual_batch<T>(L.view(0.0), n, u, f, r, omu, s_);
+    launch_restrict_batch<T>(L.view(0.0), C.view(0.0), C.transfer(), n, rf, fc, uc, s_);
   }
   if (kind == 2) f_cycle<T>(pat, lev + 1, false, true);
   else if (kind == 3) { w_cycle<T>(pat, lev + 1, false, true); w_cycle<T>(pat, lev + 1, false, false); }
   else v_cycle<T>(pat, lev + 1, false, true);
-  for (int k : act_) {
-    Slot& S = slots_[k];
-    launch_prolong<T>(L.view(S.omu), C.view(S.omu), C.transfer(), S.u[lev + 1], S.u[lev], s_);
+  for (size_t i0 = 0; i0 < act_.size(); i0 += MG_BATCH_MAX) {
+    const int n = (int)std::min<size_t>(MG_BATCH_MAX, act_.size() - i0);
+    const void* uc[MG_BATCH_MAX];
+    void* uf[MG_BATCH_MAX];
+    for (int j = 0; j < n; j++) {
+      Slot& S = slots_[act_[i0 + j]];
+      uc[j] = S.u[lev + 1];
+      uf[j] = S.u[lev];
+    }
+    launch_prolong_batch<T>(L.view(0.0), C.view(0.0), C.transfer(), n, uc, uf, s_);
   }
   return false;
 }
