@@ -33,14 +33,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s at 128^3 to rel-res 1e-6: plain FP64 vs mixed+batched+learned x0"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_line_solve launch over its algorithmic bytes, measured on
+# HBM bytes per line-substitution launch (k_line_solve, k_line_solve_x_apply,
+# k_line_solve_q, k_line_pcr_solve) over its algorithmic bytes, measured on
 # the mixed+batched 128^3 step with two rocprofv3 --pmc passes (FETCH_SIZE,
-# then WRITE_SIZE, kernels k_line_solve and the runtime's copy kernel), each
-# scaled by the same counter on the 103 MB device-to-device copies of that run
-# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 50.8 MB fetched + 12.6 MB written per
-# launch against 33.0 MB algorithmic.  The two passes of the substitution read
-# the factors twice and write and re-read z once.
-LINE_SOLVE_TRAFFIC_RATIO = (50.38e6 + 10.38e6) / 31.64e6
+# then WRITE_SIZE, those kernels and the runtime's copy kernel), each scaled
+# by the same counter on the 103 MB device-to-device copies of that run
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 50.40 MB fetched + 10.39 MB written
+# per launch against 31.64 MB algorithmic.  The two passes of the substitution
+# read the factors twice and write and re-read z once.
+LINE_SOLVE_TRAFFIC_RATIO = (50.40e6 + 10.39e6) / 31.64e6
 
 
 def parse():
