@@ -1231,8 +1231,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
 template <int CTRL>
 __device__ __forceinline__ double qdpp(double x) {
   const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 // sum over the quad (every lane gets the same value)
@@ -1326,8 +1326,11 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   const int sel = (2 * q) % VS;
   const long off_q = q == 0 ? G.ob0 : q == 1 ? G.ob1 : q == 2 ? G.oc0 : G.oc1;
 
-  QNode<RS> ring[D];
-  T riw[D];
+  // D nodes ahead are in flight: the load for node p + D goes into the slot
+  // node p - 1 used, so a slot is never copied before it is refilled
+  constexpr int R = D + 1;
+  QNode<RS> ring[R];
+  T riw[R];
   cxd tm(0, 0);   // component q of this half's coupling term at the meeting node
   auto cw = [](const QNode<RS>& n, cxd (&w)[4]) {
 #pragma unroll
@@ -1348,17 +1351,17 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     cxd rd_prev = rd0;
     cxd q_prev = cxd((double)r0.x, (double)r0.y) * rd0;
     cxd z(0, 0);
-    for (int p0 = 1; p0 <= last; p0 += D) {
+    for (int p0 = 1; p0 <= last; p0 += R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 + d;
         if (p <= last) {
-          const QNode<RS> nd = ring[d];
+          const QNode<RS>& nd = ring[d];
           const double iWe = (double)riw[d];
           {
-            const int pn = min(p + D, last);
-            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[d] = iw[pn - 1];
+            const int pn = min(p + D, last), dn = (d + D) % R;
+            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[dn] = iw[pn - 1];
           }
           cxd w[4];
           cw(nd, w);
@@ -1385,7 +1388,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
   } else if (live) {
     const int last = m + 1;
-    cxd rdn[D], qn[D];   // 1/d and q of the node above each ring entry
+    cxd rdn[R], qn[R];   // 1/d and q of the node above each ring entry
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = min(max(M - d, last), M);
@@ -1397,22 +1400,22 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       qn[d] = cxd((double)r.x, (double)r.y);
     }
     cxd zn(0, 0), rd_last(0, 0);
-    for (int p0 = M; p0 >= last; p0 -= D) {
+    for (int p0 = M; p0 >= last; p0 -= R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 - d;
         if (p >= last) {
-          const QNode<RS> nd = ring[d];
+          const QNode<RS>& nd = ring[d];
           const double iWe = (double)riw[d];
           const cxd rdm = rdn[d], rhom = qn[d];
           {
-            const int pn = max(p - D, last);
-            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[d] = iw[pn];
+            const int pn = max(p - D, last), dn = (d + D) % R;
+            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[dn] = iw[pn];
             const Ch<double> c = fac[(pn - 1) * fstep + 10 * stride];
             const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
-            rdn[d] = cxd(c.v[0], c.v[1]);
-            qn[d] = cxd((double)r.x, (double)r.y);
+            rdn[dn] = cxd(c.v[0], c.v[1]);
+            qn[dn] = cxd((double)r.x, (double)r.y);
           }
           cxd w[4];
           cw(nd, w);
@@ -1465,7 +1468,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   }
 
   // ---------------- back substitution ----------------
-  E ru[D], ra[D];
+  E ru[R], ra[R];
   if (side == 0) {
     {
       const long o = off_q + (long)m * G.sa;
@@ -1481,20 +1484,20 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       ra[d] = u[G.oa + (long)pn * G.sa];
     }
     cxd an = xm;   // component q of x_{p+1}
-    for (int p0 = last; p0 >= 1; p0 -= D) {
+    for (int p0 = last; p0 >= 1; p0 -= R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 - d;
         if (p >= 1) {
-          const QNode<RS> nd = ring[d];
+          const QNode<RS>& nd = ring[d];
           const double iWe = (double)riw[d];
           const E cu = ru[d], ca = ra[d];
           {
-            const int pn = max(p - D, 1);
-            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[d] = iw[pn];
-            ru[d] = u[off_q + (long)pn * G.sa];
-            ra[d] = u[G.oa + (long)pn * G.sa];
+            const int pn = max(p - D, 1), dn = (d + D) % R;
+            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[dn] = iw[pn];
+            ru[dn] = u[off_q + (long)pn * G.sa];
+            ra[dn] = u[G.oa + (long)pn * G.sa];
           }
           cxd w[4];
           cw(nd, w);
@@ -1529,20 +1532,20 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
     cxd xp = xm;   // component q of x_{p-1}
     cxd rdm = rdmid, qm = qmid;
-    for (int p0 = last; p0 <= M; p0 += D) {
+    for (int p0 = last; p0 <= M; p0 += R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 + d;
         if (p <= M) {
-          const QNode<RS> nd = ring[d];
+          const QNode<RS>& nd = ring[d];
           const double iWe = (double)riw[d];
           const E cu = ru[d], ca = ra[d];
           {
-            const int pn = min(p + D, M);
-            qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[d] = iw[pn - 1];
-            ru[d] = u[off_q + (long)pn * G.sa];
-            ra[d] = u[G.oa + (long)(pn - 1) * G.sa];
+            const int pn = min(p + D, M), dn = (d + D) % R;
+            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+            riw[dn] = iw[pn - 1];
+            ru[dn] = u[off_q + (long)pn * G.sa];
+            ra[dn] = u[G.oa + (long)(pn - 1) * G.sa];
           }
           cxd w[4];
           cw(nd, w);
