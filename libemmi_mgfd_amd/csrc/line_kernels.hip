@@ -897,9 +897,12 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   const long fstep = (long)FC * stride, sstep = (long)SC * stride;
   const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
 
-  Ch<RF> rf[D][FC];
-  Ch<RS> rs[D][SC];
-  T riw[D];
+  // D nodes ahead are in flight: the load for node p + D goes into the slot
+  // node p - 1 used, so a slot is never copied before it is refilled
+  constexpr int R = D + 1;
+  Ch<RF> rf[R][FC];
+  Ch<RS> rs[R][SC];
+  T riw[R];
   cxd tm[4];   // this half's coupling term at the meeting node
 #pragma unroll
   for (int i = 0; i < 4; i++) tm[i] = cxd(0, 0);
@@ -919,23 +922,19 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     cxd rd_prev((double)f0.v[20 % VF], (double)f0.v[21 % VF]);
     cxd q_prev = cxd((double)s0.v[8 % VS], (double)s0.v[9 % VS]) * rd_prev;
     cxd z[4];
-    for (int p0 = 1; p0 <= last; p0 += D) {
+    for (int p0 = 1; p0 <= last; p0 += R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 + d;
         if (p <= last) {
-          Ch<RF> cf[FC];
-          Ch<RS> cr[SC];
-#pragma unroll
-          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
-#pragma unroll
-          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
+          const Ch<RF>(&cf)[FC] = rf[d];
+          const Ch<RS>(&cr)[SC] = rs[d];
           const double iWe = (double)riw[d];
           {
-            const int pn = min(p + D, last);
-            load_node(rf[d], fac + pn * fstep, stride);
-            load_node(rs[d], scr + pn * sstep, stride);
-            riw[d] = iw[pn - 1];
+            const int pn = min(p + D, last), dn = (d + D) % R;
+            load_node(rf[dn], fac + pn * fstep, stride);
+            load_node(rs[dn], scr + pn * sstep, stride);
+            riw[dn] = iw[pn - 1];
           }
           const cxd rd = cval(cf, 20);
           const cxd q = cval(cr, 8) * rd;
@@ -964,8 +963,8 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     if (m >= 2) bmul(G, z, (double)iw[m - 1], rd_prev, tm);   // B_m z_{m-1}
   } else if (live) {
     // ring entries also carry rho_{p-1} and 1/d_{p-1} of the node above
-    Ch<RF> rdn[D];
-    Ch<RS> rqn[D];
+    Ch<RF> rdn[R];
+    Ch<RS> rqn[R];
     const int last = m + 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -977,27 +976,23 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       riw[d] = iw[pn];
     }
     cxd zn[4], rd_last(0, 0);
-    for (int p0 = M; p0 >= last; p0 -= D) {
+    for (int p0 = M; p0 >= last; p0 -= R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 - d;
         if (p >= last) {
-          Ch<RF> cf[FC];
-          Ch<RS> cr[SC];
-#pragma unroll
-          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
-#pragma unroll
-          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
-          const Ch<RF> cdm = rdn[d];
-          const Ch<RS> cqm = rqn[d];
+          const Ch<RF>(&cf)[FC] = rf[d];
+          const Ch<RS>(&cr)[SC] = rs[d];
+          const Ch<RF>& cdm = rdn[d];
+          const Ch<RS>& cqm = rqn[d];
           const double iWe = (double)riw[d];
           {
-            const int pn = max(p - D, last);
-            load_node(rf[d], fac + pn * fstep, stride);
-            load_node(rs[d], scr + pn * sstep, stride);
-            rdn[d] = fac[(pn - 1) * fstep + CD * stride];
-            rqn[d] = scr[(pn - 1) * sstep + CQ * stride];
-            riw[d] = iw[pn];
+            const int pn = max(p - D, last), dn = (d + D) % R;
+            load_node(rf[dn], fac + pn * fstep, stride);
+            load_node(rs[dn], scr + pn * sstep, stride);
+            rdn[dn] = fac[(pn - 1) * fstep + CD * stride];
+            rqn[dn] = scr[(pn - 1) * sstep + CQ * stride];
+            riw[dn] = iw[pn];
           }
           const cxd rd = cval(cf, 20);
           const cxd q = cval(cr, 8) * rd;
@@ -1058,7 +1053,7 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   }
 
   // ---------------- back substitution ----------------
-  E ru[D][5];
+  E ru[R][5];
   if (side == 0) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -1080,31 +1075,23 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     cxd an[4];   // x_{p+1}
 #pragma unroll
     for (int i = 0; i < 4; i++) an[i] = xm[i];
-    for (int p0 = last; p0 >= 1; p0 -= D) {
+    for (int p0 = last; p0 >= 1; p0 -= R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 - d;
         if (p >= 1) {
-          Ch<RF> cf[FC];
-          Ch<RS> cr[SC];
-          E cu[5];
-#pragma unroll
-          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
-#pragma unroll
-          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
-          if constexpr (A != 0) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
-          }
+          const Ch<RF>(&cf)[FC] = rf[d];
+          const Ch<RS>(&cr)[SC] = rs[d];
+          const E(&cu)[5] = ru[d];
           const double iWe = (double)riw[d];
           {
-            const int pn = max(p - D, 1);
-            load_node(rf[d], fac + pn * fstep, stride);
-            load_node(rs[d], scr + pn * sstep, stride);
-            riw[d] = iw[pn];
+            const int pn = max(p - D, 1), dn = (d + D) % R;
+            load_node(rf[dn], fac + pn * fstep, stride);
+            load_node(rs[dn], scr + pn * sstep, stride);
+            riw[dn] = iw[pn];
             if constexpr (A != 0) {
 #pragma unroll
-              for (int k = 0; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+              for (int k = 0; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
             }
           }
           const cxd rd = cval(cf, 20);
@@ -1158,32 +1145,24 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
 #pragma unroll
     for (int i = 0; i < 4; i++) xp[i] = xm[i];
     cxd rdm = rdmid, qm = qmid;   // 1/d and q of edge p-1
-    for (int p0 = last; p0 <= M; p0 += D) {
+    for (int p0 = last; p0 <= M; p0 += R) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
+      for (int d = 0; d < R; d++) {
         const int p = p0 + d;
         if (p <= M) {
-          Ch<RF> cf[FC];
-          Ch<RS> cr[SC];
-          E cu[5];
-#pragma unroll
-          for (int k = 0; k < FC; k++) cf[k] = rf[d][k];
-#pragma unroll
-          for (int k = 0; k < SC; k++) cr[k] = rs[d][k];
-          if constexpr (A != 0) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) cu[k] = ru[d][k];
-          }
+          const Ch<RF>(&cf)[FC] = rf[d];
+          const Ch<RS>(&cr)[SC] = rs[d];
+          const E(&cu)[5] = ru[d];
           const double iWe = (double)riw[d];
           {
-            const int pn = min(p + D, M);
-            load_node(rf[d], fac + pn * fstep, stride);
-            load_node(rs[d], scr + pn * sstep, stride);
-            riw[d] = iw[pn - 1];
+            const int pn = min(p + D, M), dn = (d + D) % R;
+            load_node(rf[dn], fac + pn * fstep, stride);
+            load_node(rs[dn], scr + pn * sstep, stride);
+            riw[dn] = iw[pn - 1];
             if constexpr (A != 0) {
-              ru[d][0] = u[G.oa + (long)(pn - 1) * G.sa];
+              ru[dn][0] = u[G.oa + (long)(pn - 1) * G.sa];
 #pragma unroll
-              for (int k = 1; k < 5; k++) ru[d][k] = u[off_u[k] + (long)pn * G.sa];
+              for (int k = 1; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
             }
           }
           const cxd rd = cval(cf, 20);
