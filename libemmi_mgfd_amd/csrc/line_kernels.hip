@@ -1258,11 +1258,16 @@ __device__ __forceinline__ void qload(QNode<RS>& n, const Ch<double>* __restrict
   n.rho = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + (8 / VS) * stride) + 8 % VS);
 }
 
+// NT > 0: every job of the launch has lines of NT nodes, and the passes are
+// unrolled completely.  Loop-carried ring slots make the register allocator
+// load into temporaries and copy them into the slots, and each copy waits for
+// its load: the rolled loop keeps only about one node in flight.
+//
 // With RHS the block first computes the records of its own 16 lines (the
 // work of k_line_rhs, for launches too small to pay for a launch of their
 // own) with all its BT threads, and then its first two waves substitute;
 // the other waves only help with the right-hand sides.
-template <typename T, int A, int D, bool RHS, int BT>
+template <typename T, int A, int D, bool RHS, int BT, int NT>
 __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   using E = typename cvec<T>::type;
   using RS = ScrRec<T>;
@@ -1271,7 +1276,8 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   __shared__ double xch[2][2][64];
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
-  const int N = L.n[A];
+  const int N = NT > 0 ? NT : L.n[A];
+  constexpr int UNR = NT > 0 ? 64 : 1;   // the passes' unroll count
   if (N < 2) return;   // uniform over the block: no interior node
   const int M = N - 1, m = N / 2;
   if (RHS) {
@@ -1330,6 +1336,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     cxd rd_prev = rd0;
     cxd q_prev = cxd((double)r0.x, (double)r0.y) * rd0;
     cxd z(0, 0);
+#pragma unroll UNR
     for (int p0 = 1; p0 <= last; p0 += R) {
 #pragma unroll
       for (int d = 0; d < R; d++) {
@@ -1367,18 +1374,20 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
   } else if (live) {
     const int last = m + 1;
-    cxd rdn[R], qn[R];   // 1/d and q of the node above each ring entry
+    // 1/d and rho of the node above each ring entry, kept as loaded: a
+    // conversion at load time would wait for the load
+    Ch<double> rdn[R];
+    PS qn[R];
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = min(max(M - d, last), M);
       qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
       riw[d] = iw[pn];
-      const Ch<double> c = fac[(pn - 1) * fstep + 10 * stride];
-      const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
-      rdn[d] = cxd(c.v[0], c.v[1]);
-      qn[d] = cxd((double)r.x, (double)r.y);
+      rdn[d] = fac[(pn - 1) * fstep + 10 * stride];
+      qn[d] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
     }
     cxd zn(0, 0), rd_last(0, 0);
+#pragma unroll UNR
     for (int p0 = M; p0 >= last; p0 -= R) {
 #pragma unroll
       for (int d = 0; d < R; d++) {
@@ -1386,15 +1395,13 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
         if (p >= last) {
           const QNode<RS>& nd = ring[d];
           const double iWe = (double)riw[d];
-          const cxd rdm = rdn[d], rhom = qn[d];
+          const cxd rdm(rdn[d].v[0], rdn[d].v[1]), rhom((double)qn[d].x, (double)qn[d].y);
           {
             const int pn = max(p - D, last), dn = (d + D) % R;
             qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
             riw[dn] = iw[pn];
-            const Ch<double> c = fac[(pn - 1) * fstep + 10 * stride];
-            const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
-            rdn[dn] = cxd(c.v[0], c.v[1]);
-            qn[dn] = cxd((double)r.x, (double)r.y);
+            rdn[dn] = fac[(pn - 1) * fstep + 10 * stride];
+            qn[dn] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
           }
           cxd w[4];
           cw(nd, w);
@@ -1463,6 +1470,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       ra[d] = u[G.oa + (long)pn * G.sa];
     }
     cxd an = xm;   // component q of x_{p+1}
+#pragma unroll UNR
     for (int p0 = last; p0 >= 1; p0 -= R) {
 #pragma unroll
       for (int d = 0; d < R; d++) {
@@ -1511,6 +1519,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
     cxd xp = xm;   // component q of x_{p-1}
     cxd rdm = rdmid, qm = qmid;
+#pragma unroll UNR
     for (int p0 = last; p0 <= M; p0 += R) {
 #pragma unroll
       for (int d = 0; d < R; d++) {
@@ -1600,6 +1609,14 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, doubl
   else hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
 }
 
+template <typename T, int NT>
+static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, dim3 b, hipStream_t s, double by) {
+  constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
+  if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
+  else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
+  else launch_timed(k_line_solve_q<T, 2, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
+}
+
 template <typename T>
 static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
   int maxl = 0;
@@ -1667,14 +1684,23 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     if (fuse) {
       const double by = rhs_bytes + solve_bytes;
       const dim3 b(FT);
-      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT>, g, b, s, timing::LINE, by, P);
-      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT>, g, b, s, timing::LINE, by, P);
-      else launch_timed(k_line_solve_q<T, 2, QD, true, FT>, g, b, s, timing::LINE, by, P);
+      // a common line length with an unrolled instance, else the rolled loop
+      int nt = P.job[0].L.n[axis];
+      for (int j = 1; j < P.njobs; j++)
+        if (P.job[j].L.n[axis] != nt) nt = 0;
+      switch (nt) {
+        case 4: launch_quad_fused<T, 4>(P, axis, g, b, s, by); break;
+        case 8: launch_quad_fused<T, 8>(P, axis, g, b, s, by); break;
+        case 16: launch_quad_fused<T, 16>(P, axis, g, b, s, by); break;
+        case 32: launch_quad_fused<T, 32>(P, axis, g, b, s, by); break;
+        case 64: launch_quad_fused<T, 64>(P, axis, g, b, s, by); break;
+        default: launch_quad_fused<T, 0>(P, axis, g, b, s, by); break;
+      }
     } else {
       const dim3 b(128);
-      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
-      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
-      else launch_timed(k_line_solve_q<T, 2, QD, false, 128>, g, b, s, timing::LINE, solve_bytes, P);
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
     }
   } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
