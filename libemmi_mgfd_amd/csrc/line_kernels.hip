@@ -922,44 +922,48 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     cxd rd_prev((double)f0.v[20 % VF], (double)f0.v[21 % VF]);
     cxd q_prev = cxd((double)s0.v[8 % VS], (double)s0.v[9 % VS]) * rd_prev;
     cxd z[4];
-    for (int p0 = 1; p0 <= last; p0 += R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 + d;
-        if (p <= last) {
-          const Ch<RF>(&cf)[FC] = rf[d];
-          const Ch<RS>(&cr)[SC] = rs[d];
-          const double iWe = (double)riw[d];
-          {
-            const int pn = min(p + D, last), dn = (d + D) % R;
-            load_node(rf[dn], fac + pn * fstep, stride);
-            load_node(rs[dn], scr + pn * sstep, stride);
-            riw[dn] = iw[pn - 1];
-          }
-          const cxd rd = cval(cf, 20);
-          const cxd q = cval(cr, 8) * rd;
-          const cxd dq = q_prev - q;
-          cxd g[4];
-#pragma unroll
-          for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
-          if (p >= 2) {
-            cxd t[4];
-            bmul(G, z, iWe, rd_prev, t);
-#pragma unroll
-            for (int i = 0; i < 4; i++) g[i] -= t[i];
-          }
-          symv(cf, g, z);
-          Ch<RS> zc[SC] = {};
-#pragma unroll
-          for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
-          Ch<RS>* sp = scr + p * sstep;
-#pragma unroll
-          for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
-          rd_prev = rd;
-          q_prev = q;
-        }
+    auto step = [&](const int p, const int d) {
+      const Ch<RF>(&cf)[FC] = rf[d];
+      const Ch<RS>(&cr)[SC] = rs[d];
+      const double iWe = (double)riw[d];
+      {
+        const int pn = min(p + D, last), dn = (d + D) % R;
+        load_node(rf[dn], fac + pn * fstep, stride);
+        load_node(rs[dn], scr + pn * sstep, stride);
+        riw[dn] = iw[pn - 1];
       }
+      const cxd rd = cval(cf, 20);
+      const cxd q = cval(cr, 8) * rd;
+      const cxd dq = q_prev - q;
+      cxd g[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
+      if (p >= 2) {
+        cxd t[4];
+        bmul(G, z, iWe, rd_prev, t);
+#pragma unroll
+        for (int i = 0; i < 4; i++) g[i] -= t[i];
+      }
+      symv(cf, g, z);
+      Ch<RS> zc[SC] = {};
+#pragma unroll
+      for (int i = 0; i < 4; i++) { put(zc, 2 * i, z[i].re); put(zc, 2 * i + 1, z[i].im); }
+      Ch<RS>* sp = scr + p * sstep;
+#pragma unroll
+      for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
+      rd_prev = rd;
+      q_prev = q;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = 1;
+    for (; p0 + (R - 1) <= last; p0 += R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 + d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 + d <= last) step(p0 + d, d);
     if (m >= 2) bmul(G, z, (double)iw[m - 1], rd_prev, tm);   // B_m z_{m-1}
   } else if (live) {
     // ring entries also carry rho_{p-1} and 1/d_{p-1} of the node above
@@ -976,49 +980,53 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
       riw[d] = iw[pn];
     }
     cxd zn[4], rd_last(0, 0);
-    for (int p0 = M; p0 >= last; p0 -= R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 - d;
-        if (p >= last) {
-          const Ch<RF>(&cf)[FC] = rf[d];
-          const Ch<RS>(&cr)[SC] = rs[d];
-          const Ch<RF>& cdm = rdn[d];
-          const Ch<RS>& cqm = rqn[d];
-          const double iWe = (double)riw[d];
-          {
-            const int pn = max(p - D, last), dn = (d + D) % R;
-            load_node(rf[dn], fac + pn * fstep, stride);
-            load_node(rs[dn], scr + pn * sstep, stride);
-            rdn[dn] = fac[(pn - 1) * fstep + CD * stride];
-            rqn[dn] = scr[(pn - 1) * sstep + CQ * stride];
-            riw[dn] = iw[pn];
-          }
-          const cxd rd = cval(cf, 20);
-          const cxd q = cval(cr, 8) * rd;
-          const cxd rdm((double)cdm.v[20 % VF], (double)cdm.v[21 % VF]);
-          const cxd qm = cxd((double)cqm.v[8 % VS], (double)cqm.v[9 % VS]) * rdm;
-          const cxd dq = qm - q;
-          cxd g[4];
-#pragma unroll
-          for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
-          if (p <= M - 1) {
-            cxd t[4];
-            bmul(G, zn, iWe, rd, t);
-#pragma unroll
-            for (int i = 0; i < 4; i++) g[i] -= t[i];
-          }
-          symv(cf, g, zn);
-          Ch<RS> zc[SC] = {};
-#pragma unroll
-          for (int i = 0; i < 4; i++) { put(zc, 2 * i, zn[i].re); put(zc, 2 * i + 1, zn[i].im); }
-          Ch<RS>* sp = scr + p * sstep;
-#pragma unroll
-          for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
-          rd_last = rdm;   // 1/d_{p-1}: after the loop, 1/d_m
-        }
+    auto step = [&](const int p, const int d) {
+      const Ch<RF>(&cf)[FC] = rf[d];
+      const Ch<RS>(&cr)[SC] = rs[d];
+      const Ch<RF>& cdm = rdn[d];
+      const Ch<RS>& cqm = rqn[d];
+      const double iWe = (double)riw[d];
+      {
+        const int pn = max(p - D, last), dn = (d + D) % R;
+        load_node(rf[dn], fac + pn * fstep, stride);
+        load_node(rs[dn], scr + pn * sstep, stride);
+        rdn[dn] = fac[(pn - 1) * fstep + CD * stride];
+        rqn[dn] = scr[(pn - 1) * sstep + CQ * stride];
+        riw[dn] = iw[pn];
       }
+      const cxd rd = cval(cf, 20);
+      const cxd q = cval(cr, 8) * rd;
+      const cxd rdm((double)cdm.v[20 % VF], (double)cdm.v[21 % VF]);
+      const cxd qm = cxd((double)cqm.v[8 % VS], (double)cqm.v[9 % VS]) * rdm;
+      const cxd dq = qm - q;
+      cxd g[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) g[i] = cval(cr, 2 * i) + dq * G.v[i];
+      if (p <= M - 1) {
+        cxd t[4];
+        bmul(G, zn, iWe, rd, t);
+#pragma unroll
+        for (int i = 0; i < 4; i++) g[i] -= t[i];
+      }
+      symv(cf, g, zn);
+      Ch<RS> zc[SC] = {};
+#pragma unroll
+      for (int i = 0; i < 4; i++) { put(zc, 2 * i, zn[i].re); put(zc, 2 * i + 1, zn[i].im); }
+      Ch<RS>* sp = scr + p * sstep;
+#pragma unroll
+      for (int k = 0; k < ZC; k++) sp[k * stride] = zc[k];
+      rd_last = rdm;   // 1/d_{p-1}: after the loop, 1/d_m
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = M;
+    for (; p0 - (R - 1) >= last; p0 -= R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 - d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 - d >= last) step(p0 - d, d);
     if (m + 1 <= M) bmul(G, zn, (double)iw[m], rd_last, tm);   // B_{m+1} z_{m+1}
   }
 
@@ -1075,49 +1083,53 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
     cxd an[4];   // x_{p+1}
 #pragma unroll
     for (int i = 0; i < 4; i++) an[i] = xm[i];
-    for (int p0 = last; p0 >= 1; p0 -= R) {
+    auto step = [&](const int p, const int d) {
+      const Ch<RF>(&cf)[FC] = rf[d];
+      const Ch<RS>(&cr)[SC] = rs[d];
+      const E(&cu)[5] = ru[d];
+      const double iWe = (double)riw[d];
+      {
+        const int pn = max(p - D, 1), dn = (d + D) % R;
+        load_node(rf[dn], fac + pn * fstep, stride);
+        load_node(rs[dn], scr + pn * sstep, stride);
+        riw[dn] = iw[pn];
+        if constexpr (A != 0) {
 #pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 - d;
-        if (p >= 1) {
-          const Ch<RF>(&cf)[FC] = rf[d];
-          const Ch<RS>(&cr)[SC] = rs[d];
-          const E(&cu)[5] = ru[d];
-          const double iWe = (double)riw[d];
-          {
-            const int pn = max(p - D, 1), dn = (d + D) % R;
-            load_node(rf[dn], fac + pn * fstep, stride);
-            load_node(rs[dn], scr + pn * sstep, stride);
-            riw[dn] = iw[pn];
-            if constexpr (A != 0) {
-#pragma unroll
-              for (int k = 0; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
-            }
-          }
-          const cxd rd = cval(cf, 20);
-          const cxd q = cval(cr, 8) * rd;
-          cxd t[4], wt[4], ap[4];
-          bmul(G, an, iWe, rd, t);
-          symv(cf, t, wt);
-#pragma unroll
-          for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
-          cxd dv(0, 0);
-#pragma unroll
-          for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
-          if constexpr (A == 0) {
-            put_corr(scr + p * sstep, stride, q + dv * rd, ap);
-          } else {
-            const long o = (long)p * G.sa;
-            st<T>(u, G.oa + o, cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd));
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-              st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
-          }
-#pragma unroll
-          for (int i = 0; i < 4; i++) an[i] = ap[i];
+          for (int k = 0; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
         }
       }
+      const cxd rd = cval(cf, 20);
+      const cxd q = cval(cr, 8) * rd;
+      cxd t[4], wt[4], ap[4];
+      bmul(G, an, iWe, rd, t);
+      symv(cf, t, wt);
+#pragma unroll
+      for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
+      cxd dv(0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) dv += (an[i] - ap[i]) * G.v[i];
+      if constexpr (A == 0) {
+        put_corr(scr + p * sstep, stride, q + dv * rd, ap);
+      } else {
+        const long o = (long)p * G.sa;
+        st<T>(u, G.oa + o, cxd((double)cu[0].x, (double)cu[0].y) + (q + dv * rd));
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) an[i] = ap[i];
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = last;
+    for (; p0 - (R - 1) >= 1; p0 -= R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 - d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 - d >= 1) step(p0 - d, d);
     // along-line edge 0, between the boundary node and node 1
     const Ch<RF> f0 = fac[CD * stride];
     const Ch<RS> s0 = scr[CQ * stride];
@@ -1145,52 +1157,56 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
 #pragma unroll
     for (int i = 0; i < 4; i++) xp[i] = xm[i];
     cxd rdm = rdmid, qm = qmid;   // 1/d and q of edge p-1
-    for (int p0 = last; p0 <= M; p0 += R) {
+    auto step = [&](const int p, const int d) {
+      const Ch<RF>(&cf)[FC] = rf[d];
+      const Ch<RS>(&cr)[SC] = rs[d];
+      const E(&cu)[5] = ru[d];
+      const double iWe = (double)riw[d];
+      {
+        const int pn = min(p + D, M), dn = (d + D) % R;
+        load_node(rf[dn], fac + pn * fstep, stride);
+        load_node(rs[dn], scr + pn * sstep, stride);
+        riw[dn] = iw[pn - 1];
+        if constexpr (A != 0) {
+          ru[dn][0] = u[G.oa + (long)(pn - 1) * G.sa];
 #pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 + d;
-        if (p <= M) {
-          const Ch<RF>(&cf)[FC] = rf[d];
-          const Ch<RS>(&cr)[SC] = rs[d];
-          const E(&cu)[5] = ru[d];
-          const double iWe = (double)riw[d];
-          {
-            const int pn = min(p + D, M), dn = (d + D) % R;
-            load_node(rf[dn], fac + pn * fstep, stride);
-            load_node(rs[dn], scr + pn * sstep, stride);
-            riw[dn] = iw[pn - 1];
-            if constexpr (A != 0) {
-              ru[dn][0] = u[G.oa + (long)(pn - 1) * G.sa];
-#pragma unroll
-              for (int k = 1; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
-            }
-          }
-          const cxd rd = cval(cf, 20);
-          const cxd q = cval(cr, 8) * rd;
-          cxd t[4], wt[4], ap[4];
-          bmul(G, xp, iWe, rdm, t);
-          symv(cf, t, wt);
-#pragma unroll
-          for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
-          cxd dv(0, 0);
-#pragma unroll
-          for (int i = 0; i < 4; i++) dv += (ap[i] - xp[i]) * G.v[i];
-          if constexpr (A == 0) {
-            put_corr(scr + p * sstep, stride, qm + dv * rdm, ap);
-          } else {
-            const long o = (long)p * G.sa;
-            st<T>(u, G.oa + o - G.sa, cxd((double)cu[0].x, (double)cu[0].y) + (qm + dv * rdm));
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-              st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
-          }
-#pragma unroll
-          for (int i = 0; i < 4; i++) xp[i] = ap[i];
-          rdm = rd;
-          qm = q;
+          for (int k = 1; k < 5; k++) ru[dn][k] = u[off_u[k] + (long)pn * G.sa];
         }
       }
+      const cxd rd = cval(cf, 20);
+      const cxd q = cval(cr, 8) * rd;
+      cxd t[4], wt[4], ap[4];
+      bmul(G, xp, iWe, rdm, t);
+      symv(cf, t, wt);
+#pragma unroll
+      for (int i = 0; i < 4; i++) ap[i] = cval(cr, 2 * i) - wt[i];
+      cxd dv(0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) dv += (ap[i] - xp[i]) * G.v[i];
+      if constexpr (A == 0) {
+        put_corr(scr + p * sstep, stride, qm + dv * rdm, ap);
+      } else {
+        const long o = (long)p * G.sa;
+        st<T>(u, G.oa + o - G.sa, cxd((double)cu[0].x, (double)cu[0].y) + (qm + dv * rdm));
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          st<T>(u, off_u[i + 1] + o, cxd((double)cu[i + 1].x, (double)cu[i + 1].y) + ap[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) xp[i] = ap[i];
+      rdm = rd;
+      qm = q;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = last;
+    for (; p0 + (R - 1) <= M; p0 += R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 + d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 + d <= M) step(p0 + d, d);
     // along-line edge M = N-1, between node M and the boundary node
     cxd dv(0, 0);
 #pragma unroll
@@ -1336,38 +1352,42 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     cxd rd_prev = rd0;
     cxd q_prev = cxd((double)r0.x, (double)r0.y) * rd0;
     cxd z(0, 0);
-#pragma unroll UNR
-    for (int p0 = 1; p0 <= last; p0 += R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 + d;
-        if (p <= last) {
-          const QNode<RS>& nd = ring[d];
-          const double iWe = (double)riw[d];
-          {
-            const int pn = min(p + D, last), dn = (d + D) % R;
-            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[dn] = iw[pn - 1];
-          }
-          cxd w[4];
-          cw(nd, w);
-          const cxd rd(nd.rd.x, nd.rd.y);
-          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
-          cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (q_prev - qp) * vq;
-          if (p >= 2) {
-            const cxd vz = qsum(z * vq);
-            g -= z * (cq * iWe) + (vz * rd_prev) * vq;
-          }
-          z = qsymv(w, g);
-          PS zs;
-          zs.x = (RS)z.re;
-          zs.y = (RS)z.im;
-          *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
-          rd_prev = rd;
-          q_prev = qp;
-        }
+    auto step = [&](const int p, const int d) {
+      const QNode<RS>& nd = ring[d];
+      const double iWe = (double)riw[d];
+      {
+        const int pn = min(p + D, last), dn = (d + D) % R;
+        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        riw[dn] = iw[pn - 1];
       }
+      cxd w[4];
+      cw(nd, w);
+      const cxd rd(nd.rd.x, nd.rd.y);
+      const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+      cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (q_prev - qp) * vq;
+      if (p >= 2) {
+        const cxd vz = qsum(z * vq);
+        g -= z * (cq * iWe) + (vz * rd_prev) * vq;
+      }
+      z = qsymv(w, g);
+      PS zs;
+      zs.x = (RS)z.re;
+      zs.y = (RS)z.im;
+      *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+      rd_prev = rd;
+      q_prev = qp;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = 1;
+#pragma unroll UNR
+    for (; p0 + (R - 1) <= last; p0 += R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 + d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 + d <= last) step(p0 + d, d);
     if (m >= 2) {
       const cxd vz = qsum(z * vq);
       tm = z * (cq * (double)iw[m - 1]) + (vz * rd_prev) * vq;   // B_m z_{m-1}
@@ -1387,41 +1407,45 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       qn[d] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
     }
     cxd zn(0, 0), rd_last(0, 0);
-#pragma unroll UNR
-    for (int p0 = M; p0 >= last; p0 -= R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 - d;
-        if (p >= last) {
-          const QNode<RS>& nd = ring[d];
-          const double iWe = (double)riw[d];
-          const cxd rdm(rdn[d].v[0], rdn[d].v[1]), rhom((double)qn[d].x, (double)qn[d].y);
-          {
-            const int pn = max(p - D, last), dn = (d + D) % R;
-            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[dn] = iw[pn];
-            rdn[dn] = fac[(pn - 1) * fstep + 10 * stride];
-            qn[dn] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
-          }
-          cxd w[4];
-          cw(nd, w);
-          const cxd rd(nd.rd.x, nd.rd.y);
-          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
-          const cxd qm = rhom * rdm;
-          cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (qm - qp) * vq;
-          if (p <= M - 1) {
-            const cxd vz = qsum(zn * vq);
-            g -= zn * (cq * iWe) + (vz * rd) * vq;
-          }
-          zn = qsymv(w, g);
-          PS zs;
-          zs.x = (RS)zn.re;
-          zs.y = (RS)zn.im;
-          *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
-          rd_last = rdm;
-        }
+    auto step = [&](const int p, const int d) {
+      const QNode<RS>& nd = ring[d];
+      const double iWe = (double)riw[d];
+      const cxd rdm(rdn[d].v[0], rdn[d].v[1]), rhom((double)qn[d].x, (double)qn[d].y);
+      {
+        const int pn = max(p - D, last), dn = (d + D) % R;
+        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        riw[dn] = iw[pn];
+        rdn[dn] = fac[(pn - 1) * fstep + 10 * stride];
+        qn[dn] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
       }
+      cxd w[4];
+      cw(nd, w);
+      const cxd rd(nd.rd.x, nd.rd.y);
+      const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+      const cxd qm = rhom * rdm;
+      cxd g = cxd((double)nd.s.x, (double)nd.s.y) + (qm - qp) * vq;
+      if (p <= M - 1) {
+        const cxd vz = qsum(zn * vq);
+        g -= zn * (cq * iWe) + (vz * rd) * vq;
+      }
+      zn = qsymv(w, g);
+      PS zs;
+      zs.x = (RS)zn.re;
+      zs.y = (RS)zn.im;
+      *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+      rd_last = rdm;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = M;
+#pragma unroll UNR
+    for (; p0 - (R - 1) >= last; p0 -= R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 - d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 - d >= last) step(p0 - d, d);
     if (m + 1 <= M) {
       const cxd vz = qsum(zn * vq);
       tm = zn * (cq * (double)iw[m]) + (vz * rd_last) * vq;   // B_{m+1} z_{m+1}
@@ -1470,37 +1494,41 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       ra[d] = u[G.oa + (long)pn * G.sa];
     }
     cxd an = xm;   // component q of x_{p+1}
-#pragma unroll UNR
-    for (int p0 = last; p0 >= 1; p0 -= R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 - d;
-        if (p >= 1) {
-          const QNode<RS>& nd = ring[d];
-          const double iWe = (double)riw[d];
-          const E cu = ru[d], ca = ra[d];
-          {
-            const int pn = max(p - D, 1), dn = (d + D) % R;
-            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[dn] = iw[pn];
-            ru[dn] = u[off_q + (long)pn * G.sa];
-            ra[dn] = u[G.oa + (long)pn * G.sa];
-          }
-          cxd w[4];
-          cw(nd, w);
-          const cxd rd(nd.rd.x, nd.rd.y);
-          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
-          const cxd va = qsum(an * vq);
-          const cxd t = an * (cq * iWe) + (va * rd) * vq;
-          const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
-          const cxd dv = qsum((an - ap) * vq);
-          const long o = (long)p * G.sa;
-          if (q == 0) st<T>(u, G.oa + o, cxd((double)ca.x, (double)ca.y) + (qp + dv * rd));
-          st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
-          an = ap;
-        }
+    auto step = [&](const int p, const int d) {
+      const QNode<RS>& nd = ring[d];
+      const double iWe = (double)riw[d];
+      const E cu = ru[d], ca = ra[d];
+      {
+        const int pn = max(p - D, 1), dn = (d + D) % R;
+        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        riw[dn] = iw[pn];
+        ru[dn] = u[off_q + (long)pn * G.sa];
+        ra[dn] = u[G.oa + (long)pn * G.sa];
       }
+      cxd w[4];
+      cw(nd, w);
+      const cxd rd(nd.rd.x, nd.rd.y);
+      const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+      const cxd va = qsum(an * vq);
+      const cxd t = an * (cq * iWe) + (va * rd) * vq;
+      const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
+      const cxd dv = qsum((an - ap) * vq);
+      const long o = (long)p * G.sa;
+      if (q == 0) st<T>(u, G.oa + o, cxd((double)ca.x, (double)ca.y) + (qp + dv * rd));
+      st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
+      an = ap;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = last;
+#pragma unroll UNR
+    for (; p0 - (R - 1) >= 1; p0 -= R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 - d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 - d >= 1) step(p0 - d, d);
     // along-line edge 0, between the boundary node and node 1
     const cxd rd0(fac[10 * stride].v[0], fac[10 * stride].v[1]);
     const PS r0 = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (8 / VS) * stride) + 8 % VS);
@@ -1519,39 +1547,43 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
     cxd xp = xm;   // component q of x_{p-1}
     cxd rdm = rdmid, qm = qmid;
-#pragma unroll UNR
-    for (int p0 = last; p0 <= M; p0 += R) {
-#pragma unroll
-      for (int d = 0; d < R; d++) {
-        const int p = p0 + d;
-        if (p <= M) {
-          const QNode<RS>& nd = ring[d];
-          const double iWe = (double)riw[d];
-          const E cu = ru[d], ca = ra[d];
-          {
-            const int pn = min(p + D, M), dn = (d + D) % R;
-            qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
-            riw[dn] = iw[pn - 1];
-            ru[dn] = u[off_q + (long)pn * G.sa];
-            ra[dn] = u[G.oa + (long)(pn - 1) * G.sa];
-          }
-          cxd w[4];
-          cw(nd, w);
-          const cxd rd(nd.rd.x, nd.rd.y);
-          const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
-          const cxd vx = qsum(xp * vq);
-          const cxd t = xp * (cq * iWe) + (vx * rdm) * vq;
-          const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
-          const cxd dv = qsum((ap - xp) * vq);
-          const long o = (long)p * G.sa;
-          if (q == 0) st<T>(u, G.oa + o - G.sa, cxd((double)ca.x, (double)ca.y) + (qm + dv * rdm));
-          st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
-          xp = ap;
-          rdm = rd;
-          qm = qp;
-        }
+    auto step = [&](const int p, const int d) {
+      const QNode<RS>& nd = ring[d];
+      const double iWe = (double)riw[d];
+      const E cu = ru[d], ca = ra[d];
+      {
+        const int pn = min(p + D, M), dn = (d + D) % R;
+        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        riw[dn] = iw[pn - 1];
+        ru[dn] = u[off_q + (long)pn * G.sa];
+        ra[dn] = u[G.oa + (long)(pn - 1) * G.sa];
       }
+      cxd w[4];
+      cw(nd, w);
+      const cxd rd(nd.rd.x, nd.rd.y);
+      const cxd qp = cxd((double)nd.rho.x, (double)nd.rho.y) * rd;
+      const cxd vx = qsum(xp * vq);
+      const cxd t = xp * (cq * iWe) + (vx * rdm) * vq;
+      const cxd ap = cxd((double)nd.s.x, (double)nd.s.y) - qsymv(w, t);
+      const cxd dv = qsum((ap - xp) * vq);
+      const long o = (long)p * G.sa;
+      if (q == 0) st<T>(u, G.oa + o - G.sa, cxd((double)ca.x, (double)ca.y) + (qm + dv * rdm));
+      st<T>(u, off_q + o, cxd((double)cu.x, (double)cu.y) + ap);
+      xp = ap;
+      rdm = rd;
+      qm = qp;
+    };
+    // whole groups of R nodes without a guard per node (guards inside the
+    // group make the register allocator copy the ring), then the rest
+    int p0 = last;
+#pragma unroll UNR
+    for (; p0 + (R - 1) <= M; p0 += R) {
+#pragma unroll
+      for (int d = 0; d < R; d++) step(p0 + d, d);
     }
+#pragma unroll
+    for (int d = 0; d < R - 1; d++)
+      if (p0 + d <= M) step(p0 + d, d);
     // along-line edge M = N-1, between node M and the boundary node
     const cxd dv = qsum((cxd(0, 0) - xp) * vq);
     const long o = G.oa + (long)M * G.sa;
