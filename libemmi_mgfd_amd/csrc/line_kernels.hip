@@ -624,17 +624,15 @@ __device__ __forceinline__ void symv(const Ch<R> (&w)[NC], const cxd (&y)[4], cx
 // Lines of one colour are not coupled, so these are the residuals each line
 // solve of the colour starts from.
 // the records of node p of line `line` (at (pb, pc)) of job J
+// the ten values of that record (node 0: rho only, the rest zero)
 template <typename T, int A>
-__device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, int pc, int p, long nl) {
+__device__ __forceinline__ void node_rhs_rec(const LineJob<T>& J, int pb, int pc, int p,
+                                             Ch<ScrRec<T>> (&c)[schunks<ScrRec<T>>()]) {
   constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
   const Level<T>& L = J.L;
   const typename cvec<T>::type* __restrict__ u = J.u;
   const typename cvec<T>::type* __restrict__ f = J.f;
   const long base = (long)pb * L.es[B] + (long)pc * L.es[C] + (long)p * L.es[A];
-  using R = ScrRec<T>;
-  constexpr int SC = schunks<R>(), V = Ch<R>::V;
-  Ch<R> c[SC] = {};
-  Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
   {
     int q[3];
     q[A] = p; q[B] = pb; q[C] = pc;
@@ -658,6 +656,17 @@ __device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, 
     q[C] = pc;
     g = ld<T>(f, C * L.nc + base) - apply_row<T, C>(L, u, q[C], q[A], q[B], base);
     put(c, 6, g.re); put(c, 7, g.im);
+  }
+}
+
+template <typename T, int A>
+__device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, int pc, int p, long nl) {
+  using R = ScrRec<T>;
+  constexpr int SC = schunks<R>(), V = Ch<R>::V;
+  Ch<R> c[SC] = {};
+  node_rhs_rec<T, A>(J, pb, pc, p, c);
+  Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
+  if (p >= 1) {
 #pragma unroll
     for (int k = 0; k < SC; k++) rec[k * nl] = c[k];
   } else {
@@ -1263,15 +1272,10 @@ struct QNode {
 };
 
 template <typename RS>
-__device__ __forceinline__ void qload(QNode<RS>& n, const Ch<double>* __restrict__ fnode, const Ch<RS>* __restrict__ snode,
-                                      long stride, const int (&wch)[4], long soff, int sel) {
-  using P = typename cvec<RS>::type;
+__device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restrict__ fnode, long stride, const int (&wch)[4]) {
 #pragma unroll
   for (int j = 0; j < 4; j++) n.w[j] = *reinterpret_cast<const double2*>(&fnode[wch[j] * stride]);
   n.rd = *reinterpret_cast<const double2*>(&fnode[10 * stride]);
-  n.s = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + soff) + sel);
-  constexpr int VS = Ch<RS>::V;
-  n.rho = *reinterpret_cast<const P*>(reinterpret_cast<const RS*>(snode + (8 / VS) * stride) + 8 % VS);
 }
 
 // NT > 0: every job of the launch has lines of NT nodes, and the passes are
@@ -1296,12 +1300,26 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   constexpr int UNR = NT > 0 ? 64 : 1;   // the passes' unroll count
   if (N < 2) return;   // uniform over the block: no interior node
   const int M = N - 1, m = N / 2;
+  // LR: the block's line records live in LDS (lrec[node][line][value]) and
+  // never go to memory, which takes a store and a dependent load off each end
+  // of the substitution
+  constexpr bool LR = RHS && NT > 0 && 16 * NT * SREC * sizeof(RS) <= 48 * 1024;
+  __shared__ RS lrec[LR ? 16 * NT * SREC : 1];
   if (RHS) {
     const long nl = (long)J.nlo * J.nhi;
     for (int idx = threadIdx.x; idx < 16 * N; idx += blockDim.x) {
       const int ln = blockIdx.x * 16 + (idx & 15), p = idx >> 4;
       int b2, c2;
-      if (line_of<T, A>(J, ln, b2, c2)) node_rhs<T, A>(J, ln, b2, c2, p, nl);
+      if (line_of<T, A>(J, ln, b2, c2)) {
+        if constexpr (LR) {
+          Ch<RS> c[SC] = {};
+          node_rhs_rec<T, A>(J, b2, c2, p, c);
+#pragma unroll
+          for (int v = 0; v < SREC; v++) lrec[(p * 16 + (idx & 15)) * SREC + v] = (RS)val(c, v);
+        } else {
+          node_rhs<T, A>(J, ln, b2, c2, p, nl);
+        }
+      }
     }
     __syncthreads();   // the block's records are complete (workgroup scope)
   }
@@ -1326,6 +1344,25 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   const long soff = (long)((2 * q) / VS) * stride;
   const int sel = (2 * q) % VS;
   const long off_q = q == 0 ? G.ob0 : q == 1 ? G.ob1 : q == 2 ? G.oc0 : G.oc1;
+  // component q of sigma (z once written) and rho of node p of this lane's line
+  const int lloc = lane >> 2;
+  auto rec_s = [&](int p) -> PS {
+    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * 16 + lloc) * SREC + 2 * q]);
+    else return *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + p * sstep + soff) + sel);
+  };
+  auto rec_rho = [&](int p) -> PS {
+    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * 16 + lloc) * SREC + 8]);
+    else return *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + p * sstep + (8 / VS) * stride) + 8 % VS);
+  };
+  auto put_z = [&](int p, PS z) {
+    if constexpr (LR) *reinterpret_cast<PS*>(&lrec[(p * 16 + lloc) * SREC + 2 * q]) = z;
+    else *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = z;
+  };
+  auto qnode = [&](QNode<RS>& n, int p) {
+    qload_f(n, fac + p * fstep, stride, wch);
+    n.s = rec_s(p);
+    n.rho = rec_rho(p);
+  };
 
   // D nodes ahead are in flight: the load for node p + D goes into the slot
   // node p - 1 used, so a slot is never copied before it is refilled
@@ -1344,11 +1381,11 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = max(min(1 + d, last), 1);
-      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      qnode(ring[d], pn);
       riw[d] = iw[pn - 1];
     }
     const cxd rd0(fac[10 * stride].v[0], fac[10 * stride].v[1]);
-    const PS r0 = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (8 / VS) * stride) + 8 % VS);
+    const PS r0 = rec_rho(0);
     cxd rd_prev = rd0;
     cxd q_prev = cxd((double)r0.x, (double)r0.y) * rd0;
     cxd z(0, 0);
@@ -1357,7 +1394,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       const double iWe = (double)riw[d];
       {
         const int pn = min(p + D, last), dn = (d + D) % R;
-        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        qnode(ring[dn], pn);
         riw[dn] = iw[pn - 1];
       }
       cxd w[4];
@@ -1373,7 +1410,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       PS zs;
       zs.x = (RS)z.re;
       zs.y = (RS)z.im;
-      *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+      put_z(p, zs);
       rd_prev = rd;
       q_prev = qp;
     };
@@ -1401,10 +1438,10 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = min(max(M - d, last), M);
-      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      qnode(ring[d], pn);
       riw[d] = iw[pn];
       rdn[d] = fac[(pn - 1) * fstep + 10 * stride];
-      qn[d] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+      qn[d] = rec_rho(pn - 1);
     }
     cxd zn(0, 0), rd_last(0, 0);
     auto step = [&](const int p, const int d) {
@@ -1413,10 +1450,10 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       const cxd rdm(rdn[d].v[0], rdn[d].v[1]), rhom((double)qn[d].x, (double)qn[d].y);
       {
         const int pn = max(p - D, last), dn = (d + D) % R;
-        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        qnode(ring[dn], pn);
         riw[dn] = iw[pn];
         rdn[dn] = fac[(pn - 1) * fstep + 10 * stride];
-        qn[dn] = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (pn - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+        qn[dn] = rec_rho(pn - 1);
       }
       cxd w[4];
       cw(nd, w);
@@ -1432,7 +1469,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       PS zs;
       zs.x = (RS)zn.re;
       zs.y = (RS)zn.im;
-      *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = zs;
+      put_z(p, zs);
       rd_last = rdm;
     };
     // whole groups of R nodes without a guard per node (guards inside the
@@ -1462,9 +1499,9 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   cxd xm, qmid, rdmid;
   {
     QNode<RS> nd;
-    qload(nd, fac + m * fstep, scr + m * sstep, stride, wch, soff, sel);
+    qnode(nd, m);
     const Ch<double> c = fac[(m - 1) * fstep + 10 * stride];
-    const PS r = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (m - 1) * sstep + (8 / VS) * stride) + 8 % VS);
+    const PS r = rec_rho(m - 1);
     cxd w[4];
     cw(nd, w);
     rdmid = cxd(nd.rd.x, nd.rd.y);
@@ -1488,7 +1525,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = max(last - d, 1);
-      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      qnode(ring[d], pn);
       riw[d] = iw[pn];
       ru[d] = u[off_q + (long)pn * G.sa];
       ra[d] = u[G.oa + (long)pn * G.sa];
@@ -1500,7 +1537,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       const E cu = ru[d], ca = ra[d];
       {
         const int pn = max(p - D, 1), dn = (d + D) % R;
-        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        qnode(ring[dn], pn);
         riw[dn] = iw[pn];
         ru[dn] = u[off_q + (long)pn * G.sa];
         ra[dn] = u[G.oa + (long)pn * G.sa];
@@ -1531,7 +1568,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       if (p0 - d >= 1) step(p0 - d, d);
     // along-line edge 0, between the boundary node and node 1
     const cxd rd0(fac[10 * stride].v[0], fac[10 * stride].v[1]);
-    const PS r0 = *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + (8 / VS) * stride) + 8 % VS);
+    const PS r0 = rec_rho(0);
     const cxd q0 = cxd((double)r0.x, (double)r0.y) * rd0;
     const cxd dv = qsum(an * vq);
     if (q == 0) st<T>(u, G.oa, ld<T>(u, G.oa) + (q0 + dv * rd0));
@@ -1540,7 +1577,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
       const int pn = min(last + d, M);
-      qload(ring[d], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+      qnode(ring[d], pn);
       riw[d] = iw[pn - 1];
       ru[d] = u[off_q + (long)pn * G.sa];
       ra[d] = u[G.oa + (long)(pn - 1) * G.sa];
@@ -1553,7 +1590,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       const E cu = ru[d], ca = ra[d];
       {
         const int pn = min(p + D, M), dn = (d + D) % R;
-        qload(ring[dn], fac + pn * fstep, scr + pn * sstep, stride, wch, soff, sel);
+        qnode(ring[dn], pn);
         riw[dn] = iw[pn - 1];
         ru[dn] = u[off_q + (long)pn * G.sa];
         ra[dn] = u[G.oa + (long)(pn - 1) * G.sa];
