@@ -69,7 +69,7 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 // Launches of at most this many lines per job (FP32-storage / FP64 cycles)
 // use the quad-lane substitution, whose ring holds EMMI_LINE_QUAD_RING nodes.
 #ifndef EMMI_LINE_QUAD_MAX_LINES_F32
-#define EMMI_LINE_QUAD_MAX_LINES_F32 1024
+#define EMMI_LINE_QUAD_MAX_LINES_F32 2048
 #endif
 #ifndef EMMI_LINE_QUAD_MAX_LINES_F64
 #define EMMI_LINE_QUAD_MAX_LINES_F64 4096
@@ -81,7 +81,7 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 // FP64 cycles) compute their right-hand sides themselves (no k_line_rhs
 // launch)
 #ifndef EMMI_LINE_FUSE_MAX_LINES_F32
-#define EMMI_LINE_FUSE_MAX_LINES_F32 1024
+#define EMMI_LINE_FUSE_MAX_LINES_F32 2048
 #endif
 #ifndef EMMI_LINE_FUSE_MAX_LINES_F64
 #define EMMI_LINE_FUSE_MAX_LINES_F64 4096
