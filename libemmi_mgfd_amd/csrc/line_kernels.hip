@@ -1278,6 +1278,12 @@ __device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restri
   n.rd = *reinterpret_cast<const double2*>(&fnode[10 * stride]);
 }
 
+// fused launches with lines of NT nodes keep the block's line records in LDS
+template <typename T>
+__host__ __device__ constexpr bool quad_lds_records(int nt) {
+  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= 48 * 1024;
+}
+
 // NT > 0: every job of the launch has lines of NT nodes, and the passes are
 // unrolled completely.  Loop-carried ring slots make the register allocator
 // load into temporaries and copy them into the slots, and each copy waits for
@@ -1303,7 +1309,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   // LR: the block's line records live in LDS (lrec[node][line][value]) and
   // never go to memory, which takes a store and a dependent load off each end
   // of the substitution
-  constexpr bool LR = RHS && NT > 0 && 16 * NT * SREC * sizeof(RS) <= 48 * 1024;
+  constexpr bool LR = RHS && quad_lds_records<T>(NT);
   __shared__ RS lrec[LR ? 16 * NT * SREC : 1];
   if (RHS) {
     const long nl = (long)J.nlo * J.nhi;
@@ -1690,7 +1696,7 @@ template <typename T>
 static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
   int maxl = 0;
   long maxnodes = 0;
-  double rhs_bytes = 0, solve_bytes = 0;
+  double rhs_bytes = 0, solve_bytes = 0, rec_bytes = 0;
   const double es = sizeof(typename cvec<T>::type), cs = sizeof(T);
   for (int j = 0; j < P.njobs; j++) {
     const long nl = (long)P.job[j].nlo * P.job[j].nhi;
@@ -1703,6 +1709,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     rhs_bytes += rows * (es + cs) + 3.0 * (double)P.job[j].L.nc * es + (double)nl * na * SREC * sizeof(ScrRec<T>);
     // substitution: records and factors read once, the rows of u read and written
     solve_bytes += (double)nl * na * (SREC * sizeof(ScrRec<T>) + FREC * sizeof(FacRec<T>)) + rows * 2.0 * es;
+    rec_bytes += (double)nl * na * SREC * sizeof(ScrRec<T>);
   }
   if (maxl == 0 || P.njobs == 0) return;
   int maxna = 0;
@@ -1751,12 +1758,14 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
     const dim3 g((maxl + 15) / 16, P.njobs);
     if (fuse) {
-      const double by = rhs_bytes + solve_bytes;
       const dim3 b(FT);
       // a common line length with an unrolled instance, else the rolled loop
       int nt = P.job[0].L.n[axis];
       for (int j = 1; j < P.njobs; j++)
         if (P.job[j].L.n[axis] != nt) nt = 0;
+      if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64) nt = 0;
+      // records kept in LDS are neither written to nor read from memory
+      const double by = rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0);
       switch (nt) {
         case 4: launch_quad_fused<T, 4>(P, axis, g, b, s, by); break;
         case 8: launch_quad_fused<T, 8>(P, axis, g, b, s, by); break;
