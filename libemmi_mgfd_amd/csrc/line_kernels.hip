@@ -1685,8 +1685,11 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, doubl
 }
 
 template <typename T, int NT>
-static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, dim3 b, hipStream_t s, double by) {
-  constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
+static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_t s, double by) {
+  // 64-node lines come 2048 to a colour: blocks of 256 threads let two blocks
+  // share a CU (registers), so the launch runs in one round of blocks
+  constexpr int QD = EMMI_LINE_QUAD_RING, FT = NT == 64 ? 256 : EMMI_LINE_FUSE_THREADS;
+  const dim3 b(FT);
   if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
   else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
   else launch_timed(k_line_solve_q<T, 2, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
@@ -1758,7 +1761,6 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
     const dim3 g((maxl + 15) / 16, P.njobs);
     if (fuse) {
-      const dim3 b(FT);
       // a common line length with an unrolled instance, else the rolled loop
       int nt = P.job[0].L.n[axis];
       for (int j = 1; j < P.njobs; j++)
@@ -1767,12 +1769,12 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
       // records kept in LDS are neither written to nor read from memory
       const double by = rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0);
       switch (nt) {
-        case 4: launch_quad_fused<T, 4>(P, axis, g, b, s, by); break;
-        case 8: launch_quad_fused<T, 8>(P, axis, g, b, s, by); break;
-        case 16: launch_quad_fused<T, 16>(P, axis, g, b, s, by); break;
-        case 32: launch_quad_fused<T, 32>(P, axis, g, b, s, by); break;
-        case 64: launch_quad_fused<T, 64>(P, axis, g, b, s, by); break;
-        default: launch_quad_fused<T, 0>(P, axis, g, b, s, by); break;
+        case 4: launch_quad_fused<T, 4>(P, axis, g, s, by); break;
+        case 8: launch_quad_fused<T, 8>(P, axis, g, s, by); break;
+        case 16: launch_quad_fused<T, 16>(P, axis, g, s, by); break;
+        case 32: launch_quad_fused<T, 32>(P, axis, g, s, by); break;
+        case 64: launch_quad_fused<T, 64>(P, axis, g, s, by); break;
+        default: launch_quad_fused<T, 0>(P, axis, g, s, by); break;
       }
     } else {
       const dim3 b(128);
