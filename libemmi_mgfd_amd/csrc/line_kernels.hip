@@ -69,7 +69,7 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 // Launches of at most this many lines per job (FP32-storage / FP64 cycles)
 // use the quad-lane substitution, whose ring holds EMMI_LINE_QUAD_RING nodes.
 #ifndef EMMI_LINE_QUAD_MAX_LINES_F32
-#define EMMI_LINE_QUAD_MAX_LINES_F32 2048
+#define EMMI_LINE_QUAD_MAX_LINES_F32 4096
 #endif
 #ifndef EMMI_LINE_QUAD_MAX_LINES_F64
 #define EMMI_LINE_QUAD_MAX_LINES_F64 4096
@@ -81,7 +81,7 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 // FP64 cycles) compute their right-hand sides themselves (no k_line_rhs
 // launch)
 #ifndef EMMI_LINE_FUSE_MAX_LINES_F32
-#define EMMI_LINE_FUSE_MAX_LINES_F32 2048
+#define EMMI_LINE_FUSE_MAX_LINES_F32 4096
 #endif
 #ifndef EMMI_LINE_FUSE_MAX_LINES_F64
 #define EMMI_LINE_FUSE_MAX_LINES_F64 4096
@@ -1281,13 +1281,14 @@ __device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restri
 // fused launches with lines of NT nodes keep the block's line records in LDS
 template <typename T>
 __host__ __device__ constexpr bool quad_lds_records(int nt) {
-  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= 48 * 1024;
+  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= 80 * 1024;
 }
 
-// NT > 0: every job of the launch has lines of NT nodes, and the passes are
-// unrolled completely.  Loop-carried ring slots make the register allocator
-// load into temporaries and copy them into the slots, and each copy waits for
-// its load: the rolled loop keeps only about one node in flight.
+// NT > 0: every job of the launch has lines of NT nodes; up to 64 nodes the
+// passes are unrolled completely (longer lines keep the loop: the code would
+// not fit the instruction cache).  The rolled loop's guard-free groups keep
+// the ring in place (see k_line_solve); the unrolled passes also drop the
+// loop overhead of these short chains.
 //
 // With RHS the block first computes the records of its own 16 lines (the
 // work of k_line_rhs, for launches too small to pay for a launch of their
@@ -1303,7 +1304,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int N = NT > 0 ? NT : L.n[A];
-  constexpr int UNR = NT > 0 ? 64 : 1;   // the passes' unroll count
+  constexpr int UNR = NT > 0 && NT <= 64 ? 64 : 1;   // the passes' unroll count
   if (N < 2) return;   // uniform over the block: no interior node
   const int M = N - 1, m = N / 2;
   // LR: the block's line records live in LDS (lrec[node][line][value]) and
@@ -1686,9 +1687,10 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, doubl
 
 template <typename T, int NT>
 static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_t s, double by) {
-  // 64-node lines come 2048 to a colour: blocks of 256 threads let two blocks
-  // share a CU (registers), so the launch runs in one round of blocks
-  constexpr int QD = EMMI_LINE_QUAD_RING, FT = NT == 64 ? 256 : EMMI_LINE_FUSE_THREADS;
+  // lines of 64 and more nodes come thousands to a colour: blocks of 256
+  // threads let two blocks share a CU (registers, LDS), so the launch needs
+  // fewer rounds of blocks
+  constexpr int QD = EMMI_LINE_QUAD_RING, FT = NT >= 64 ? 256 : EMMI_LINE_FUSE_THREADS;
   const dim3 b(FT);
   if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
   else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
@@ -1741,7 +1743,10 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     else hipLaunchKernelGGL((k_line_factor<T, 2>), g, b, 0, s, P);
     return;
   }
-  const bool quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>());
+  // along x the quad-lane kernel's updates of u are scattered rows (see
+  // k_line_solve_x_apply): the finest FP32 x sweeps keep the one-lane kernel
+  const bool quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>() &&
+                                           (axis != 0 || maxl <= 2048 || sizeof(T) == 8));
   const bool fuse = quad && maxl <= fuse_max_lines<T>();
   if (fuse) {
     // the substitution kernel computes its own right-hand sides
@@ -1765,7 +1770,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
       int nt = P.job[0].L.n[axis];
       for (int j = 1; j < P.njobs; j++)
         if (P.job[j].L.n[axis] != nt) nt = 0;
-      if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64) nt = 0;
+      if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64 && nt != 128) nt = 0;
       // records kept in LDS are neither written to nor read from memory
       const double by = rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0);
       switch (nt) {
@@ -1774,6 +1779,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
         case 16: launch_quad_fused<T, 16>(P, axis, g, s, by); break;
         case 32: launch_quad_fused<T, 32>(P, axis, g, s, by); break;
         case 64: launch_quad_fused<T, 64>(P, axis, g, s, by); break;
+        case 128: launch_quad_fused<T, 128>(P, axis, g, s, by); break;
         default: launch_quad_fused<T, 0>(P, axis, g, s, by); break;
       }
     } else {
