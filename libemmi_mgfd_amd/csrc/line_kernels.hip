@@ -1763,10 +1763,10 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
   }
   if (quad) {
     // few lines: four lanes per line half shorten the serial chain
-    constexpr int QD = EMMI_LINE_QUAD_RING, FT = EMMI_LINE_FUSE_THREADS;
+    constexpr int QD = EMMI_LINE_QUAD_RING;
     const dim3 g((maxl + 15) / 16, P.njobs);
     if (fuse) {
-      // a common line length with an unrolled instance, else the rolled loop
+      // a common line length with an instance of its own, else the generic one
       int nt = P.job[0].L.n[axis];
       for (int j = 1; j < P.njobs; j++)
         if (P.job[j].L.n[axis] != nt) nt = 0;
