@@ -1690,7 +1690,10 @@ static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_
   // lines of 64 and more nodes come thousands to a colour: blocks of 256
   // threads let two blocks share a CU (registers, LDS), so the launch needs
   // fewer rounds of blocks
-  constexpr int QD = EMMI_LINE_QUAD_RING, FT = NT >= 64 ? 256 : EMMI_LINE_FUSE_THREADS;
+  // the 128-node lines keep their loop (not unrolled): a deeper ring pays
+  // there in FP32 cycles (in FP64 it spills)
+  constexpr int QD = NT == 128 && sizeof(T) == 4 ? 5 : EMMI_LINE_QUAD_RING;
+  constexpr int FT = NT >= 64 ? 256 : EMMI_LINE_FUSE_THREADS;
   const dim3 b(FT);
   if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
   else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
