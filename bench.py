@@ -38,10 +38,10 @@ HBM_PEAK_GBS = 8000.0
 # the mixed+batched 128^3 step with two rocprofv3 --pmc passes (FETCH_SIZE,
 # then WRITE_SIZE, those kernels and the runtime's copy kernel), each scaled
 # by the same counter on the 103 MB device-to-device copies of that run
-# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 70.48 MB fetched + 7.49 MB written
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00): 70.61 MB fetched + 7.49 MB written
 # per launch against 39.44 MB algorithmic.  The two passes of the one-lane
 # substitution read the factors twice and write and re-read z once.
-LINE_SOLVE_TRAFFIC_RATIO = (70.48e6 + 7.49e6) / 39.44e6
+LINE_SOLVE_TRAFFIC_RATIO = (70.61e6 + 7.49e6) / 39.44e6
 
 
 def parse():
