@@ -147,10 +147,13 @@ typedef struct {
 typedef struct {
   int cycles;         /* completed cycles */
   int status;         /* EMMI_CONVERGED, EMMI_NOT_CONVERGED, EMMI_NONFINITE */
-  double rnorm0;      /* ||f|| */
+  double rnorm0;      /* ||f|| (FP64 V- and W-cycles: the first in-cycle norm,
+                         after the pre-smoothing of cycle 0, as src/gmg.c:1594-1599) */
   double rnorm;       /* last ||f - A u|| */
-  int x0_rejected;    /* 1: the initial guess had the larger residual after its
-                         first cycle and the solve restarted from zero */
+  int x0_rejected;    /* 1: the initial guess had a residual ||f - A x0|| not
+                         below ||f|| (or not finite) and the solve started
+                         from zero instead */
+  double x0_rnorm;    /* ||f - A x0|| of the initial guess, 0 without one */
 } emmi_stats_t;
 
 enum { EMMI_CONVERGED = 0, EMMI_NOT_CONVERGED = 1, EMMI_NONFINITE = 2, EMMI_ERROR = -1 };
@@ -237,6 +240,13 @@ int emmi_inject_adj_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *x_dev, v
  * (sigma33 of the multigrid, rho33 of the model).  Same arithmetic as the
  * host entry point, bit for bit. */
 int emmi_extract_electric_field_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_dev, void *e_dev,
+                                    void *stream);
+/* extract_magnetic_field (src/inject_extract.c:638-785) on the device: H of
+ * the device field vector h_dev (complex double, as gmg_apply returns it in
+ * b) on the model's cell centres, written to e_dev as complex float
+ * [3][nz][ny][nx] (H1, H2, H3) instead of emf->H1..H3[ifreq].  Same
+ * arithmetic as the host entry point, bit for bit. */
+int emmi_extract_magnetic_field_dev(acq_t *acq, emf_t *emf, int ifreq, const void *h_dev, void *e_dev,
                                     void *stream);
 /* build_gradient (src/build_gradient.c:13-111) on the device: ef_dev[f] and
  * ea_dev[f] (f < emf->nfreq <= 32) are the forward and adjoint fields of

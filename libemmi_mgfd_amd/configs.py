@@ -200,4 +200,97 @@ def tiny(n1=12, n2=10, n3=16, nrec=5, rho_air=1e8) -> Survey:
                   lextend=8000.0, rho_air=rho_air, bathy=bathy)
 
 
-CONFIGS = {"config1": config1, "config2": config2, "tiny": tiny}
+def _flat1000(x, y):
+    return np.full_like(x, 1000.0)
+
+
+# multigrid size of each frequency of the batched survey: lower frequencies
+# diffuse further and get the coarser grid
+CONFIG3_GRIDS = {0.1: (96, 96, 96), 0.25: (128, 128, 128), 0.5: (128, 128, 128), 1.0: (160, 160, 128)}
+
+
+def config3() -> Survey:
+    """Batched survey: 25 shots x 4 frequencies, each frequency on its own
+    multigrid size (``extra["grids"]``).
+
+    The configuration-1 earth (flat seafloor at 1000 m, 100 ohm-m layer at
+    1480-1600 m, 1 ohm-m below) on a 161x161x71 grid of 100/100/40 m cells
+    with origin (-8050, -8050, 0).  The receiver lines reach x = +-8 km,
+    outside configuration 1's +-5.05 km model, which the reference's
+    acquisition check (src/acq_init_free.c:207-212) refuses.  The model is
+    71 cells deep (to 2840 m) rather than 51: the reference finds the source
+    cell with a bisection over nx+2nb = 181 nodes of the z axis
+    (src/regrid_init_free.c:194), whose first probe (node 90) lies past a
+    72-node z axis; with 92 nodes every probe stays on the axis.  The extra
+    cells repeat the 1 ohm-m half-space that the padding below would
+    repeat anyway.  Sources on a 5x5 grid,
+    x, y in {-4, -2, 0, 2, 4} km (x fastest), 50 m above the seafloor,
+    azimuth 0; 5 receiver lines y in {-4, ..., 4} km of 81 receivers,
+    x = -8..8 km every 200 m, on the seafloor; every source records every
+    receiver."""
+    nx = ny = 161
+    nz = 71
+    dx = dy = 100.0
+    dz = 40.0
+    ox = oy = -8050.0
+    rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0, _flat1000, [(1480.0, 1600.0, 100.0)])
+    km = [-4000.0, -2000.0, 0.0, 2000.0, 4000.0]
+    srcs = [(x, y, 950.0, 0.0, 0.0) for y in km for x in km]
+    recs = [(float(x), y, 1000.0, 0.0, 0.0) for y in km for x in np.arange(-8000.0, 8000.0 + 1, 200.0)]
+    table = [(s + 1, r + 1) for s in range(len(srcs)) for r in range(len(recs))]
+    freqs = [0.1, 0.25, 0.5, 1.0]
+    s = Survey("config3", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh.copy(), rv, srcs, recs, table,
+               freqs, 128, 128, 128, bathy=bathy)
+    s.extra["grids"] = dict(CONFIG3_GRIDS)
+    return s
+
+
+def config3_single(shot: int, freq: float) -> Survey:
+    """One (shot, frequency) right-hand side of :func:`config3` as the
+    reference runs it: one source (1-based ``shot``), that shot's receivers,
+    one frequency on that frequency's multigrid size."""
+    s = config3()
+    src = s.sources[shot - 1]
+    recs = [s.receivers[r - 1] for (i, r) in s.table if i == shot]
+    n1, n2, n3 = s.extra["grids"][freq]
+    return Survey(f"config3_s{shot}_f{freq:g}", s.nx, s.ny, s.nz, s.dx, s.dy, s.dz, s.ox, s.oy, s.oz,
+                  s.rho11, s.rho22, s.rho33, [src], recs, [(1, i + 1) for i in range(len(recs))],
+                  [freq], n1, n2, n3, bathy=s.bathy)
+
+
+CONFIG5_BOXES = [(-1500.0, 0.0, 1400.0, 3000.0, 2000.0, 150.0, 50.0),
+                 (1500.0, 500.0, 2100.0, 3000.0, 2000.0, 150.0, 50.0)]
+
+
+def config5(true_model: bool = True) -> Survey:
+    """Mode-2 gradient at survey scale: 25 shots x 3 frequencies on a
+    160x160x128 multigrid grid.
+
+    Configuration-2 acquisition on its 50/50/20 m input grid with origin
+    (-5025, -5025, 0): 201 inline seafloor receivers (x = -5..5 km every
+    50 m, y = 0) and 25 inline x-dipole sources, x = -4.8..4.8 km every
+    400 m, 50 m above the seafloor.  The true earth is the configuration-1
+    background (flat seafloor at 1000 m, 1 ohm-m sediment, 100 ohm-m layer
+    at 1480-1600 m) plus two 50 ohm-m boxes of 3x2x0.15 km centred at depths
+    1400 m and 2100 m (``CONFIG5_BOXES``); the grid is 126 cells deep (to
+    2520 m) so that the deeper box lies inside the model.  With
+    ``true_model=False`` the model is the starting model of the gradient: a
+    1 ohm-m half-space under the seawater."""
+    nx = ny = 201
+    nz = 126
+    dx = dy = 50.0
+    dz = 20.0
+    ox = oy = -5025.0
+    if true_model:
+        rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0, _flat1000, [(1480.0, 1600.0, 100.0)],
+                                 blocks=CONFIG5_BOXES)
+    else:
+        rh, rv, bathy = _layered(nx, ny, nz, dx, dy, dz, ox, oy, 0.0, _flat1000, [])
+    srcs = [(float(x), 0.0, 950.0, 0.0, 0.0) for x in np.arange(-4800.0, 4800.0 + 1, 400.0)]
+    recs = [(float(x), 0.0, 1000.0, 0.0, 0.0) for x in np.arange(-5000.0, 5000.0 + 1, 50.0)]
+    table = [(s + 1, r + 1) for s in range(len(srcs)) for r in range(len(recs))]
+    return Survey("config5" if true_model else "config5_start", nx, ny, nz, dx, dy, dz, ox, oy, 0.0,
+                  rh, rh.copy(), rv, srcs, recs, table, [0.25, 0.75, 1.25], 160, 160, 128, bathy=bathy)
+
+
+CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config5": config5, "tiny": tiny}

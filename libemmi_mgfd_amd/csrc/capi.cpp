@@ -135,6 +135,7 @@ int emmi_solver_solve_batch(emmi_solver_t* h, int n, const void* const* b, void*
         st[k].rnorm0 = ss[k].rnorm0;
         st[k].rnorm = ss[k].rnorm;
         st[k].x0_rejected = ss[k].x0_rejected;
+        st[k].x0_rnorm = ss[k].x0_rnorm;
       }
       worst = std::max(worst, ss[k].status);
     }

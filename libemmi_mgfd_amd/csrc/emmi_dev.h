@@ -79,7 +79,11 @@ __device__ __forceinline__ void ijk_of(long t, long es1, int n2p1, int& i, int& 
   j = (int)(q - (unsigned)k * m);
 }
 
-// Geometry of one multigrid level as the kernels see it.
+// Geometry of one multigrid level as the kernels see it.  The 1D geometry is
+// FP64 whatever the storage precision T of the vectors and of sbar: the
+// discrete curl of a discrete gradient cancels exactly only when every
+// operator of the hierarchy uses the same widths, so FP32-storage cycles see
+// the FP64 operator's widths, not rounded ones.
 //   W[a][i]  cell width along axis a (i in [0, n_a])
 //   IW[a][i] 1/W[a][i]
 //   H[a][i]  dual width 0.5*(W[a][i] + W[a][i-1]) (i in [1, n_a])
@@ -91,10 +95,10 @@ struct Level {
   int n[3];
   long es[3];
   long nc;
-  const T* W[3];
-  const T* IW[3];
-  const T* H[3];
-  const T* X[3];
+  const double* W[3];
+  const double* IW[3];
+  const double* H[3];
+  const double* X[3];
   const T* sbar;
   double omu;
 };

@@ -902,7 +902,7 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   const Ch<RF>* __restrict__ fac = reinterpret_cast<const Ch<RF>*>(J.fac) + line;
   Ch<RS>* __restrict__ scr = reinterpret_cast<Ch<RS>*>(J.scr) + line;
   E* __restrict__ u = J.u;
-  const T* __restrict__ iw = L.IW[A];
+  const double* __restrict__ iw = L.IW[A];
   const long fstep = (long)FC * stride, sstep = (long)SC * stride;
   const long off_u[5] = {G.oa, G.ob0, G.ob1, G.oc0, G.oc1};
 
@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(128) k_line_solve(LinePack<T> P) {
   constexpr int R = D + 1;
   Ch<RF> rf[R][FC];
   Ch<RS> rs[R][SC];
-  T riw[R];
+  double riw[R];
   cxd tm[4];   // this half's coupling term at the meeting node
 #pragma unroll
   for (int i = 0; i < 4; i++) tm[i] = cxd(0, 0);
@@ -1341,7 +1341,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   const Ch<double>* __restrict__ fac = reinterpret_cast<const Ch<double>*>(J.fac) + line;
   Ch<RS>* __restrict__ scr = reinterpret_cast<Ch<RS>*>(J.scr) + line;
   E* __restrict__ u = J.u;
-  const T* __restrict__ iw = L.IW[A];
+  const double* __restrict__ iw = L.IW[A];
   const long fstep = (long)FC * stride, sstep = (long)SC * stride;
   // this lane's component: v_q, the coupling coefficient (l_q = cq / W_a), the
   // chunks of row q of W^-1, and where component q of a line record sits
@@ -1375,7 +1375,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   // node p - 1 used, so a slot is never copied before it is refilled
   constexpr int R = D + 1;
   QNode<RS> ring[R];
-  T riw[R];
+  double riw[R];
   cxd tm(0, 0);   // component q of this half's coupling term at the meeting node
   auto cw = [](const QNode<RS>& n, cxd (&w)[4]) {
 #pragma unroll

@@ -10,7 +10,7 @@ namespace emmi {
 template <typename T>
 struct Transfer {
   int sc[3];
-  const T* wr[3];
+  const double* wr[3];
 };
 
 // The per-slot vectors of one launch over several right-hand sides that share
@@ -51,7 +51,7 @@ void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, 
 template <typename T>
 void launch_zero(void* v, long n, hipStream_t s);
 template <typename T>
-void launch_scale_rhs(const Level<T>& L, const T* D0, const T* D1, const T* D2, const void* b, void* f, hipStream_t s);
+void launch_scale_rhs(const Level<T>& L, const double* D0, const double* D1, const double* D2, const void* b, void* f, hipStream_t s);
 template <typename T>
 void launch_curl_h(const Level<T>& L, const void* u, void* h, hipStream_t s);
 void launch_demote(const void* hi, void* lo, long n, hipStream_t s);

@@ -182,7 +182,7 @@ __global__ void k_fill_zero(typename cvec<T>::type* __restrict__ v, long n) {
 // f = b * dual volume (src/gmg.c:1899-1910); volumes from the 1D widths
 // D[a] = dual widths d (reference's d1/d2/d3 incl. d[0]), W = cell widths
 template <typename T>
-__global__ void k_scale_rhs(Level<T> L, const T* __restrict__ D0, const T* __restrict__ D1, const T* __restrict__ D2,
+__global__ void k_scale_rhs(Level<T> L, const double* __restrict__ D0, const double* __restrict__ D1, const double* __restrict__ D2,
                             const typename cvec<T>::type* __restrict__ b, typename cvec<T>::type* __restrict__ f) {
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= L.nc) return;
@@ -376,7 +376,7 @@ void launch_zero(void* v, long n, hipStream_t s) {
 }
 
 template <typename T>
-void launch_scale_rhs(const Level<T>& L, const T* D0, const T* D1, const T* D2, const void* b, void* f, hipStream_t s) {
+void launch_scale_rhs(const Level<T>& L, const double* D0, const double* D1, const double* D2, const void* b, void* f, hipStream_t s) {
   using V = typename cvec<T>::type;
   hipLaunchKernelGGL(k_scale_rhs<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, D0, D1, D2, (const V*)b, (V*)f);
 }
@@ -423,7 +423,7 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
   template void launch_prolong<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*,  \
                                   hipStream_t);                                                                \
   template void launch_zero<T>(void*, long, hipStream_t);                                                      \
-  template void launch_scale_rhs<T>(const Level<T>&, const T*, const T*, const T*, const void*, void*,         \
+  template void launch_scale_rhs<T>(const Level<T>&, const double*, const double*, const double*, const void*, void*,         \
                                     hipStream_t);                                                              \
   template void launch_curl_h<T>(const Level<T>&, const void*, void*, hipStream_t);                           \
   template void launch_edge_sigma<T>(int, int, int, const double*, const double*, const double*, T*, hipStream_t);

@@ -184,6 +184,36 @@ extern "C" int emmi_extract_electric_field_dev(acq_t* acq, emf_t* emf, int ifreq
   }
 }
 
+// src/inject_extract.c:638-785
+extern "C" int emmi_extract_magnetic_field_dev(acq_t* acq, emf_t* emf, int ifreq, const void* h_dev, void* e_dev,
+                                               void* stream) {
+  (void)acq;
+  (void)ifreq;
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    const emmi::CentreTables T(emf);
+    const int n1 = emf->n1, n2 = emf->n2, n3 = emf->n3;
+    // every gather reads corners (v, v+1) of its table entries
+    in_range(T.i1, 0, n1 - 1, "x"); in_range(T.i2, 0, n2 - 1, "y"); in_range(T.i3, 0, n3 - 1, "z");
+    in_range(T.i1s, 0, n1 - 1, "x"); in_range(T.i2s, 0, n2 - 1, "y"); in_range(T.i3s, 0, n3 - 1, "z");
+    const size_t tb = 6 * padded(sizeof(int) * (size_t)std::max(T.nx, std::max(T.ny, T.nz))) * 2;
+    char* dt = (char*)g_tab.get(tb);
+    size_t off = 0;
+    emmi::CentreDev D;
+    put(dt, off, T.i1, D.i1, s); put(dt, off, T.i1s, D.i1s, s); put(dt, off, T.w1, D.w1, s); put(dt, off, T.w1s, D.w1s, s);
+    put(dt, off, T.i2, D.i2, s); put(dt, off, T.i2s, D.i2s, s); put(dt, off, T.w2, D.w2, s); put(dt, off, T.w2s, D.w2s, s);
+    put(dt, off, T.i3, D.i3, s); put(dt, off, T.i3s, D.i3s, s); put(dt, off, T.w3, D.w3, s); put(dt, off, T.w3s, D.w3s, s);
+    D.nx = T.nx; D.ny = T.ny; D.nz = T.nz;
+    emmi::launch_hfield(D, h_dev, n1, n2, n3, e_dev, s);
+    check(hipStreamSynchronize(s), "sync");
+    check(hipGetLastError(), "hfield");
+    return 0;
+  } catch (const std::exception& e) {
+    g_path_err = e.what();
+    return -1;
+  }
+}
+
 // src/build_gradient.c:13-111
 extern "C" int emmi_build_gradient_dev(acq_t* acq, emf_t* emf, fwi_t* fwi, const void* const* ef_dev,
                                        const void* const* ea_dev, float* grad_dev, float* phess_dev, void* stream) {

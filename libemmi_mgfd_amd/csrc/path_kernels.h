@@ -22,6 +22,10 @@ struct CentreDev {
 void launch_efield(const CentreDev& T, const void* x, int n1, int n2, int n3, const float* s33, const float* r33,
                    void* e, hipStream_t s);
 
+// H1, H2, H3 of one frequency on the model's cell centres: e = [3][nz][ny][nx]
+// complex float, from the magnetic field vector h (complex double)
+void launch_hfield(const CentreDev& T, const void* h, int n1, int n2, int n3, void* e, hipStream_t s);
+
 // b += curl M over the nodes 1..n of each axis (src/inject_extract.c:303-312);
 // m = the dipole vector, d1..d3 the dual widths (n+1 floats each)
 void launch_add_curl_m(void* b, const void* m, int n1, int n2, int n3, const float* d1, const float* d2,

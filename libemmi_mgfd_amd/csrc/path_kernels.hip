@@ -87,6 +87,24 @@ __global__ void k_efield(CentreDev T, const double2* __restrict__ x, long n123, 
   e[2 * nxyz + o] = make_float2((float)ez.re, (float)ez.im);
 }
 
+// H on the model's cell centres (src/inject_extract.c:638-785): trilinear
+// gathers of the three staggered components of h
+__global__ void k_hfield(CentreDev T, const double2* __restrict__ h, long n123, int n1, int n2,
+                         float2* __restrict__ e) {
+#pragma clang fp contract(off)
+  const long nxyz = (long)T.nx * T.ny * T.nz;
+  const long o = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (o >= nxyz) return;
+  const int i = (int)(o % T.nx), j = (int)((o / T.nx) % T.ny), k = (int)(o / ((long)T.nx * T.ny));
+  const long s1 = n1 + 1, s2 = (long)(n1 + 1) * (n2 + 1);
+  cd v = gather8(h, T.i1[i] + s1 * T.i2s[j] + s2 * T.i3s[k], s1, s2, T.w1[i], T.w2s[j], T.w3s[k]);
+  e[o] = make_float2((float)v.re, (float)v.im);
+  v = gather8(h, n123 + T.i1s[i] + s1 * T.i2[j] + s2 * T.i3s[k], s1, s2, T.w1s[i], T.w2[j], T.w3s[k]);
+  e[nxyz + o] = make_float2((float)v.re, (float)v.im);
+  v = gather8(h, 2 * n123 + T.i1s[i] + s1 * T.i2s[j] + s2 * T.i3[k], s1, s2, T.w1s[i], T.w2s[j], T.w3[k]);
+  e[2 * nxyz + o] = make_float2((float)v.re, (float)v.im);
+}
+
 // b += curl M with the dual widths (mu_r = 1): one thread per node, which
 // owns the three entries of b at that node
 __global__ void k_add_curl_m(double2* __restrict__ b, const double2* __restrict__ m, int n1, int n2, int n3,
@@ -177,6 +195,12 @@ void launch_efield(const CentreDev& T, const void* x, int n1, int n2, int n3, co
   const long n123 = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
   hipLaunchKernelGGL(k_efield, dim3(nb(nxyz)), dim3(256), 0, s, T, (const double2*)x, n123, n1, n2, s33, r33,
                      (float2*)e);
+}
+
+void launch_hfield(const CentreDev& T, const void* h, int n1, int n2, int n3, void* e, hipStream_t s) {
+  const long nxyz = (long)T.nx * T.ny * T.nz;
+  const long n123 = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  hipLaunchKernelGGL(k_hfield, dim3(nb(nxyz)), dim3(256), 0, s, T, (const double2*)h, n123, n1, n2, (float2*)e);
 }
 
 void launch_add_curl_m(void* b, const void* m, int n1, int n2, int n3, const float* d1, const float* d2,

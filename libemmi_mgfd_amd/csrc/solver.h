@@ -36,6 +36,7 @@ struct SolveStats {
   int status = 0;        // 0 converged, 1 not converged, 2 non-finite residual
   double rnorm0 = 0, rnorm = 0;
   int x0_rejected = 0;   // the initial guess lost to the zero guess and was dropped
+  double x0_rnorm = 0;   // ||f - A x0|| of the initial guess (0 without one)
   std::vector<double> history;
 };
 
@@ -52,14 +53,14 @@ struct DevLevel {
   int n[3] = {0, 0, 0};
   int sc[3] = {1, 1, 1};
   long nc = 0;
-  T* geo = nullptr;   // per axis a, at offset 5*a*(nmax+1): W, IW, H, X, D (n_a+1 each)
+  double* geo = nullptr;   // per axis a, at offset 5*a*(nmax+1): W, IW, H, X, D (n_a+1 each)
   int gstride = 0;
-  T* wr = nullptr;    // weights of the finer level towards this one (3 x finer nmax+1)
+  double* wr = nullptr;    // weights of the finer level towards this one (3 x finer nmax+1)
   int wstride = 0;
   T* sbar = nullptr;  // 3*nc
   Level<T> view(double omu) const;
   Transfer<T> transfer() const;
-  const T* D(int a) const { return geo + (5 * a + 4) * gstride; }
+  const double* D(int a) const { return geo + (5 * a + 4) * gstride; }
 };
 
 template <typename T>
@@ -86,7 +87,6 @@ struct Slot {
   double* partial = nullptr;         // residual-norm partial sums
   double rnorm0 = 0;
   SolveStats* st = nullptr;
-  bool x0_live = false;              // started from a caller's guess, not yet judged
   double seen_omu[3] = {-1.0, -1.0, -1.0};  // frequency whose factors a cycle of pattern p has built
 };
 
@@ -173,6 +173,7 @@ class Solver {
   std::vector<int> act_;      // active slots
   int icycle_ = 0;
   bool r0_ready_ = false;     // r[0] holds f - A u of the level-0 test (FP64 form)
+  bool rnorm0_in_cycle_ = false;  // rnorm0 is the cycle-0 in-cycle norm (FP64 V/W cycles)
   // captured cycle bodies, keyed by pattern and active-slot mask; valid for
   // the slots' frequencies at capture time
   std::map<std::pair<int, unsigned>, hipGraphExec_t> graphs_;

@@ -254,36 +254,37 @@ void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cel
       if (c > 0 && lev == 0) { L = H.pat[0][0]; continue; }
       const int nmax = std::max(n[0], std::max(n[1], n[2]));
       L.gstride = nmax + 1;
-      std::vector<T> g(15 * (size_t)L.gstride, T(0));
+      // 1D geometry in FP64 for every storage precision (see Level)
+      std::vector<double> g(15 * (size_t)L.gstride, 0.0);
       for (int a = 0; a < 3; a++) {
         const HostAxis& ax = axes_[c][lev][a];
         for (int i = 0; i <= n[a]; i++) {
-          g[(5 * a + 0) * L.gstride + i] = (T)ax.ds[i];
-          g[(5 * a + 1) * L.gstride + i] = (T)(1.0 / ax.ds[i]);
-          g[(5 * a + 2) * L.gstride + i] = i > 0 ? (T)(0.5 * (ax.ds[i] + ax.ds[i - 1])) : T(0);
-          g[(5 * a + 3) * L.gstride + i] = (T)ax.x[i];
-          g[(5 * a + 4) * L.gstride + i] = (T)ax.d[i];
+          g[(5 * a + 0) * L.gstride + i] = ax.ds[i];
+          g[(5 * a + 1) * L.gstride + i] = 1.0 / ax.ds[i];
+          g[(5 * a + 2) * L.gstride + i] = i > 0 ? 0.5 * (ax.ds[i] + ax.ds[i - 1]) : 0.0;
+          g[(5 * a + 3) * L.gstride + i] = ax.x[i];
+          g[(5 * a + 4) * L.gstride + i] = ax.d[i];
         }
       }
-      L.geo = (T*)dalloc(sizeof(T) * g.size(), owned_);
-      HIPCHECK(hipMemcpyAsync(L.geo, g.data(), sizeof(T) * g.size(), hipMemcpyHostToDevice, s_));
+      L.geo = (double*)dalloc(sizeof(double) * g.size(), owned_);
+      HIPCHECK(hipMemcpyAsync(L.geo, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, s_));
       if (lev > 0) {
         // weights of the finer level's nodes towards this level (src/gmg.c:1478-1488)
         const auto& nf = dims_[c][lev - 1];
         const int nfmax = std::max(nf[0], std::max(nf[1], nf[2]));
         L.wstride = nfmax + 1;
-        std::vector<T> w(3 * (size_t)L.wstride, T(0));
+        std::vector<double> w(3 * (size_t)L.wstride, 0.0);
         for (int a = 0; a < 3; a++) {
           const HostAxis& fa = axes_[c][lev - 1][a];
           const HostAxis& ca = axes_[c][lev][a];
           for (int q = 0; q <= nf[a]; q++) {
             const int i = L.sc[a] == 2 ? q / 2 : q;
             if (i + 1 > n[a]) continue;
-            w[a * L.wstride + q] = (T)((fa.x[q] - ca.x[i]) / (ca.x[i + 1] - ca.x[i]));
+            w[a * L.wstride + q] = (fa.x[q] - ca.x[i]) / (ca.x[i + 1] - ca.x[i]);
           }
         }
-        L.wr = (T*)dalloc(sizeof(T) * w.size(), owned_);
-        HIPCHECK(hipMemcpyAsync(L.wr, w.data(), sizeof(T) * w.size(), hipMemcpyHostToDevice, s_));
+        L.wr = (double*)dalloc(sizeof(double) * w.size(), owned_);
+        HIPCHECK(hipMemcpyAsync(L.wr, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, s_));
       }
       L.sbar = (T*)dalloc(sizeof(T) * 3 * L.nc, owned_);
       const auto& cl = cells[c * lmax_ + lev];
@@ -447,22 +448,9 @@ bool Solver::check_active(int pat) {
   std::vector<double> rn;
   norms(act_, u.data(), f.data(), r.data(), rn);
   std::vector<int> keep;
-  const long nv = 3 * nc0_;
   for (size_t i = 0; i < act_.size(); i++) {
     Slot& S = slots_[act_[i]];
     SolveStats* st = S.st;
-    if (S.x0_live && (icycle_ >= 1 || !std::isfinite(rn[i]))) {
-      // The guess is judged at the first test after a cycle from it: when
-      // its residual is not below the zero guess's (||f||), the slot
-      // restarts from zero; the cycle spent on it still counts.
-      S.x0_live = false;
-      if (!(rn[i] <= S.rnorm0)) {
-        launch_zero<double>(S.u64, nv, s_);
-        HIPCHECK(hipMemcpyAsync(S.r64, S.f64, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
-        st->x0_rejected = 1;
-        rn[i] = S.rnorm0;
-      }
-    }
     st->history.push_back(rn[i]);
     if (p_.verb) printf("slot=%d icycle=%d rnorm=%e\n", act_[i], icycle_, rn[i]);
     if (!std::isfinite(rn[i])) {
@@ -471,6 +459,15 @@ bool Solver::check_active(int pat) {
       continue;
     }
     st->rnorm = rn[i];
+    if (icycle_ == 0 && rnorm0_in_cycle_) {
+      // FP64 V- and W-cycles: the reference takes rnorm0 from the first
+      // in-cycle norm, after the pre-smoothing of cycle 0, and tests from
+      // cycle 1 on (src/gmg.c:1594-1599, 1651-1656)
+      S.rnorm0 = rn[i];
+      st->rnorm0 = rn[i];
+      keep.push_back(act_[i]);
+      continue;
+    }
     if (rn[i] < S.rnorm0 * p_.tol) {
       st->cycles = icycle_;
       continue;
@@ -696,12 +693,38 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
       slots_[act_[i]].st->rnorm0 = rn[i];
     }
   }
-  for (int k = 0; k < n; k++) {
-    slots_[k].x0_live = x0 && x0[k];
-    if (slots_[k].x0_live) HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+  // Initial guesses are judged before any cycle: a guess whose residual
+  // ||f - A x0|| is not below the zero guess's ||f|| (or is not finite) is
+  // dropped and the slot starts from zero.  The stopping test stays relative
+  // to ||f|| either way.
+  {
+    std::vector<int> g;
+    std::vector<const void*> u, f;
+    std::vector<void*> r;
+    for (int k = 0; k < n; k++)
+      if (x0 && x0[k]) {
+        HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
+        g.push_back(k);
+        u.push_back(slots_[k].u64);
+        f.push_back(slots_[k].f64);
+        r.push_back(slots_[k].r64);
+      }
+    if (!g.empty()) {
+      std::vector<double> rn;
+      norms(g, u.data(), f.data(), r.data(), rn);
+      for (size_t i = 0; i < g.size(); i++) {
+        Slot& S = slots_[g[i]];
+        S.st->x0_rnorm = rn[i];
+        if (!(rn[i] <= S.rnorm0)) {
+          launch_zero<double>(S.u64, nv, s_);
+          S.st->x0_rejected = 1;
+        }
+      }
+    }
   }
 
   const int kind = p_.cycleopt;
+  rnorm0_in_cycle_ = p_.precision == 0 && kind != 2;
   for (icycle_ = 0; icycle_ < p_.ncycle && !act_.empty(); icycle_++) {
     const int pat = p_.isemicoarsen ? icycle_ % 3 : 0;
     const bool test_first = p_.precision != 0 || (kind == 2 && coarsens<double>(pat, 0));

@@ -26,7 +26,7 @@ class Params(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("status", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
-                ("x0_rejected", C.c_int)]
+                ("x0_rejected", C.c_int), ("x0_rnorm", C.c_double)]
 
 
 STATUS = {0: "converged", 1: "not converged", 2: "non-finite residual", -1: "error"}
@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
         L.emmi_inject_adj_source_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_extract_electric_field_dev.restype = C.c_int
         L.emmi_extract_electric_field_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.emmi_extract_magnetic_field_dev.restype = C.c_int
+        L.emmi_extract_magnetic_field_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_build_gradient_dev.restype = C.c_int
         L.emmi_build_gradient_dev.argtypes = [PA, PE, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                               C.c_void_p, C.c_void_p, C.c_void_p]

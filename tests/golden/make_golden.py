@@ -7,12 +7,20 @@ synthetic surveys of libemmi_mgfd_amd.configs and keeps only small outputs:
   tiny_units.npz   kernel known answers on a 8x6x14 grid (inputs and outputs of
                    residual, line/point Gauss-Seidel, transfer, curl)
   tiny_model.npz   the 12x10x16 two-frequency survey: grid, indices, source
-                   term, E/H of the first frequency, receiver data and residual
+                   term, E/H of the first frequency, receiver data, the E and
+                   H volumes on the input grid (icheck=1: extract_electric_field
+                   and extract_magnetic_field, both frequencies) and residual
                    histories at the default tolerance and at tol=1e-10
+  tiny_cycles.npz  the same survey solved with the V-cycle (cycleopt=1), the
+                   W-cycle (cycleopt=3) and the F-cycle without
+                   semi-coarsening (isemicoarsen=0: point Gauss-Seidel on
+                   every level): E/H of the first frequency and residual
+                   histories
   tiny_emf_0001.txt the reference's receiver-data text file for that survey
   tiny_grad.npz    one mode-2 gradient (forward + adjoint) on the same survey
   config1.npz      grid arrays, medium hashes, receiver data and residual
-                   histories of the 64^3 survey (default and tol=1e-10)
+                   histories of the 64^3 survey (default and tol=1e-10), and
+                   the y = 0 planes of its E and H volumes (icheck=1)
   config2.npz      (with --config2) the 128^3 three-frequency survey: grid,
                    medium hashes, receiver data and residual histories at the
                    default tolerance and at tol=1e-10
@@ -98,8 +106,10 @@ def tiny_units(tmp):
 
 def tiny_model(tmp):
     s = configs.tiny()
-    d, hist, _ = run(s, "model", os.path.join(tmp, "model"), ["dumpfields=1"])
+    d, hist, _ = run(s, "model", os.path.join(tmp, "model"), ["dumpfields=1", "icheck=1"])
     out = {"n": np.array([s.n1, s.n2, s.n3])}
+    for k in ("E1", "E2", "E3", "H1", "H2", "H3"):
+        out[k] = ld(d, k, np.complex64)   # [nfreq][nz][ny][nx]
     out.update(grid_dict(d, s))
     out["rec_idx"] = ld(d, "rec_idx", np.int32)
     out["rec_w"] = ld(d, "rec_w", np.float32)
@@ -118,6 +128,23 @@ def tiny_model(tmp):
     out["hist_tight"] = pad_hist(hist2)
     np.savez_compressed(os.path.join(HERE, "tiny_model.npz"), **out)
     return out
+
+
+CYCLE_VARIANTS = {"v": ["cycleopt=1"], "w": ["cycleopt=3"], "f_point": ["cycleopt=2", "isemicoarsen=0"]}
+
+
+def tiny_cycles(tmp):
+    """The other cycle types and the point smoother (src/gmg.c:1586-1668,
+    116-336, 1301-1313) on the tiny survey, default tolerance."""
+    s = configs.tiny()
+    out = {}
+    for name, extra in CYCLE_VARIANTS.items():
+        d, hist, _ = run(s, "model", os.path.join(tmp, "cyc_" + name), ["dumpfields=1"] + extra)
+        out[f"{name}_E_f0"] = ld(d, "E_f0")
+        out[f"{name}_H_f0"] = ld(d, "H_f0")
+        out[f"{name}_hist"] = pad_hist(hist)
+        out[f"{name}_dcal"] = ld(d, "dcal", np.complex64)
+    np.savez_compressed(os.path.join(HERE, "tiny_cycles.npz"), **out)
 
 
 def tiny_grad(tmp, model):
@@ -152,8 +179,11 @@ def tiny_grad(tmp, model):
 
 def config1(tmp):
     s = configs.config1()
-    d, hist, _ = run(s, "model", os.path.join(tmp, "c1"))
+    d, hist, _ = run(s, "model", os.path.join(tmp, "c1"), ["icheck=1"])
     out = {"n": np.array([s.n1, s.n2, s.n3])}
+    for k in ("E1", "E2", "E3", "H1", "H2", "H3"):
+        # the y = 0 plane (iy = ny // 2) of the one frequency: [nz][nx]
+        out[k + "_y0"] = ld(d, k, np.complex64).reshape(s.nz, s.ny, s.nx)[:, s.ny // 2, :].copy()
     g = grid_dict(d, s)
     for k in GRID_KEYS:
         out[k] = g[k]
@@ -198,6 +228,7 @@ def main():
             return
         tiny_units(tmp)
         model = tiny_model(tmp)
+        tiny_cycles(tmp)
         tiny_grad(tmp, model)
         if "--skip-config1" not in sys.argv:
             config1(tmp)

@@ -1,10 +1,12 @@
-"""The device-resident steps of the gradient path against the product's host
-entry points, which are themselves bit-exact with the reference's fixtures
-(test_host_path.py): the adjoint source (src/inject_extract.c:124-314), E on
-the model's cell centres (src/inject_extract.c:477-635) and the gradient with
-its pseudo-Hessian and mutes (src/build_gradient.c:13-111).  The adjoint
-source, E and the gradient must match bit for bit; the pseudo-Hessian goes
-through the device hypot and is held to float rounding (2 ulp)."""
+"""The device-resident steps of the gradient path against the reference's
+fixtures and the product's host entry points (themselves bit-exact with the
+fixtures, test_host_path.py): the adjoint source
+(src/inject_extract.c:124-314), E and H on the model's cell centres
+(src/inject_extract.c:477-785, the reference's icheck=1 volumes) and the
+gradient with its pseudo-Hessian and mutes (src/build_gradient.c:13-111).
+The adjoint source, E, H and the gradient must match bit for bit; the
+pseudo-Hessian goes through the device hypot and is held to float rounding
+(2 ulp)."""
 import ctypes as C
 import os
 
@@ -51,8 +53,30 @@ def test_extract_electric_field_dev_matches_host():
     print(f"E on the model grid, tiny survey {s.nx}x{s.ny}x{s.nz}", flush=True)
     _check(native.lib().emmi_extract_electric_field_dev(sc.acq, sc.emf, 0, xd.ptr, ed.ptr, None))
     e = ed.download(np.complex64, (3, s.nz, s.ny, s.nx))
+    shp = (len(s.freqs), s.nz, s.ny, s.nx)
     for c in range(3):
         np.testing.assert_array_equal(e[c], arrs[f"E{c + 1}"][0], err_msg=f"E{c + 1}")
+        np.testing.assert_array_equal(e[c], d[f"E{c + 1}"].reshape(shp)[0], err_msg=f"E{c + 1} vs reference")
+    assert np.abs(e).max() > 0
+
+
+def test_extract_magnetic_field_dev_matches_reference_fixture():
+    d = np.load(os.path.join(GOLD, "tiny_model.npz"))
+    s = configs.tiny()
+    sc = regridded(s)
+    arrs = _field_tables(sc, s, ["H1", "H2", "H3"])
+    h = np.ascontiguousarray(d["H_f0"])
+    native.lib().extract_magnetic_field(sc.acq, sc.emf, h.ctypes.data, 0)
+    hd = native.DeviceBuffer(h.nbytes).upload(h)
+    nxyz = s.nx * s.ny * s.nz
+    ed = native.DeviceBuffer(3 * nxyz * 8)
+    print(f"H on the model grid, tiny survey {s.nx}x{s.ny}x{s.nz}", flush=True)
+    _check(native.lib().emmi_extract_magnetic_field_dev(sc.acq, sc.emf, 0, hd.ptr, ed.ptr, None))
+    e = ed.download(np.complex64, (3, s.nz, s.ny, s.nx))
+    shp = (len(s.freqs), s.nz, s.ny, s.nx)
+    for c in range(3):
+        np.testing.assert_array_equal(e[c], d[f"H{c + 1}"].reshape(shp)[0], err_msg=f"H{c + 1} vs reference")
+        np.testing.assert_array_equal(e[c], arrs[f"H{c + 1}"][0], err_msg=f"H{c + 1}")
     assert np.abs(e).max() > 0
 
 

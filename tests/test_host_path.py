@@ -93,8 +93,10 @@ def _field_tables(sc, s, names):
     return arrs
 
 
-def test_extract_fields_match_oracle():
-    """E and H on the input-grid cell centres (src/inject_extract.c:477-785)."""
+def test_extract_fields_match_reference_fixture():
+    """E and H on the input-grid cell centres (src/inject_extract.c:477-785):
+    the reference's own icheck=1 volumes of the fixture's first frequency,
+    and the oracle."""
     d = np.load(os.path.join(GOLD, "tiny_model.npz"))
     s = configs.tiny()
     sc = regridded(s)
@@ -106,7 +108,10 @@ def test_extract_fields_match_oracle():
     mg = O.mg_struct(O.regrid(s), sc.emf.rho_water)
     Eo = O.extract_efield(mg, s, E)
     Ho = O.extract_hfield(mg, s, H)
+    shp = (len(s.freqs), s.nz, s.ny, s.nx)
     for i in range(3):
+        np.testing.assert_array_equal(arrs[f"E{i + 1}"][0], d[f"E{i + 1}"].reshape(shp)[0])
+        np.testing.assert_array_equal(arrs[f"H{i + 1}"][0], d[f"H{i + 1}"].reshape(shp)[0])
         np.testing.assert_array_equal(arrs[f"E{i + 1}"][0], Eo[i])
         np.testing.assert_array_equal(arrs[f"H{i + 1}"][0], Ho[i])
     assert np.abs(arrs["E1"][0]).max() > 0

@@ -4,8 +4,11 @@ multigrid path) against the golden fixtures that the compiled reference wrote
 restatement keeps the reference's float/double evaluation order.
 
 Fixtures: tiny_units.npz (single kernels on an 8x6x14 grid), tiny_model.npz
-(12x10x16 two-frequency survey end to end), tiny_emf_0001.txt, tiny_grad.npz
-(mode-2 gradient), config1.npz (64^3 survey: grid, medium, receiver data).
+(12x10x16 two-frequency survey end to end, with its E and H volumes on the
+input grid), tiny_cycles.npz (the same survey with V- and W-cycles and with
+point smoothing on full coarsening), tiny_emf_0001.txt, tiny_grad.npz (mode-2
+gradient), config1.npz (64^3 survey: grid, medium, receiver data, the y = 0
+planes of its E and H volumes).
 """
 import os
 
@@ -121,6 +124,42 @@ def test_model_solve_bitexact(model):
     np.testing.assert_array_equal(dc, ref)
 
 
+def test_model_field_volumes_bitexact(model):
+    """E and H on the input-grid cell centres from the reference's first
+    frequency fields (src/inject_extract.c:477-635, 638-785, the icheck=1
+    volumes of src/do_modelling.c:58-61)."""
+    d, s, _, mg = model
+    E = O.extract_efield(mg, s, d["E_f0"])
+    H = O.extract_hfield(mg, s, d["H_f0"])
+    shp = (len(s.freqs), s.nz, s.ny, s.nx)
+    for q in range(3):
+        np.testing.assert_array_equal(E[q], d[f"E{q + 1}"].reshape(shp)[0], err_msg=f"E{q + 1}")
+        np.testing.assert_array_equal(H[q], d[f"H{q + 1}"].reshape(shp)[0], err_msg=f"H{q + 1}")
+    # the volumes are not trivially zero or constant
+    assert np.abs(E[0]).max() > 0 and np.abs(H[1]).max() > 0
+    assert np.unique(np.abs(E[0])).size > 100
+
+
+CYCLES = {"v": dict(cycleopt=1), "w": dict(cycleopt=3), "f_point": dict(cycleopt=2, isemicoarsen=0)}
+
+
+@pytest.mark.parametrize("name", list(CYCLES))
+def test_model_cycles_bitexact(model, name):
+    """V-cycle, W-cycle, and the F-cycle with full coarsening and the
+    vertex-patch point smoother (src/gmg.c:1586-1668, 116-336, 1301-1313):
+    residual history (printed digits), E and H equal the reference's."""
+    d, s, g, mg = model
+    cy = load("tiny_cycles.npz")
+    n = (s.n1, s.n2, s.n3)
+    b = O.inject_source(mg, s.sources, s.chsrc, s.freqs[0])
+    E, H, hist = O.gmg_solve(n, g["x1"], g["x2"], g["x3"], g["sigma11"], g["sigma22"], g["sigma33"],
+                             s.freqs[0], b, **CYCLES[name])
+    ref_hist = cy[f"{name}_hist"][0]
+    np.testing.assert_allclose(np.asarray(hist), ref_hist[np.isfinite(ref_hist)], rtol=5e-7, atol=0)
+    np.testing.assert_array_equal(E, cy[f"{name}_E_f0"])
+    np.testing.assert_array_equal(H, cy[f"{name}_H_f0"])
+
+
 def test_model_emf_text_file(model):
     """The reference's receiver-data file holds the same numbers (printed
     with %e) as the binary dump of dcal."""
@@ -215,16 +254,35 @@ def test_config1_grid_and_medium(c1):
         assert h == str(d[k + "_sha256"]), k
 
 
-def test_config1_receiver_data_bitexact(c1):
-    """64^3 survey end to end: regrid, injection, F-cycle solve to 1e-6 and
-    extraction reproduce the reference's receiver data bit for bit."""
+@pytest.fixture(scope="module")
+def c1_solve(c1):
     d, s, g = c1
     n = (s.n1, s.n2, s.n3)
     rw = float(min(s.rho11.min(), s.rho22.min(), s.rho33.min()))
     mg = O.mg_struct(g, rw)
     b = O.inject_source(mg, s.sources, s.chsrc, s.freqs[0])
     E, H, hist = O.gmg_solve(n, g["x1"], g["x2"], g["x3"], g["sigma11"], g["sigma22"], g["sigma33"], s.freqs[0], b)
+    return mg, E, H, hist
+
+
+def test_config1_receiver_data_bitexact(c1, c1_solve):
+    """64^3 survey end to end: regrid, injection, F-cycle solve to 1e-6 and
+    extraction reproduce the reference's receiver data bit for bit."""
+    d, s, g = c1
+    mg, E, H, hist = c1_solve
     ref_hist = d["hist"][0]
     np.testing.assert_allclose(np.asarray(hist), ref_hist[np.isfinite(ref_hist)], rtol=5e-7, atol=0)
     dc = O.extract_data(mg, s.receivers, s.chrec, E, H)
     np.testing.assert_array_equal(dc, d["dcal"].reshape(len(s.chrec), 1, -1)[:, 0])
+
+
+def test_config1_field_volumes_bitexact(c1, c1_solve):
+    """The 64^3 survey's E and H volumes on the input grid (icheck=1),
+    compared on the y = 0 plane, where the source and receivers lie."""
+    d, s, g = c1
+    mg, E, H, _ = c1_solve
+    Ev = O.extract_efield(mg, s, E)
+    Hv = O.extract_hfield(mg, s, H)
+    for q in range(3):
+        np.testing.assert_array_equal(Ev[q][:, s.ny // 2, :], d[f"E{q + 1}_y0"], err_msg=f"E{q + 1}")
+        np.testing.assert_array_equal(Hv[q][:, s.ny // 2, :], d[f"H{q + 1}_y0"], err_msg=f"H{q + 1}")

@@ -231,10 +231,13 @@ def dense_operator(L, omu):
     return A, idx
 
 
-def test_solve_matches_dense_direct_solution():
+@pytest.mark.parametrize("cycleopt,isemi", [(1, 1), (2, 1), (3, 1), (1, 0), (2, 0), (3, 0)])
+def test_solve_matches_dense_direct_solution(cycleopt, isemi):
     """Brute force: the multigrid solve to 1e-12 equals A^{-1} (D b) from a
     dense LU, D the dual-volume scaling of the right-hand side
-    (src/gmg.c:1899-1910)."""
+    (src/gmg.c:1899-1910), for the V-, F- and W-cycles (src/gmg.c:1586-1668)
+    with semi-coarsening and line smoothing and with full coarsening and the
+    vertex-patch point smoother (src/gmg.c:116-336, 1301-1313)."""
     n = (4, 4, 6)
     x = grid(n, 5)
     s = medium(n, 6)
@@ -251,8 +254,9 @@ def test_solve_matches_dense_direct_solution():
     k, j, i = np.meshgrid(np.arange(n[2] + 1), np.arange(n[1] + 1), np.arange(n[0] + 1), indexing="ij")
     scale = np.stack([W[0][i] * D[1][j] * D[2][k], D[0][i] * W[1][j] * D[2][k], D[0][i] * D[1][j] * W[2][k]]).reshape(-1)
     xd = np.linalg.solve(A, (bvec * scale)[idx])
-    E, H, hist = O.gmg_solve(n, *x, *s, freq, bvec, tol=1e-13, ncycle=400)
-    assert hist[-1] < 1e-12 * hist[0]
+    E, H, hist = O.gmg_solve(n, *x, *s, freq, bvec, tol=1e-13, ncycle=3000, cycleopt=cycleopt,
+                             isemicoarsen=isemi)
+    assert hist[-1] < 1e-12 * hist[0], (len(hist), hist[-1] / hist[0])
     assert np.abs(E[idx] - xd).max() <= 1e-8 * np.abs(xd).max()
     assert np.abs(np.delete(E, idx)).max() == 0.0
 
