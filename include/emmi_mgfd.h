@@ -212,8 +212,10 @@ void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
  * u = in1 with f = in2; op 2 restriction of in1 from level lev to lev+1;
  * op 3 prolongation: out = in2 + P in1 (in1 on level lev+1, in2 on lev);
  * op 4 H = curl in1 / (i omega mu0) (level 0); op 5 coarsest-level smoothing
- * of u = in1 with f = in2.  `precision` selects the FP64 or FP32 kernels (the
- * solver must have been created with that precision). */
+ * of u = in1 with f = in2; op 6 one vertex-patch Gauss-Seidel sweep (8 node
+ * colours, reversed when iter is odd) of u = in1 with f = in2.  `precision`
+ * selects the FP64 or FP32 kernels (the solver must have been created with
+ * that precision). */
 int emmi_solver_check_op(emmi_solver_t *s, int op, int pattern, int lev, int axis, int iter,
                          const emmi_c128 *in1, const emmi_c128 *in2, emmi_c128 *out);
 

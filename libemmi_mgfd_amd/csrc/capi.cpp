@@ -256,7 +256,7 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
   const DevLevel<T>& L = H.pat[pat][lev];
   const Level<T> V = L.view(S.omu());
   const long nv = 3 * L.nc;
-  if (op == 0 || op == 1 || op == 5) {
+  if (op == 0 || op == 1 || op == 5 || op == 6) {
     DevVec<T> u(nv), f(nv), r(nv);
     u.up(in1);
     f.up(in2);
@@ -264,6 +264,13 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
       launch_residual<T>(V, u.p, f.p, r.p, nullptr, s);
       check_hip(hipStreamSynchronize(s), "sync");
       r.down(out);
+    } else if (op == 6) {
+      void* uu[1] = {u.p};
+      const void* ff[1] = {f.p};
+      const double om[1] = {S.omu()};
+      launch_point_gs_batch<T>(V, 1, uu, ff, om, iter, s);
+      check_hip(hipStreamSynchronize(s), "sync");
+      u.down(out);
     } else {
       S.line_sweep_on<T>(pat, lev, axis, iter, u.p, f.p);
       check_hip(hipStreamSynchronize(s), "sync");

@@ -32,6 +32,11 @@ void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, d
 template <typename T>
 void launch_residual_batch(const Level<T>& L, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s);
+// one vertex-patch Gauss-Seidel sweep (src/gmg.c:116-336) of u_k against f_k
+// for n slots of one level: 8 node colours, reversed for odd iter
+template <typename T>
+void launch_point_gs_batch(const Level<T>& L, int n, void* const* u, const void* const* f, const double* omu,
+                           int iter, hipStream_t s);
 // fc_k = R rf_k, uc_k zeroed, for n slots, one launch
 template <typename T>
 void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,

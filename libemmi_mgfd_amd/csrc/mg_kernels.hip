@@ -70,6 +70,88 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, dou
   if (partial) block_sum_store(acc, partial, blk);
 }
 
+// ---------------------------------------------------------------------------
+// vertex-patch Gauss-Seidel (src/gmg.c:116-336, lu_solve :39-66)
+//
+// The six edges incident to an interior node (i, j, k) -- E_a(node - e_a)
+// and E_a(node) for a = x, y, z -- are solved together: u_patch +=
+// A_patch^-1 (f - A u)_patch.  A_patch couples each edge with itself and with
+// the perpendicular edges of the patch (collinear edges share no face): for
+// row a (p = node - e_a for the lower edge, node for the upper) and the
+// b-edges, +-H_c(node_c) W_a(p_a)/W_a(p_a), for the c-edges +-H_b(node_b)
+// W_a/W_a, negative when both edges lie on the same side of the node; the
+// diagonal is the row's own coefficient of apply_row.  Nodes are swept in 8
+// colours (i%2, j%2, k%2): two nodes of one colour differ by 2 along some
+// axis, so their patches neither overlap nor couple, and a colour is one
+// parallel Gauss-Seidel step.  The 6x6 complex system is eliminated without
+// pivoting, as the reference's lu_solve, in FP64 for both storage precisions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int parity_count(int n, int par, int& first) {
+  first = par ? 1 : 2;   // nodes 1..n-1 with i % 2 == par
+  return first <= n - 1 ? (n - 1 - first) / 2 + 1 : 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_point_gs(Level<T> L, SlotVecs<T> V, int colour) {
+  const int sl = blockIdx.y;
+  L.omu = V.omu[sl];
+  typename cvec<T>::type* __restrict__ u = V.out0[sl];
+  const typename cvec<T>::type* __restrict__ f = V.in1[sl];
+  int f1, f2, f3;
+  const int m1 = parity_count(L.n[0], colour & 1, f1);
+  const int m2 = parity_count(L.n[1], (colour >> 1) & 1, f2);
+  const int m3 = parity_count(L.n[2], (colour >> 2) & 1, f3);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)m1 * m2 * m3) return;
+  const int node[3] = {f1 + 2 * (int)(t % m1), f2 + 2 * (int)((t / m1) % m2), f3 + 2 * (int)(t / ((long)m1 * m2))};
+  const int i = node[0], j = node[1], k = node[2];
+  const long o = i + L.es[1] * j + L.es[2] * k;
+  // patch edges: 2a = E_a(node - e_a), 2a + 1 = E_a(node)
+  long off[6];
+  for (int a = 0; a < 3; a++) {
+    off[2 * a] = a * L.nc + o - L.es[a];
+    off[2 * a + 1] = a * L.nc + o;
+  }
+  cxd r[6];
+  r[0] = ld<T>(f, off[0]) - apply_row<T, 0>(L, u, i - 1, j, k, o - 1);
+  r[1] = ld<T>(f, off[1]) - apply_row<T, 0>(L, u, i, j, k, o);
+  r[2] = ld<T>(f, off[2]) - apply_row<T, 1>(L, u, j - 1, k, i, o - L.es[1]);
+  r[3] = ld<T>(f, off[3]) - apply_row<T, 1>(L, u, j, k, i, o);
+  r[4] = ld<T>(f, off[4]) - apply_row<T, 2>(L, u, k - 1, i, j, o - L.es[2]);
+  r[5] = ld<T>(f, off[5]) - apply_row<T, 2>(L, u, k, i, j, o);
+  cxd A[6][6];
+  for (int m = 0; m < 6; m++) {
+    const int a = m >> 1, lo = (m & 1) == 0;
+    const int b = ax_b(a), c = ax_c(a);
+    const int pa = node[a] - lo;
+    const double Wa = L.W[a][pa], w = Wa * L.IW[a][pa];
+    const double Hb = L.H[b][node[b]], Hc = L.H[c][node[c]];
+    const double dre = Wa * (Hc * (L.IW[b][node[b]] + L.IW[b][node[b] - 1]) + Hb * (L.IW[c][node[c]] + L.IW[c][node[c] - 1]));
+    A[m][m] = cxd(dre, -L.omu * (double)L.sbar[off[m]]);
+    A[m][m ^ 1] = cxd(0.0, 0.0);
+    for (int q = 0; q < 2; q++) {
+      const double sg = (lo == (q == 0)) ? -1.0 : 1.0;   // same side of the node: negative
+      A[m][2 * b + q] = cxd(sg * Hc * w, 0.0);
+      A[m][2 * c + q] = cxd(sg * Hb * w, 0.0);
+    }
+  }
+  // A x = r by elimination without pivoting
+  for (int p = 0; p < 6; p++) {
+    const cxd piv = cinv(A[p][p]);
+    for (int q = p + 1; q < 6; q++) {
+      const cxd l = A[q][p] * piv;
+      for (int m = p + 1; m < 6; m++) A[q][m] -= l * A[p][m];
+      r[q] -= l * r[p];
+    }
+  }
+  for (int p = 5; p >= 0; p--) {
+    cxd v = r[p];
+    for (int m = p + 1; m < 6; m++) v -= A[p][m] * r[m];
+    r[p] = v * cinv(A[p][p]);
+  }
+  for (int m = 0; m < 6; m++) st<T>(u, off[m], ld<T>(u, off[m]) + r[m]);
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
   double v = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
@@ -329,6 +411,33 @@ void launch_sum_partials(const double* partial, int n, double* out, hipStream_t 
 }
 
 template <typename T>
+void launch_point_gs_batch(const Level<T>& L, int n, void* const* u, const void* const* f, const double* omu,
+                           int iter, hipStream_t s) {
+  using V = typename cvec<T>::type;
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("point smoother batch of 1..8 slots");
+  SlotVecs<T> B = {};
+  for (int k = 0; k < n; k++) {
+    B.out0[k] = (V*)u[k];
+    B.in1[k] = (const V*)f[k];
+    B.omu[k] = omu[k];
+  }
+  for (int q = 0; q < 8; q++) {
+    const int colour = (iter % 2 == 1) ? 7 - q : q;
+    long m = 1;
+    for (int a = 0; a < 3; a++) {
+      const int par = (colour >> a) & 1, first = par ? 1 : 2;
+      m *= first <= L.n[a] - 1 ? (L.n[a] - 1 - first) / 2 + 1 : 0;
+    }
+    if (m == 0) continue;
+    const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
+    // per node: the six patch edges of u read and written, their f and sbar
+    // read; the stencil neighbours are the other colours' patch edges
+    timing::Scope ts(timing::OTHER, s, n * (double)m * 6.0 * (3 * sz + sr));
+    hipLaunchKernelGGL(k_point_gs<T>, dim3(nblk(m, 128), n), dim3(128), 0, s, L, B, colour);
+  }
+}
+
+template <typename T>
 void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,
                            void* const* fc, void* const* uc, hipStream_t s) {
   using V = typename cvec<T>::type;
@@ -418,6 +527,8 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
                                          const void* const*, void* const*, void* const*, hipStream_t);         \
   template void launch_prolong_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, int,             \
                                         const void* const*, void* const*, hipStream_t);                        \
+  template void launch_point_gs_batch<T>(const Level<T>&, int, void* const*, const void* const*, const double*,  \
+                                         int, hipStream_t);                                                    \
   template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, void*, \
                                    hipStream_t);                                                               \
   template void launch_prolong<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*,  \

@@ -137,6 +137,7 @@ class Solver {
   template <typename T> void alloc_vectors(Slot& S, const Hier<T>& H);
   template <typename T> void smooth(int pat, int lev, int iter);
   template <typename T> void line_sweep(int pat, int lev, int axis, int iter);
+  template <typename T> void point_sweep(int pat, int lev, int iter);
   template <typename T> bool coarsens(int pat, int lev);
   bool exact_coarsest(int pat) const;
   template <typename T> LineFactors& factors(Slot& S, int pat, int lev, int axis);

@@ -406,7 +406,30 @@ void Solver::line_sweep(int pat, int lev, int axis, int iter) {
   }
 }
 
-// src/gmg.c:1301-1313
+// one vertex-patch Gauss-Seidel sweep for every active slot
+template <typename T>
+void Solver::point_sweep(int pat, int lev, int iter) {
+  const DevLevel<T>& L = hier<T>().pat[pat][lev];
+  for (size_t i0 = 0; i0 < act_.size(); i0 += MG_BATCH_MAX) {
+    const int n = (int)std::min<size_t>(MG_BATCH_MAX, act_.size() - i0);
+    void* u[MG_BATCH_MAX];
+    const void* f[MG_BATCH_MAX];
+    double omu[MG_BATCH_MAX];
+    for (int j = 0; j < n; j++) {
+      Slot& S = slots_[act_[i0 + j]];
+      u[j] = S.u[lev];
+      f[j] = S.f[lev];
+      omu[j] = S.omu;
+    }
+    launch_point_gs_batch<T>(L.view(0.0), n, u, f, omu, iter, s_);
+  }
+}
+
+// src/gmg.c:1301-1313: line sweeps along the axes coarsened on the next
+// level with semi-coarsening; the vertex-patch point smoother everywhere
+// without it.  On the coarsest level of a semi-coarsening pattern the lines
+// along the kept axis replace the reference's point smoother (one line
+// sweep there is an exact solve, see exact_coarsest).
 template <typename T>
 void Solver::smooth(int pat, int lev, int iter) {
   if (p_.isemicoarsen && lev < lmax_ - 1) {
@@ -416,7 +439,7 @@ void Solver::smooth(int pat, int lev, int iter) {
   } else if (p_.isemicoarsen) {
     for (int s = 0; s < p_.coarse_sweeps; s++) line_sweep<T>(pat, lev, pat, iter + s);
   } else {
-    for (int a = 0; a < 3; a++) line_sweep<T>(pat, lev, a, iter);
+    point_sweep<T>(pat, lev, iter);
   }
 }
 

@@ -261,7 +261,8 @@ class Solver:
         nc = nv(self.dims(pattern, lev + 1)) if op in (2, 3) and lev + 1 < lmax else None
         in1 = np.ascontiguousarray(in1, np.complex128).reshape(-1)
         in2 = None if in2 is None else np.ascontiguousarray(in2, np.complex128).reshape(-1)
-        want = {0: (nf, nf, nf), 1: (nf, nf, nf), 2: (nf, None, nc), 3: (nc, nf, nf), 4: (nf, None, nf), 5: (nf, nf, nf)}[op]
+        want = {0: (nf, nf, nf), 1: (nf, nf, nf), 2: (nf, None, nc), 3: (nc, nf, nf), 4: (nf, None, nf), 5: (nf, nf, nf),
+                6: (nf, nf, nf)}[op]
         if want[0] is None or in1.size != want[0] or (want[1] is not None and (in2 is None or in2.size != want[1])):
             raise ValueError("vector sizes do not match the level")
         out = np.zeros(want[2], np.complex128)
