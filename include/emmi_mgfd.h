@@ -198,6 +198,19 @@ emmi_solver_t *emmi_solver_create_batch(int n1, int n2, int n3, const float *x1,
                                         int sig_on_device, const float *omegas, int nslots, const emmi_params_t *p);
 int emmi_solver_solve_batch(emmi_solver_t *s, int n, const void *const *b, void *const *x, void *const *h,
                             const void *const *x0, emmi_stats_t *st);
+/* Survey solves (no counterpart in the reference, which runs one rank per
+ * shot, src/acq_init_free.c:120-142): nmed grids and media of one multigrid
+ * size -- x1[m]..x3[m], sig11[m]..sig33[m] as for emmi_solver_create, e.g.
+ * the shots of a survey each regridded about its own source -- and nslots
+ * <= 64 right-hand sides, slot k on grid slot_med[k] at omegas[k].  Every
+ * kernel launch of a cycle serves all unconverged slots whatever their grid;
+ * each slot stops at its own convergence test.  Solve with
+ * emmi_solver_solve_batch. */
+emmi_solver_t *emmi_solver_create_survey(int n1, int n2, int n3, int nmed, const float *const *x1,
+                                         const float *const *x2, const float *const *x3, const float *const *sig11,
+                                         const float *const *sig22, const float *const *sig33, int sig_on_device,
+                                         const int *slot_med, const float *omegas, int nslots,
+                                         const emmi_params_t *p);
 int emmi_solver_set_omega_slot(emmi_solver_t *s, int slot, float omega);
 int emmi_solver_history_slot(emmi_solver_t *s, int slot, double *out, int maxn);
 int emmi_solver_slots(emmi_solver_t *s);

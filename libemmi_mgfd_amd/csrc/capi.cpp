@@ -97,9 +97,40 @@ emmi_solver_t* emmi_solver_create_batch(int n1, int n2, int n3, const float* x1,
     if (n1 < 2 || n2 < 2 || n3 < 2) throw std::runtime_error("n1, n2, n3 must be >= 2");
     if (!x1 || !x2 || !x3 || !sig11 || !sig22 || !sig33) throw std::runtime_error("null grid or medium");
     if (!omegas || nslots < 1) throw std::runtime_error("need at least one frequency");
+    if (nslots > Solver::MAX_SLOTS) throw std::runtime_error("at most 64 right-hand sides per solver");
     check_hip(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     h->precision = q.precision;
     h->s = new Solver(n1, n2, n3, x1, x2, x3, sig11, sig22, sig33, sig_on_device, omegas, nslots, q, h->stream);
+    h->hist.assign(nslots, {});
+    return h;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return nullptr;
+  }
+}
+
+emmi_solver_t* emmi_solver_create_survey(int n1, int n2, int n3, int nmed, const float* const* x1,
+                                         const float* const* x2, const float* const* x3, const float* const* sig11,
+                                         const float* const* sig22, const float* const* sig33, int sig_on_device,
+                                         const int* slot_med, const float* omegas, int nslots,
+                                         const emmi_params_t* p) {
+  emmi_solver* h = new emmi_solver;
+  try {
+    SolveParams q = to_params(p);
+    validate(q);
+    if (n1 < 2 || n2 < 2 || n3 < 2) throw std::runtime_error("n1, n2, n3 must be >= 2");
+    if (nmed < 1 || !x1 || !x2 || !x3 || !sig11 || !sig22 || !sig33) throw std::runtime_error("null grid or medium");
+    for (int m = 0; m < nmed; m++)
+      if (!x1[m] || !x2[m] || !x3[m] || !sig11[m] || !sig22[m] || !sig33[m])
+        throw std::runtime_error("null grid or medium in the survey");
+    if (!omegas || !slot_med || nslots < 1) throw std::runtime_error("need at least one right-hand side");
+    if (nslots > Solver::MAX_SLOTS) throw std::runtime_error("at most 64 right-hand sides per solver");
+    check_hip(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    h->precision = q.precision;
+    h->s = new Solver(n1, n2, n3, nmed, x1, x2, x3, sig11, sig22, sig33, sig_on_device, slot_med, omegas, nslots, q,
+                      h->stream);
     h->hist.assign(nslots, {});
     return h;
   } catch (const std::exception& e) {
@@ -268,7 +299,7 @@ int check_op(Solver& S, const Hier<T>& H, int op, int pat, int lev, int axis, in
       void* uu[1] = {u.p};
       const void* ff[1] = {f.p};
       const double om[1] = {S.omu()};
-      launch_point_gs_batch<T>(V, 1, uu, ff, om, iter, s);
+      launch_point_gs_batch<T>(&V, 1, uu, ff, om, iter, s);
       check_hip(hipStreamSynchronize(s), "sync");
       u.down(out);
     } else {

@@ -13,9 +13,13 @@ struct Transfer {
   const double* wr[3];
 };
 
-// The per-slot vectors of one launch over several right-hand sides that share
-// a level (grid y = slot): in0/in1 are read, out0/out1 written, omu is the
-// slot's omega mu0.
+// The per-slot vectors of one launch over several right-hand sides whose
+// levels have the same shape (grid y = slot): in0/in1 are read, out0/out1
+// written, omu is the slot's omega mu0.  Each slot may have its own grid and
+// medium (the shots of a survey are regridded about their own sources): geo
+// is the base of the slot's geometry block (W, IW, H, X, D per axis, gstride
+// apart), sbar its edge medium and wr the base of its transfer weights
+// (wstride apart per axis).
 constexpr int MG_BATCH_MAX = 8;
 template <typename T>
 struct SlotVecs {
@@ -24,26 +28,32 @@ struct SlotVecs {
   typename cvec<T>::type* out0[MG_BATCH_MAX];
   typename cvec<T>::type* out1[MG_BATCH_MAX];
   double omu[MG_BATCH_MAX];
+  const double* geo[MG_BATCH_MAX];
+  const T* sbar[MG_BATCH_MAX];
+  const double* wr[MG_BATCH_MAX];
+  long gstride, wstride;
 };
 
 template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s);
-// r_k = f_k - A(omu_k) u_k for n slots of one level, one launch
+// r_k = f_k - A(omu_k) u_k for n slots of levels of one shape (Ls[k]: slot
+// k's level), one launch
 template <typename T>
-void launch_residual_batch(const Level<T>& L, int n, const void* const* u, const void* const* f, void* const* r,
+void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s);
 // one vertex-patch Gauss-Seidel sweep (src/gmg.c:116-336) of u_k against f_k
-// for n slots of one level: 8 node colours, reversed for odd iter
+// for n slots of levels of one shape: 8 node colours, reversed for odd iter
 template <typename T>
-void launch_point_gs_batch(const Level<T>& L, int n, void* const* u, const void* const* f, const double* omu,
+void launch_point_gs_batch(const Level<T>* Ls, int n, void* const* u, const void* const* f, const double* omu,
                            int iter, hipStream_t s);
-// fc_k = R rf_k, uc_k zeroed, for n slots, one launch
+// fc_k = R rf_k, uc_k zeroed, for n slots, one launch (F, C: the shapes;
+// Xs[k]: slot k's transfer weights)
 template <typename T>
-void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,
+void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* rf,
                            void* const* fc, void* const* uc, hipStream_t s);
 // uf_k += P uc_k for n slots, one launch
 template <typename T>
-void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* uc,
+void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* uc,
                           void* const* uf, hipStream_t s);
 int residual_blocks(long nc);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);

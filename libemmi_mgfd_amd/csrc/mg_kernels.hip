@@ -23,6 +23,28 @@
 
 namespace emmi {
 
+// slot sl's geometry and medium in a batched launch (same shape as L)
+template <typename T>
+__device__ __forceinline__ void slot_level(Level<T>& L, const SlotVecs<T>& V, int sl) {
+  const double* g = V.geo[sl];
+  const long gs = V.gstride;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    L.W[a] = g + (5 * a + 0) * gs;
+    L.IW[a] = g + (5 * a + 1) * gs;
+    L.H[a] = g + (5 * a + 2) * gs;
+    L.X[a] = g + (5 * a + 3) * gs;
+  }
+  L.sbar = V.sbar[sl];
+  L.omu = V.omu[sl];
+}
+
+template <typename T>
+__device__ __forceinline__ void slot_transfer(Transfer<T>& X, const SlotVecs<T>& V, int sl) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) X.wr[a] = V.wr[sl] + a * V.wstride;
+}
+
 // block-wide sum of doubles into partial[blk]; blockDim multiple of 64
 __device__ __forceinline__ void block_sum_store(double v, double* partial, long blk) {
   __shared__ double red[16];
@@ -42,7 +64,7 @@ __device__ __forceinline__ void block_sum_store(double v, double* partial, long 
 template <typename T>
 __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, double* partial) {
   const int sl = blockIdx.y;
-  L.omu = V.omu[sl];
+  slot_level(L, V, sl);
   const typename cvec<T>::type* __restrict__ u = V.in0[sl];
   const typename cvec<T>::type* __restrict__ f = V.in1[sl];
   typename cvec<T>::type* __restrict__ r = V.out0[sl];
@@ -94,7 +116,7 @@ __device__ __forceinline__ int parity_count(int n, int par, int& first) {
 template <typename T>
 __global__ void __launch_bounds__(128) k_point_gs(Level<T> L, SlotVecs<T> V, int colour) {
   const int sl = blockIdx.y;
-  L.omu = V.omu[sl];
+  slot_level(L, V, sl);
   typename cvec<T>::type* __restrict__ u = V.out0[sl];
   const typename cvec<T>::type* __restrict__ f = V.in1[sl];
   int f1, f2, f3;
@@ -177,6 +199,7 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // coarse solve starts from zero)
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X, SlotVecs<T> V) {
+  slot_transfer(X, V, blockIdx.y);
   const typename cvec<T>::type* __restrict__ rf = V.in0[blockIdx.y];
   typename cvec<T>::type* __restrict__ fc = V.out0[blockIdx.y];
   typename cvec<T>::type* __restrict__ uc = V.out1[blockIdx.y];
@@ -223,6 +246,7 @@ __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Trans
 // prolongation: u_f += P u_c on interior fine edges
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(Level<T> F, Level<T> Cl, Transfer<T> X, SlotVecs<T> V) {
+  slot_transfer(X, V, blockIdx.y);
   const typename cvec<T>::type* __restrict__ uc = V.in0[blockIdx.y];
   typename cvec<T>::type* __restrict__ uf = V.out0[blockIdx.y];
   const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -374,18 +398,44 @@ __global__ void k_edge_sigma(int n1, int n2, int n3, const double* __restrict__ 
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// per-slot geometry of a batch: every level must have the shape of Ls[0]
 template <typename T>
-void launch_residual_batch(const Level<T>& L, int n, const void* const* u, const void* const* f, void* const* r,
+static void slot_levels(SlotVecs<T>& B, const Level<T>* Ls, int n) {
+  B.gstride = Ls[0].IW[0] - Ls[0].W[0];
+  for (int k = 0; k < n; k++) {
+    for (int a = 0; a < 3; a++)
+      if (Ls[k].n[a] != Ls[0].n[a]) throw std::runtime_error("batched levels differ in shape");
+    if (Ls[k].IW[0] - Ls[k].W[0] != B.gstride) throw std::runtime_error("batched levels differ in layout");
+    B.geo[k] = Ls[k].W[0];
+    B.sbar[k] = Ls[k].sbar;
+  }
+}
+
+template <typename T>
+static void slot_transfers(SlotVecs<T>& B, const Transfer<T>* Xs, int n) {
+  B.wstride = Xs[0].wr[1] - Xs[0].wr[0];
+  for (int k = 0; k < n; k++) {
+    for (int a = 0; a < 3; a++)
+      if (Xs[k].sc[a] != Xs[0].sc[a]) throw std::runtime_error("batched transfers differ in coarsening");
+    if (Xs[k].wr[1] - Xs[k].wr[0] != B.wstride) throw std::runtime_error("batched transfers differ in layout");
+    B.wr[k] = Xs[k].wr[0];
+  }
+}
+
+template <typename T>
+void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s) {
   using V = typename cvec<T>::type;
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("residual batch of 1..8 slots");
   SlotVecs<T> B = {};
+  slot_levels(B, Ls, n);
   for (int k = 0; k < n; k++) {
     B.in0[k] = (const V*)u[k];
     B.in1[k] = (const V*)f[k];
     B.out0[k] = (V*)r[k];
     B.omu[k] = omu[k];
   }
+  const Level<T>& L = Ls[0];
   const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
   timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (3 * sz + sr));
   hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256), n), dim3(256), 0, s, L, B, (double*)nullptr);
@@ -395,6 +445,7 @@ template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s) {
   using V = typename cvec<T>::type;
   SlotVecs<T> B = {};
+  slot_levels(B, &L, 1);
   B.in0[0] = (const V*)u;
   B.in1[0] = (const V*)f;
   B.out0[0] = (V*)r;
@@ -411,11 +462,13 @@ void launch_sum_partials(const double* partial, int n, double* out, hipStream_t 
 }
 
 template <typename T>
-void launch_point_gs_batch(const Level<T>& L, int n, void* const* u, const void* const* f, const double* omu,
+void launch_point_gs_batch(const Level<T>* Ls, int n, void* const* u, const void* const* f, const double* omu,
                            int iter, hipStream_t s) {
   using V = typename cvec<T>::type;
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("point smoother batch of 1..8 slots");
   SlotVecs<T> B = {};
+  slot_levels(B, Ls, n);
+  const Level<T>& L = Ls[0];
   for (int k = 0; k < n; k++) {
     B.out0[k] = (V*)u[k];
     B.in1[k] = (const V*)f[k];
@@ -438,11 +491,13 @@ void launch_point_gs_batch(const Level<T>& L, int n, void* const* u, const void*
 }
 
 template <typename T>
-void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* rf,
+void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* rf,
                            void* const* fc, void* const* uc, hipStream_t s) {
   using V = typename cvec<T>::type;
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("restriction batch of 1..8 slots");
   SlotVecs<T> B = {};
+  slot_transfers(B, Xs, n);
+  const Transfer<T>& X = Xs[0];
   for (int k = 0; k < n; k++) {
     B.in0[k] = (const V*)rf[k];
     B.out0[k] = (V*)fc[k];
@@ -456,15 +511,17 @@ template <typename T>
 void launch_restrict(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* rf, void* fc, void* uc,
                      hipStream_t s) {
   void* ucs[1] = {uc};
-  launch_restrict_batch<T>(F, C, X, 1, &rf, &fc, ucs, s);
+  launch_restrict_batch<T>(F, C, &X, 1, &rf, &fc, ucs, s);
 }
 
 template <typename T>
-void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, int n, const void* const* uc,
+void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* uc,
                           void* const* uf, hipStream_t s) {
   using V = typename cvec<T>::type;
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("prolongation batch of 1..8 slots");
   SlotVecs<T> B = {};
+  slot_transfers(B, Xs, n);
+  const Transfer<T>& X = Xs[0];
   for (int k = 0; k < n; k++) {
     B.in0[k] = (const V*)uc[k];
     B.out0[k] = (V*)uf[k];
@@ -475,7 +532,7 @@ void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T
 
 template <typename T>
 void launch_prolong(const Level<T>& F, const Level<T>& C, const Transfer<T>& X, const void* uc, void* uf, hipStream_t s) {
-  launch_prolong_batch<T>(F, C, X, 1, &uc, &uf, s);
+  launch_prolong_batch<T>(F, C, &X, 1, &uc, &uf, s);
 }
 
 template <typename T>
@@ -521,13 +578,13 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 
 #define EMMI_INSTANTIATE(T)                                                                                    \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
-  template void launch_residual_batch<T>(const Level<T>&, int, const void* const*, const void* const*,         \
+  template void launch_residual_batch<T>(const Level<T>*, int, const void* const*, const void* const*,         \
                                          void* const*, const double*, hipStream_t);                            \
-  template void launch_restrict_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, int,            \
+  template void launch_restrict_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>*, int,            \
                                          const void* const*, void* const*, void* const*, hipStream_t);         \
-  template void launch_prolong_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, int,             \
+  template void launch_prolong_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>*, int,             \
                                         const void* const*, void* const*, hipStream_t);                        \
-  template void launch_point_gs_batch<T>(const Level<T>&, int, void* const*, const void* const*, const double*,  \
+  template void launch_point_gs_batch<T>(const Level<T>*, int, void* const*, const void* const*, const double*,  \
                                          int, hipStream_t);                                                    \
   template void launch_restrict<T>(const Level<T>&, const Level<T>&, const Transfer<T>&, const void*, void*, void*, \
                                    hipStream_t);                                                               \

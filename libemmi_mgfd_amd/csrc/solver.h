@@ -75,8 +75,18 @@ struct LineFactors {
   double omu = -1.0;   // frequency the records were built for
 };
 
+// grid and medium of one shot: host geometry and the device hierarchy in the
+// cycle precision, plus the FP64 level-0 operator of the refinement
+struct Medium {
+  std::vector<std::vector<std::array<HostAxis, 3>>> axes;  // [pattern][level][axis]
+  Hier<double> h64;
+  Hier<float> h32;
+  DevLevel<double> l0_64;
+};
+
 // one right-hand side
 struct Slot {
+  int med = 0;                       // its grid and medium
   double omu = 0;
   std::vector<void*> u, f, r;        // per level, working precision
   void* u64 = nullptr;               // level-0 FP64 state (aliases u/f/r[0] in FP64 mode)
@@ -92,12 +102,19 @@ struct Slot {
 
 class Solver {
  public:
+  static constexpr int MAX_SLOTS = 64;   // active slots are tracked in a 64-bit mask
   // x1..x3: float node coordinates (n+1); sig: float [n3][n2][n1] conductivities
   // (host pointers when sig_on_device == 0, device pointers otherwise);
   // omegas: the angular frequency of each of the nslots right-hand sides
   Solver(int n1, int n2, int n3, const float* x1, const float* x2, const float* x3, const float* sig11,
          const float* sig22, const float* sig33, int sig_on_device, const float* omegas, int nslots,
          const SolveParams& p, hipStream_t stream);
+  // nmed grids and media of one multigrid size (e.g. the shots of a survey,
+  // each regridded about its own source); slot k solves on grid slot_med[k]
+  // at omegas[k].  Slots on different grids still share every launch.
+  Solver(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2, const float* const* x3,
+         const float* const* sig11, const float* const* sig22, const float* const* sig33, int sig_on_device,
+         const int* slot_med, const float* omegas, int nslots, const SolveParams& p, hipStream_t stream);
   ~Solver();
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -122,9 +139,10 @@ class Solver {
   void set_ncycle(int ncycle) { p_.ncycle = ncycle; }
   const int* dims(int pat, int lev) const { return dims_[pat][lev].data(); }
 
-  // kernel-level access for the element-wise checks (FP64 hierarchy)
-  const Hier<double>& h64() const { return h64_; }
-  const Hier<float>& h32() const { return h32_; }
+  // kernel-level access for the element-wise checks (hierarchies of slot 0's grid)
+  const Hier<double>& h64() const { return meds_[slots_[0].med].h64; }
+  const Hier<float>& h32() const { return meds_[slots_[0].med].h32; }
+  int nmedia() const { return (int)meds_.size(); }
   hipStream_t stream() const { return s_; }
   // one line sweep (all four colours) along `axis` on vectors u, f of level
   // `lev` of pattern `pat`, with the frequency of slot 0, in the working
@@ -132,8 +150,12 @@ class Solver {
   template <typename T> void line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f);
 
  private:
-  template <typename T> Hier<T>& hier();
-  template <typename T> void build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev);
+  void init(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2,
+            const float* const* x3, const float* const* sig11, const float* const* sig22,
+            const float* const* sig33, int sig_on_device, const int* slot_med, const float* omegas, int nslots);
+  template <typename T> Hier<T>& hier(int med);
+  template <typename T> void build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev,
+                                        int med);
   template <typename T> void alloc_vectors(Slot& S, const Hier<T>& H);
   template <typename T> void smooth(int pat, int lev, int iter);
   template <typename T> void line_sweep(int pat, int lev, int axis, int iter);
@@ -161,10 +183,7 @@ class Solver {
   SolveParams p_;
   hipStream_t s_;
   std::vector<std::vector<std::array<int, 3>>> dims_, scs_;   // [pattern][level]
-  std::vector<std::vector<std::array<HostAxis, 3>>> axes_;    // [pattern][level][axis]
-  Hier<double> h64_;
-  Hier<float> h32_;
-  DevLevel<double> l0_64_;    // level-0 FP64 operator of the refinement
+  std::vector<Medium> meds_;
   std::vector<Slot> slots_;
   size_t lscr_bytes_ = 0;
   double* dnorm_ = nullptr;   // one per slot
@@ -177,7 +196,7 @@ class Solver {
   bool rnorm0_in_cycle_ = false;  // rnorm0 is the cycle-0 in-cycle norm (FP64 V/W cycles)
   // captured cycle bodies, keyed by pattern and active-slot mask; valid for
   // the slots' frequencies at capture time
-  std::map<std::pair<int, unsigned>, hipGraphExec_t> graphs_;
+  std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs_;
   std::vector<double> graph_omu_;
 };
 

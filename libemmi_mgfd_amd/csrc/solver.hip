@@ -93,8 +93,8 @@ static void* dalloc(size_t bytes, std::vector<void*>& owned) {
   return p;
 }
 
-template <> Hier<double>& Solver::hier<double>() { return h64_; }
-template <> Hier<float>& Solver::hier<float>() { return h32_; }
+template <> Hier<double>& Solver::hier<double>(int m) { return meds_[m].h64; }
+template <> Hier<float>& Solver::hier<float>(int m) { return meds_[m].h32; }
 
 
 static constexpr double MU0 = 4. * 3.14159265358979323846264 * 1e-7;
@@ -103,59 +103,89 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
                const float* sig22, const float* sig33, int sig_on_device, const float* omegas, int nslots,
                const SolveParams& p, hipStream_t stream)
     : p_(p), s_(stream) {
+  std::vector<int> med(nslots > 0 ? nslots : 1, 0);
+  init(n1, n2, n3, 1, &x1, &x2, &x3, &sig11, &sig22, &sig33, sig_on_device, med.data(), omegas, nslots);
+}
+
+Solver::Solver(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2,
+               const float* const* x3, const float* const* sig11, const float* const* sig22,
+               const float* const* sig33, int sig_on_device, const int* slot_med, const float* omegas, int nslots,
+               const SolveParams& p, hipStream_t stream)
+    : p_(p), s_(stream) {
+  init(n1, n2, n3, nmed, x1, x2, x3, sig11, sig22, sig33, sig_on_device, slot_med, omegas, nslots);
+}
+
+void Solver::init(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2,
+                  const float* const* x3, const float* const* sig11, const float* const* sig22,
+                  const float* const* sig33, int sig_on_device, const int* slot_med, const float* omegas,
+                  int nslots) {
   n_[0] = n1; n_[1] = n2; n_[2] = n3;
   for (int a = 0; a < 3; a++)
     if (n_[a] < 2) throw std::runtime_error("multigrid size must be at least 2 cells per axis");
   if (nslots < 1) throw std::runtime_error("a solver needs at least one right-hand-side slot");
+  if (nslots > MAX_SLOTS) throw std::runtime_error("at most 64 right-hand-side slots per solver");
+  if (nmed < 1) throw std::runtime_error("a solver needs at least one grid and medium");
+  for (int k = 0; k < nslots; k++)
+    if (slot_med[k] < 0 || slot_med[k] >= nmed) throw std::runtime_error("slot medium index out of range");
   nc0_ = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
   lmax_ = std::max(depth_of(n1), std::max(depth_of(n2), depth_of(n3)));
   if (lmax_ < 1) lmax_ = 1;
   npat_ = p_.isemicoarsen ? 3 : 1;
 
-  // ---- host geometry of all levels and patterns ----
-  const float* xf[3] = {x1, x2, x3};
-  std::array<HostAxis, 3> a0;
-  for (int a = 0; a < 3; a++) {
-    std::vector<double> nodes(n_[a] + 1);
-    for (int i = 0; i <= n_[a]; i++) nodes[i] = xf[a][i];
-    a0[a].from_nodes(nodes);
-  }
+  // ---- level shapes of all patterns (the same for every grid of this size) ----
   dims_.assign(npat_, {});
   scs_.assign(npat_, {});
-  axes_.assign(npat_, {});
   for (int c = 0; c < npat_; c++) {
     dims_[c].push_back({n1, n2, n3});
     scs_[c].push_back({1, 1, 1});
-    axes_[c].push_back(a0);
     for (int lev = 1; lev < lmax_; lev++) {
       const auto& nf = dims_[c][lev - 1];
       std::array<int, 3> sc, n;
       for (int a = 0; a < 3; a++) sc[a] = (nf[a] % 2 == 0 && nf[a] > 2) ? 2 : 1;
       if (p_.isemicoarsen) sc[c] = 1;
       for (int a = 0; a < 3; a++) n[a] = nf[a] / sc[a];
-      std::array<HostAxis, 3> ax;
-      for (int a = 0; a < 3; a++) {
-        std::vector<double> nodes(n[a] + 1);
-        for (int i = 0; i <= n[a]; i++) nodes[i] = axes_[c][lev - 1][a].x[sc[a] * i];
-        ax[a].from_nodes(nodes);
-      }
       dims_[c].push_back(n);
       scs_[c].push_back(sc);
-      axes_[c].push_back(ax);
     }
   }
 
-  // ---- medium: sigma*vol cells on the device, all levels and patterns ----
-  const long ncell0 = (long)n1 * n2 * n3;
-  std::vector<void*> tmp;
-  std::vector<std::vector<double*>> cells(npat_ * lmax_);  // [pat*lmax+lev][3]
-  {
+  meds_.resize(nmed);
+  for (int m = 0; m < nmed; m++) {
+    Medium& M = meds_[m];
+    // ---- host geometry of all levels and patterns ----
+    const float* xf[3] = {x1[m], x2[m], x3[m]};
+    std::array<HostAxis, 3> a0;
+    for (int a = 0; a < 3; a++) {
+      std::vector<double> nodes(n_[a] + 1);
+      for (int i = 0; i <= n_[a]; i++) nodes[i] = xf[a][i];
+      a0[a].from_nodes(nodes);
+    }
+    M.axes.assign(npat_, {});
+    for (int c = 0; c < npat_; c++) {
+      M.axes[c].push_back(a0);
+      for (int lev = 1; lev < lmax_; lev++) {
+        const auto& n = dims_[c][lev];
+        const auto& sc = scs_[c][lev];
+        std::array<HostAxis, 3> ax;
+        for (int a = 0; a < 3; a++) {
+          std::vector<double> nodes(n[a] + 1);
+          for (int i = 0; i <= n[a]; i++) nodes[i] = M.axes[c][lev - 1][a].x[sc[a] * i];
+          ax[a].from_nodes(nodes);
+        }
+        M.axes[c].push_back(ax);
+      }
+    }
+
+    // ---- medium: sigma*vol cells on the device, all levels and patterns ----
+    const long ncell0 = (long)n1 * n2 * n3;
+    std::vector<void*> tmp;
+    std::vector<std::vector<double*>> cells(npat_ * lmax_);  // [pat*lmax+lev][3]
     double* W[3];
     for (int a = 0; a < 3; a++) {
       W[a] = (double*)dalloc(sizeof(double) * (n_[a] + 1), tmp);
       HIPCHECK(hipMemcpyAsync(W[a], a0[a].ds.data(), sizeof(double) * (n_[a] + 1), hipMemcpyHostToDevice, s_));
     }
-    const float* sg[3] = {sig11, sig22, sig33};
+    const float* sg[3] = {sig11[m], sig22[m], sig33[m]};
     std::vector<double*> c0(3);
     for (int q = 0; q < 3; q++) {
       const float* src = sg[q];
@@ -180,17 +210,19 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
         cells[c * lmax_ + lev] = cl;
       }
     }
-  }
 
-  if (p_.precision == 0) {
-    build_hier<double>(h64_, cells, lmax_);
-    l0_64_ = h64_.pat[0][0];
-  } else {
-    build_hier<float>(h32_, cells, lmax_);
-    // FP64 level-0 operator of the refinement
-    Hier<double> one;
-    build_hier<double>(one, cells, 1);
-    l0_64_ = one.pat[0][0];
+    if (p_.precision == 0) {
+      build_hier<double>(M.h64, cells, lmax_, m);
+      M.l0_64 = M.h64.pat[0][0];
+    } else {
+      build_hier<float>(M.h32, cells, lmax_, m);
+      // FP64 level-0 operator of the refinement
+      Hier<double> one;
+      build_hier<double>(one, cells, 1, m);
+      M.l0_64 = one.pat[0][0];
+    }
+    HIPCHECK(hipStreamSynchronize(s_));
+    for (void* t : tmp) HIPCHECK(hipFree(t));
   }
 
   // records of the line solves: max over levels, patterns, axes and colours
@@ -204,14 +236,15 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
   slots_.resize(nslots);
   for (int k = 0; k < nslots; k++) {
     Slot& S = slots_[k];
+    S.med = slot_med[k];
     S.omu = (double)omegas[k] * MU0;   // omega stored as float (src/gmg.c:1709)
     if (p_.precision == 0) {
-      alloc_vectors<double>(S, h64_);
+      alloc_vectors<double>(S, meds_[S.med].h64);
       S.u64 = S.u[0];
       S.f64 = S.f[0];
       S.r64 = S.r[0];
     } else {
-      alloc_vectors<float>(S, h32_);
+      alloc_vectors<float>(S, meds_[S.med].h32);
       const size_t b = sizeof(double2) * 3 * (size_t)nc0_;
       S.u64 = dalloc(b, owned_);
       S.f64 = dalloc(b, owned_);
@@ -224,7 +257,6 @@ Solver::Solver(int n1, int n2, int n3, const float* x1, const float* x2, const f
   dnorm_ = (double*)dalloc(sizeof(double) * nslots, owned_);
   HIPCHECK(hipHostMalloc((void**)&hnorm_, sizeof(double) * nslots, hipHostMallocDefault));
   HIPCHECK(hipStreamSynchronize(s_));
-  for (void* t : tmp) HIPCHECK(hipFree(t));
 }
 
 Solver::~Solver() {
@@ -240,7 +272,8 @@ void Solver::set_omega(int slot, float omega) {
 }
 
 template <typename T>
-void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev) {
+void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cells, int nlev, int med) {
+  const auto& axes = meds_[med].axes;
   H.pat.assign(npat_, std::vector<DevLevel<T>>(nlev));
   H.maxnc.assign(nlev, 0);
   for (int c = 0; c < npat_; c++)
@@ -257,7 +290,7 @@ void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cel
       // 1D geometry in FP64 for every storage precision (see Level)
       std::vector<double> g(15 * (size_t)L.gstride, 0.0);
       for (int a = 0; a < 3; a++) {
-        const HostAxis& ax = axes_[c][lev][a];
+        const HostAxis& ax = axes[c][lev][a];
         for (int i = 0; i <= n[a]; i++) {
           g[(5 * a + 0) * L.gstride + i] = ax.ds[i];
           g[(5 * a + 1) * L.gstride + i] = 1.0 / ax.ds[i];
@@ -275,8 +308,8 @@ void Solver::build_hier(Hier<T>& H, const std::vector<std::vector<double*>>& cel
         L.wstride = nfmax + 1;
         std::vector<double> w(3 * (size_t)L.wstride, 0.0);
         for (int a = 0; a < 3; a++) {
-          const HostAxis& fa = axes_[c][lev - 1][a];
-          const HostAxis& ca = axes_[c][lev][a];
+          const HostAxis& fa = axes[c][lev - 1][a];
+          const HostAxis& ca = axes[c][lev][a];
           for (int q = 0; q <= nf[a]; q++) {
             const int i = L.sc[a] == 2 ? q / 2 : q;
             if (i + 1 > n[a]) continue;
@@ -338,7 +371,7 @@ LineFactors& Solver::factors(Slot& S, int pat, int lev, int axis) {
   const int pe = lev == 0 ? 0 : pat;
   LineFactors& F = S.facs[((size_t)pe * lmax_ + lev) * 3 + axis];
   if (F.omu == S.omu) return F;
-  const DevLevel<T>& L = hier<T>().pat[pe][lev];
+  const DevLevel<T>& L = hier<T>(S.med).pat[pe][lev];
   LinePack<T> P;
   P.njobs = 4;
   for (int col = 0; col < 4; col++) {
@@ -360,7 +393,7 @@ template <typename T>
 void Solver::line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f) {
   Slot& S = slots_[0];
   const LineFactors& F = factors<T>(S, pat, lev, axis);
-  const DevLevel<T>& L = hier<T>().pat[pat][lev];
+  const DevLevel<T>& L = hier<T>(S.med).pat[pat][lev];
   LinePack<T> P;
   P.njobs = 1;
   LineJob<T>& J = P.job[0];
@@ -383,7 +416,6 @@ template void Solver::line_sweep_on<float>(int, int, int, int, void*, const void
 // launches of up to LINE_PACK_MAX jobs
 template <typename T>
 void Solver::line_sweep(int pat, int lev, int axis, int iter) {
-  const DevLevel<T>& L = hier<T>().pat[pat][lev];
   std::vector<const LineFactors*> F(act_.size());
   for (size_t i = 0; i < act_.size(); i++) F[i] = &factors<T>(slots_[act_[i]], pat, lev, axis);
   for (int k = 0; k < 4; k++) {
@@ -394,7 +426,7 @@ void Solver::line_sweep(int pat, int lev, int axis, int iter) {
       for (int j = 0; j < P.njobs; j++) {
         Slot& S = slots_[act_[i0 + j]];
         LineJob<T>& J = P.job[j];
-        J.L = L.view(S.omu);
+        J.L = hier<T>(S.med).pat[pat][lev].view(S.omu);
         J.u = (typename cvec<T>::type*)S.u[lev];
         J.f = (const typename cvec<T>::type*)S.f[lev];
         J.fac = (typename LineFac<T>::type*)F[i0 + j]->fac[colour];
@@ -409,19 +441,20 @@ void Solver::line_sweep(int pat, int lev, int axis, int iter) {
 // one vertex-patch Gauss-Seidel sweep for every active slot
 template <typename T>
 void Solver::point_sweep(int pat, int lev, int iter) {
-  const DevLevel<T>& L = hier<T>().pat[pat][lev];
   for (size_t i0 = 0; i0 < act_.size(); i0 += MG_BATCH_MAX) {
     const int n = (int)std::min<size_t>(MG_BATCH_MAX, act_.size() - i0);
     void* u[MG_BATCH_MAX];
     const void* f[MG_BATCH_MAX];
     double omu[MG_BATCH_MAX];
+    Level<T> Ls[MG_BATCH_MAX];
     for (int j = 0; j < n; j++) {
       Slot& S = slots_[act_[i0 + j]];
       u[j] = S.u[lev];
       f[j] = S.f[lev];
       omu[j] = S.omu;
+      Ls[j] = hier<T>(S.med).pat[pat][lev].view(S.omu);
     }
-    launch_point_gs_batch<T>(L.view(0.0), n, u, f, omu, iter, s_);
+    launch_point_gs_batch<T>(Ls, n, u, f, omu, iter, s_);
   }
 }
 
@@ -450,8 +483,9 @@ void Solver::norms(const std::vector<int>& which, const void* const* u, const vo
                    std::vector<double>& out) {
   for (size_t i = 0; i < which.size(); i++) {
     Slot& S = slots_[which[i]];
-    launch_residual<double>(l0_64_.view(S.omu), u[i], f[i], r[i], S.partial, s_);
-    launch_sum_partials(S.partial, residual_blocks(l0_64_.nc), dnorm_ + which[i], s_);
+    const DevLevel<double>& L0 = meds_[S.med].l0_64;
+    launch_residual<double>(L0.view(S.omu), u[i], f[i], r[i], S.partial, s_);
+    launch_sum_partials(S.partial, residual_blocks(L0.nc), dnorm_ + which[i], s_);
   }
   HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double) * slots_.size(), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
@@ -509,8 +543,9 @@ bool Solver::check_active(int pat) {
 // true when the level-0 convergence test retired every slot
 template <typename T>
 bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess, int kind) {
-  const DevLevel<T>& L = hier<T>().pat[pat][lev];
-  const DevLevel<T>& C = hier<T>().pat[pat][lev + 1];
+  // level shapes (the same for every slot's grid)
+  const DevLevel<T>& L = hier<T>(0).pat[pat][lev];
+  const DevLevel<T>& C = hier<T>(0).pat[pat][lev + 1];
   bool from_f = false, r0_was_ready = false;
   if (lev == 0 && norm_here) {
     // level-0 residual with its norm (FP64 direct form: r64 is the level's r)
@@ -534,6 +569,8 @@ bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess
     void* fc[MG_BATCH_MAX];
     void* uc[MG_BATCH_MAX];
     double omu[MG_BATCH_MAX];
+    Level<T> Ls[MG_BATCH_MAX];
+    Transfer<T> Xs[MG_BATCH_MAX];
     for (int j = 0; j < n; j++) {
       Slot& S = slots_[act_[i0 + j]];
       u[j] = S.u[lev];
@@ -543,9 +580,11 @@ bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess
       fc[j] = S.f[lev + 1];
       uc[j] = S.u[lev + 1];
       omu[j] = S.omu;
+      Ls[j] = hier<T>(S.med).pat[pat][lev].view(S.omu);
+      Xs[j] = hier<T>(S.med).pat[pat][lev + 1].transfer();
     }
-    if (resid) launch_residual_batch<T>(L.view(0.0), n, u, f, r, omu, s_);
-    launch_restrict_batch<T>(L.view(0.0), C.view(0.0), C.transfer(), n, rf, fc, uc, s_);
+    if (resid) launch_residual_batch<T>(Ls, n, u, f, r, omu, s_);
+    launch_restrict_batch<T>(L.view(0.0), C.view(0.0), Xs, n, rf, fc, uc, s_);
   }
   if (kind == 2) f_cycle<T>(pat, lev + 1, false, true);
   else if (kind == 3) { w_cycle<T>(pat, lev + 1, false, true); w_cycle<T>(pat, lev + 1, false, false); }
@@ -554,12 +593,14 @@ bool Solver::coarse_correction(int pat, int lev, bool norm_here, bool zero_guess
     const int n = (int)std::min<size_t>(MG_BATCH_MAX, act_.size() - i0);
     const void* uc[MG_BATCH_MAX];
     void* uf[MG_BATCH_MAX];
+    Transfer<T> Xs[MG_BATCH_MAX];
     for (int j = 0; j < n; j++) {
       Slot& S = slots_[act_[i0 + j]];
       uc[j] = S.u[lev + 1];
       uf[j] = S.u[lev];
+      Xs[j] = hier<T>(S.med).pat[pat][lev + 1].transfer();
     }
-    launch_prolong_batch<T>(L.view(0.0), C.view(0.0), C.transfer(), n, uc, uf, s_);
+    launch_prolong_batch<T>(L.view(0.0), C.view(0.0), Xs, n, uc, uf, s_);
   }
   return false;
 }
@@ -583,7 +624,7 @@ bool Solver::v_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
 template <typename T>
 bool Solver::f_cycle(int pat, int lev, bool norm_here, bool zero_guess) {
   if (lev == lmax_ - 1) {
-    for (int k : act_) launch_zero<T>(slots_[k].u[lev], 3 * hier<T>().pat[pat][lev].nc, s_);
+    for (int k : act_) launch_zero<T>(slots_[k].u[lev], 3 * hier<T>(slots_[k].med).pat[pat][lev].nc, s_);
     zero_guess = true;
   } else if (coarsens<T>(pat, lev)) {
     if (coarse_correction<T>(pat, lev, norm_here && p_.cycleopt == 2, zero_guess, 2)) return true;
@@ -643,7 +684,7 @@ void Solver::drop_graphs() {
 // first cycle of a pattern at a new frequency runs eagerly: it builds the
 // line factors, which must not be captured.
 void Solver::cycle_body(int pat) {
-  if (!p_.graphs || timing::enabled() || act_.size() > 32 || slots_.size() > 32) {
+  if (!p_.graphs || timing::enabled() || slots_.size() > (size_t)MAX_SLOTS) {
     cycle_body_eager(pat);
     return;
   }
@@ -661,8 +702,8 @@ void Solver::cycle_body(int pat) {
     drop_graphs();
     graph_omu_ = om;
   }
-  unsigned mask = 0;
-  for (int k : act_) mask |= 1u << k;
+  unsigned long long mask = 0;
+  for (int k : act_) mask |= 1ull << k;
   const auto key = std::make_pair(pat, mask);
   auto it = graphs_.find(key);
   if (it == graphs_.end()) {
@@ -700,7 +741,8 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
     *S.st = SolveStats();
     S.st->cycles = -1;
     // f = b * dual volume (src/gmg.c:1899-1910)
-    launch_scale_rhs<double>(l0_64_.view(S.omu), l0_64_.D(0), l0_64_.D(1), l0_64_.D(2), b[k], S.f64, s_);
+    const DevLevel<double>& L0 = meds_[S.med].l0_64;
+    launch_scale_rhs<double>(L0.view(S.omu), L0.D(0), L0.D(1), L0.D(2), b[k], S.f64, s_);
     launch_zero<double>(S.u64, nv, s_);
     act_.push_back(k);
   }
@@ -778,7 +820,7 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
     if (s->status == 0 && !(s->rnorm < S.rnorm0 * p_.tol)) s->status = 1;
     if (s->history.empty()) s->status = 1;
     HIPCHECK(hipMemcpyAsync(x[k], S.u64, sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
-    launch_curl_h<double>(l0_64_.view(S.omu), S.u64, h[k], s_);
+    launch_curl_h<double>(meds_[S.med].l0_64.view(S.omu), S.u64, h[k], s_);
   }
   HIPCHECK(hipStreamSynchronize(s_));
   HIPCHECK(hipGetLastError());

@@ -53,6 +53,9 @@ def lib() -> C.CDLL:
         L.emmi_solver_create_batch.restype = C.c_void_p
         L.emmi_solver_create_batch.argtypes = ([C.c_int] * 3 + [C.c_void_p] * 6 +
                                                [C.c_int, C.c_void_p, C.c_int, C.POINTER(Params)])
+        L.emmi_solver_create_survey.restype = C.c_void_p
+        L.emmi_solver_create_survey.argtypes = ([C.c_int] * 4 + [C.c_void_p] * 6 +
+                                                [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Params)])
         L.emmi_solver_solve_batch.restype = C.c_int
         L.emmi_solver_solve_batch.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.POINTER(Stats)]
         L.emmi_solver_set_omega_slot.restype = C.c_int
@@ -194,6 +197,42 @@ class Solver:
                                                 C.byref(self.p))
         if not self.h:
             raise NativeError(last_error())
+
+    @classmethod
+    def survey(cls, n, grids, slot_med, omegas, sig_on_device=False, **kw):
+        """A solver over several grids and media of one multigrid size:
+        grids[m] = (x1, x2, x3, sig11, sig22, sig33) of grid m (host arrays,
+        or device pointers for the conductivities with sig_on_device); slot k
+        solves on grid slot_med[k] at omegas[k]."""
+        self = cls.__new__(cls)
+        self.n = tuple(int(v) for v in n)
+        self.p = params(**kw)
+        self._keep = []
+        nmed = len(grids)
+        tabs = [(C.c_void_p * nmed)() for _ in range(6)]
+        for m, g in enumerate(grids):
+            for q in range(3):
+                a = np.ascontiguousarray(g[q], np.float32)
+                self._keep.append(a)
+                tabs[q][m] = a.ctypes.data
+            for q in range(3, 6):
+                if sig_on_device:
+                    tabs[q][m] = int(g[q])
+                else:
+                    a = np.ascontiguousarray(g[q], np.float32)
+                    self._keep.append(a)
+                    tabs[q][m] = a.ctypes.data
+        med = np.ascontiguousarray(slot_med, np.int32)
+        om = np.ascontiguousarray(omegas, np.float32)
+        self._keep += [med, om] + tabs
+        self.nslots = int(om.size)
+        self.h = lib().emmi_solver_create_survey(self.n[0], self.n[1], self.n[2], nmed,
+                                                 *[C.cast(t, C.c_void_p) for t in tabs],
+                                                 1 if sig_on_device else 0, med.ctypes.data, om.ctypes.data,
+                                                 self.nslots, C.byref(self.p))
+        if not self.h:
+            raise NativeError(last_error())
+        return self
 
     @property
     def nvec(self) -> int:
