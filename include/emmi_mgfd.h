@@ -340,14 +340,27 @@ int emmi_device_sync(void);
  * exactly.  Graphs a solver captured earlier keep the kernels they captured.
  * Returns the previous mode, or -1 for an unknown mode. */
 int emmi_set_line_kernel(int mode);
-/* kernel timing: accumulated milliseconds, launch counts and algorithmic
- * bytes (each input read once, each output written once per launch) per
- * kernel class (0 residual, 1 line sweep, 2 restriction, 3 prolongation,
+/* kernel timing: accumulated milliseconds, launch counts and compulsory
+ * bytes (each input read once, each output written once per launch at its
+ * level's shape; a full line sweep, all four colours, counts 3 Ne s + 3 Nc c)
+ * per kernel class (0 residual, 1 line sweep, 2 restriction, 3 prolongation,
  * 4 other, 5 line right-hand sides), measured with HIP events recorded on
  * the stream of each launch (by the dispatch itself for the line kernels)
- * while enabled; enabling clears the counters */
+ * while enabled; enabling clears the counters.  emmi_timing_read_layout
+ * gives, per class, the bytes of the kernels' own data layout (for the line
+ * kernels: their stored factor and line records as well). */
+/* Kernel rates on the solver's own grid (no counterpart in the reference):
+ * the level-0 kernels of slot 0, in the solver's cycle precision, each
+ * launched reps times back to back on the solver's stream between HIP
+ * events; ms[k] is the average launch time, bytes[k] its compulsory bytes,
+ * k = 0 residual (+ norm partials), 1 restriction and 2 prolongation
+ * between level 0 and level 1 of semi-coarsening pattern 0, 3..5 a full line
+ * sweep (four colours, right-hand sides included) along x, y, z.  Overwrites
+ * slot 0's level vectors.  Returns 6 or EMMI_ERROR; n >= 6. */
+int emmi_solver_kernel_rates(emmi_solver_t *s, int reps, double *ms, double *bytes, int n);
 void emmi_timing_enable(int on);
 int emmi_timing_read(double *ms, long *count, double *bytes, int n);
+int emmi_timing_read_layout(double *bytes, int n);
 
 #ifdef __cplusplus
 }

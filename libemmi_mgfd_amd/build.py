@@ -78,7 +78,7 @@ def _run(cmd):
 def build_cli() -> None:
     """libemmi_cli.so and emmi_main (host code only, g++)."""
     cli = os.path.join(CSRC, "cli")
-    flags = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(ROOT, "include"),
+    flags = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-pthread", "-I" + os.path.join(ROOT, "include"),
              "-I" + cli]
     objs = []
     for src in CLI_LIB_SOURCES:
@@ -90,7 +90,7 @@ def build_cli() -> None:
     main_obj = os.path.join(BUILD, "cli_emmi_main.cpp.o")
     _run(["g++"] + flags + ["-c", os.path.join(cli, "emmi_main.cpp"), "-o", main_obj])
     _run(["g++", "-o", CLI_EXE + ".tmp", main_obj, "-L" + HERE, "-lemmi_cli", "-lemmi_mgfd",
-          "-Wl,-rpath,$ORIGIN", "-lm"])
+          "-Wl,-rpath,$ORIGIN", "-lm", "-pthread"])
     os.replace(CLI_EXE + ".tmp", CLI_EXE)
 
 

@@ -252,8 +252,20 @@ int emmi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -
 
 int emmi_set_line_kernel(int mode) { return emmi::set_line_kernel(mode); }
 
+int emmi_solver_kernel_rates(emmi_solver_t* h, int reps, double* ms, double* bytes, int n) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    if (n < 6 || !ms || !bytes) throw std::runtime_error("kernel_rates needs room for 6 entries");
+    return h->s->kernel_rates(reps, ms, bytes);
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
 void emmi_timing_enable(int on) { timing::enable(on != 0); }
 int emmi_timing_read(double* ms, long* count, double* bytes, int n) { return timing::read(ms, count, bytes, n); }
+int emmi_timing_read_layout(double* bytes, int n) { return timing::read_layout(bytes, n); }
 
 }  // extern "C"
 

@@ -25,11 +25,14 @@
 //
 // Extensions (parameters the reference does not read): precision=0|1 (FP64
 // cycles, or FP32 cycles inside FP64 refinement, default 1; either stops at
-// the same FP64 residual test), graphs=0|1, coarse_sweeps=N, nproc=N.
+// the same FP64 residual test), graphs=0|1, coarse_sweeps=N, nproc=N,
+// batch_shots=N (modes 1 and 2: the shots solved together in one survey
+// solve; default 24 / nfreq).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
+#include <thread>
 #include <vector>
 
 #include "arrays.h"
@@ -112,6 +115,84 @@ struct ShotSolve {
       if (st[f].status == EMMI_NONFINITE) emmi_cli_err("%s solve of freq=%g Hz produced non-finite values", what,
                                                        emf->freqs[f]);
     }
+  }
+};
+
+// The shots of a gradient evaluation solved together: each shot keeps its own
+// grid and medium (a copy of emf after its regrid_init, with the rank's data
+// tables bound), and one survey solver holds all of them with one slot per
+// (shot, frequency), slot j*nfreq + f.  Every kernel launch of a cycle serves
+// every unconverged slot; each slot stops at its own convergence test, so it
+// gets the iterates a solve of that shot alone would give.
+struct ShotGroup {
+  int nf = 0, ns = 0;
+  long nv = 0;
+  std::vector<emf_t> E;
+  std::vector<acq_t*> acq;
+  emmi_solver_t* solver = nullptr;
+  std::vector<void*> b, x;
+  std::vector<emmi_stats_t> st;
+
+  // base: the emf of the job (model, frequencies, parameters); emfs[j] is
+  // base with rank j's data tables and verbosity
+  ShotGroup(const std::vector<emf_t>& emfs, const std::vector<acq_t*>& acqs, const emmi_params_t& p)
+      : E(emfs), acq(acqs) {
+    ns = (int)E.size();
+    nf = E[0].nfreq;
+    // the shots' regrids are independent host work (no shared state)
+    std::vector<std::thread> th;
+    for (int j = 0; j < ns; ++j) th.emplace_back([this, j] { regrid_init(acq[j], &E[j]); });
+    for (auto& t : th) t.join();
+    const emf_t& e = E[0];
+    nv = 3L * (e.n1 + 1) * (e.n2 + 1) * (e.n3 + 1);
+    std::vector<const float*> x1(ns), x2(ns), x3(ns), s11(ns), s22(ns), s33(ns);
+    for (int j = 0; j < ns; ++j) {
+      x1[j] = E[j].x1; x2[j] = E[j].x2; x3[j] = E[j].x3;
+      s11[j] = &E[j].sigma11[0][0][0]; s22[j] = &E[j].sigma22[0][0][0]; s33[j] = &E[j].sigma33[0][0][0];
+    }
+    std::vector<int> med(ns * nf);
+    std::vector<float> om(ns * nf);
+    for (int j = 0; j < ns; ++j)
+      for (int f = 0; f < nf; ++f) {
+        med[j * nf + f] = j;
+        om[j * nf + f] = E[j].omegas[f];
+      }
+    solver = emmi_solver_create_survey(e.n1, e.n2, e.n3, ns, x1.data(), x2.data(), x3.data(), s11.data(),
+                                       s22.data(), s33.data(), 0, med.data(), om.data(), ns * nf, &p);
+    if (!solver) emmi_cli_err("solver: %s", emmi_last_error());
+    for (int k = 0; k < ns * nf; ++k) {
+      b.push_back(emmi_device_alloc(sizeof(emmi_c128) * nv));
+      x.push_back(emmi_device_alloc(sizeof(emmi_c128) * nv));
+      if (!b.back() || !x.back()) emmi_cli_err("device allocation of the field vectors failed");
+    }
+    st.resize(ns * nf);
+  }
+  ~ShotGroup() {
+    for (void* q : b) emmi_device_free(q);
+    for (void* q : x) emmi_device_free(q);
+    emmi_solver_destroy(solver);
+    for (int j = 0; j < ns; ++j) regrid_free(&E[j]);
+  }
+  void* bs(int j, int f) { return b[j * nf + f]; }
+  void* xs(int j, int f) { return x[j * nf + f]; }
+  // all slots from zero initial guesses (src/gmg.c:1900); H overwrites the
+  // source vector, as gmg_apply does (src/gmg.c:1929)
+  void solve(const char* what) {
+    const int rc = emmi_solver_solve_batch(solver, ns * nf, const_cast<const void* const*>(b.data()), x.data(),
+                                           b.data(), nullptr, st.data());
+    if (rc == EMMI_ERROR) emmi_cli_err("%s solve: %s", what, emmi_last_error());
+    for (int j = 0; j < ns; ++j)
+      for (int f = 0; f < nf; ++f) {
+        const emmi_stats_t& t = st[j * nf + f];
+        if (E[j].verb)
+          printf("%s solve freq=%g Hz: %d cycles, rnorm/rnorm0=%e%s\n", what, E[j].freqs[f], t.cycles,
+                 t.rnorm0 > 0 ? t.rnorm / t.rnorm0 : 0.0, t.status == EMMI_CONVERGED ? "" : " (not converged)");
+        else if (t.status == EMMI_NOT_CONVERGED)
+          printf("%s solve of shot %d, freq=%g Hz: not converged after %d cycles\n", what, acq[j]->shot_idx[0],
+                 E[j].freqs[f], t.cycles);
+        if (t.status == EMMI_NONFINITE)
+          emmi_cli_err("%s solve of freq=%g Hz produced non-finite values", what, E[j].freqs[f]);
+      }
   }
 };
 
@@ -276,11 +357,18 @@ class Inversion {
     fwi_.iter = 0;
     fwi_.firstgrad = 1;
     fwi_.alpha = 1;
-    // device buffers of the gradient evaluations
+    // shots solved together per survey solve (batch_shots=, extension):
+    // every right-hand side of a group is one slot of the solver (at most
+    // 64), and the default keeps a group near 24 slots
+    batch_shots_ = par_int("batch_shots", std::max(1, 24 / std::max(1, (int)nf)));
+    if (batch_shots_ < 1 || batch_shots_ * (int)nf > 64)
+      emmi_cli_err("batch_shots=%d: need 1 <= batch_shots * nfreq <= 64", batch_shots_);
+    batch_shots_ = std::min(batch_shots_, nproc);
+    // device buffers of the gradient evaluations, per shot of a group
     const size_t ebytes = sizeof(emmi_c64) * 3 * nxyz_;
-    ef_.resize(nf);
-    ea_.resize(nf);
-    for (size_t f = 0; f < nf; ++f) {
+    ef_.resize(nf * batch_shots_);
+    ea_.resize(nf * batch_shots_);
+    for (size_t f = 0; f < ef_.size(); ++f) {
       ef_[f] = emmi_device_alloc(ebytes);
       ea_[f] = emmi_device_alloc(ebytes);
       if (!ef_[f] || !ea_[f]) emmi_cli_err("device allocation of the model-grid fields failed");
@@ -456,7 +544,8 @@ class Inversion {
   std::vector<float> xref_;
   size_t nxyz_ = 0;
   std::vector<Rank> ranks_;
-  std::vector<void*> ef_, ea_;
+  std::vector<void*> ef_, ea_;   // [shot of a group][frequency]
+  int batch_shots_ = 1;
   float *grad_d_ = nullptr, *phess_d_ = nullptr;
 
   void set_verb() { emf_->verb = verb_set_ ? verb_par_ : (iproc == 0); }
@@ -488,43 +577,70 @@ class Inversion {
     std::vector<float> gsum(fwi_.n, 0.0f), hsum(fwi_.preco == 1 ? fwi_.n : 0, 0.0f), part(fwi_.n);
     float fcost_dat = 0;
     int ndata0 = 0;  // rank 0's data count, times nproc below, as rank 0 forms it
-    for (iproc = 0; iproc < nproc; ++iproc) {
+    // the ranks in groups of batch_shots, each group one survey solve; every
+    // step that forms a sum over ranks runs in rank order
+    for (int r0 = 0; r0 < nproc; r0 += batch_shots_) {
+      const int ns = std::min(batch_shots_, nproc - r0);
+      std::vector<emf_t> emfs;
+      std::vector<acq_t*> acqs;
+      for (int j = 0; j < ns; ++j) {
+        iproc = r0 + j;
+        set_verb();
+        bind(ranks_[iproc]);
+        emfs.push_back(*emf);
+        acqs.push_back(&ranks_[iproc].acq);
+      }
+      iproc = r0;
       set_verb();
-      Rank& r = ranks_[iproc];
-      acq_t* acq = &r.acq;
-      bind(r);
       if (emf->verb) printf("--------step 1: forward modelling------------\n");
-      regrid_init(acq, emf);
-      {
-        // the forward and adjoint solves share grid, medium and frequencies
-        ShotSolve S(emf, p_);
-        for (size_t f = 0; f < nf; ++f) check(emmi_inject_source_dev(acq, emf, f, S.b[f], nullptr), "inject_source");
-        S.solve("forward");
+      // the forward and adjoint solves share grids, media and frequencies
+      ShotGroup S(emfs, acqs, p_);
+      std::vector<std::vector<void*>> ef(ns), ea(ns);
+      for (int j = 0; j < ns; ++j) {
+        ef[j].assign(ef_.begin() + (size_t)j * nf, ef_.begin() + (size_t)(j + 1) * nf);
+        ea[j].assign(ea_.begin() + (size_t)j * nf, ea_.begin() + (size_t)(j + 1) * nf);
+        for (size_t f = 0; f < nf; ++f)
+          check(emmi_inject_source_dev(S.acq[j], &S.E[j], f, S.bs(j, f), nullptr), "inject_source");
+      }
+      S.solve("forward");
+      for (int j = 0; j < ns; ++j) {
         for (size_t f = 0; f < nf; ++f) {
-          check(emmi_extract_emf_data_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "extract_emf_data");
-          check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ef_[f], nullptr), "extract_electric_field");
+          check(emmi_extract_emf_data_dev(S.acq[j], &S.E[j], f, S.xs(j, f), S.bs(j, f), nullptr), "extract_emf_data");
+          check(emmi_extract_electric_field_dev(S.acq[j], &S.E[j], f, S.xs(j, f), ef[j][f], nullptr),
+                "extract_electric_field");
         }
+        iproc = r0 + j;
+        set_verb();
+        acq_t* acq = S.acq[j];
+        emf_t* e = &S.E[j];
+        e->verb = emf->verb;
         char fname[32];
         snprintf(fname, sizeof fname, "syn_%04d.txt", acq->shot_idx[iproc]);
-        write_data(acq, emf, fname, emf->dcal_fd);
+        write_data(acq, e, fname, e->dcal_fd);
 
-        if (emf->verb) printf("------step 2: compute fcost + adjoint source---------\n");
-        const float fcost = emmi_data_misfit(acq, emf);
-        write_rmse(acq, emf);
-        const int n = emf->nchrec * emf->nfreq * acq->nrec;
+        if (e->verb) printf("------step 2: compute fcost + adjoint source---------\n");
+        const float fcost = emmi_data_misfit(acq, e);
+        write_rmse(acq, e);
+        const int n = e->nchrec * e->nfreq * acq->nrec;
         printf("isrc=%d, fcost=%g, rmse=%g\n", acq->shot_idx[iproc], fcost, std::sqrt(2. * fcost / n));
         fcost_dat = fcost_dat + fcost;
         if (iproc == 0) ndata0 = n;
-
-        if (emf->verb) printf("-----------step3: adjoint modelling------------\n");
         for (size_t f = 0; f < nf; ++f)
-          check(emmi_inject_adj_source_dev(acq, emf, f, S.x[f], S.b[f], nullptr), "inject_adj_source");
-        S.solve("adjoint");
+          check(emmi_inject_adj_source_dev(acq, e, f, S.xs(j, f), S.bs(j, f), nullptr), "inject_adj_source");
+      }
+      iproc = r0;
+      set_verb();
+      if (emf->verb) printf("-----------step3: adjoint modelling------------\n");
+      S.solve("adjoint");
+      for (int j = 0; j < ns; ++j) {
+        iproc = r0 + j;
+        set_verb();
         for (size_t f = 0; f < nf; ++f)
-          check(emmi_extract_electric_field_dev(acq, emf, f, S.x[f], ea_[f], nullptr), "extract_electric_field");
-
+          check(emmi_extract_electric_field_dev(S.acq[j], &S.E[j], f, S.xs(j, f), ea[j][f], nullptr),
+                "extract_electric_field");
         if (emf->verb) printf("----------step 4: compute gradient --------------\n");
-        check(emmi_build_gradient_dev(acq, emf, &fwi_, ef_.data(), ea_.data(), grad_d_, phess_d_, nullptr),
+        check(emmi_build_gradient_dev(S.acq[j], &S.E[j], &fwi_, ef[j].data(), ea[j].data(), grad_d_, phess_d_,
+                                      nullptr),
               "build_gradient");
         // the sums over ranks (MPI_Allreduce in the reference), rank order
         if (emmi_memcpy_d2h(part.data(), grad_d_, sizeof(float) * fwi_.n)) emmi_cli_err("gradient copy failed");
@@ -534,7 +650,6 @@ class Inversion {
           for (int i = 0; i < fwi_.n; ++i) hsum[i] = hsum[i] + part[i];
         }
       }
-      regrid_free(emf);
     }
     iproc = 0;
     set_verb();

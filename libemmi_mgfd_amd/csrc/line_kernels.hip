@@ -1677,16 +1677,21 @@ size_t line_scratch_bytes(const int* n, int a, int colour, size_t esize) {
 
 // A launch; when timing is on, its events are recorded by the dispatch itself
 // (hipExtLaunchKernelGGL), so their elapsed time is the kernel's alone.
+// bytes = {compulsory, layout} of the launch
+struct Bytes {
+  double comp, layout;
+};
+
 template <typename K, typename... Args>
-static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, double bytes, Args... args) {
+static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, Bytes bytes, Args... args) {
   hipEvent_t e0, e1;
-  timing::kernel_events(cls, bytes, &e0, &e1);
+  timing::kernel_events(cls, bytes.comp, &e0, &e1, bytes.layout);
   if (e0) hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
   else hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
 }
 
 template <typename T, int NT>
-static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_t s, double by) {
+static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_t s, Bytes by) {
   // lines of 64 and more nodes come thousands to a colour: blocks of 256
   // threads let two blocks share a CU (registers, LDS), so the launch needs
   // fewer rounds of blocks
@@ -1704,9 +1709,13 @@ template <typename T>
 static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
   int maxl = 0;
   long maxnodes = 0;
-  double rhs_bytes = 0, solve_bytes = 0, rec_bytes = 0;
+  double rhs_bytes = 0, solve_bytes = 0, rec_bytes = 0, comp = 0;
   const double es = sizeof(typename cvec<T>::type), cs = sizeof(T);
   for (int j = 0; j < P.njobs; j++) {
+    // compulsory bytes of one colour: a quarter of a full sweep's
+    // 3 Ne s + 3 Nc c (u read and written, f read, sbar read)
+    const Level<T>& L = P.job[j].L;
+    comp += 0.25 * (3.0 * 3.0 * (double)L.nc * es + 3.0 * (double)L.n[0] * L.n[1] * L.n[2] * cs);
     const long nl = (long)P.job[j].nlo * P.job[j].nhi;
     const int na = P.job[j].L.n[axis];
     if (nl > maxl) maxl = (int)nl;
@@ -1731,7 +1740,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
       else hipLaunchKernelGGL((k_line_pcr_factor<T, 2>), g, dim3(PCR_MAX_NODES), 0, s, P);
       return;
     }
-    const double by = rhs_bytes + solve_bytes;
+    const Bytes by{comp, rhs_bytes + solve_bytes};
     const dim3 b(4 * PCR_MAX_NODES);
     if (axis == 0) launch_timed(k_line_pcr_solve<T, 0>, g, b, s, timing::LINE, by, P);
     else if (axis == 1) launch_timed(k_line_pcr_solve<T, 1>, g, b, s, timing::LINE, by, P);
@@ -1758,11 +1767,11 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
     const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
     const dim3 g((unsigned)(nlb * npb), P.njobs), b(XT * XT);
-    launch_timed(k_line_rhs_x<T>, g, b, s, timing::LINE_RHS, rhs_bytes, P, nlb);
+    launch_timed(k_line_rhs_x<T>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P, nlb);
   } else {
     const dim3 g((unsigned)((maxnodes + 255) / 256), P.njobs), b(256);
-    if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
-    else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, rhs_bytes, P);
+    if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P);
+    else launch_timed(k_line_rhs<T, 2>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P);
   }
   if (quad) {
     // few lines: four lanes per line half shorten the serial chain
@@ -1775,7 +1784,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
         if (P.job[j].L.n[axis] != nt) nt = 0;
       if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64 && nt != 128) nt = 0;
       // records kept in LDS are neither written to nor read from memory
-      const double by = rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0);
+      const Bytes by{comp, rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0)};
       switch (nt) {
         case 4: launch_quad_fused<T, 4>(P, axis, g, s, by); break;
         case 8: launch_quad_fused<T, 8>(P, axis, g, s, by); break;
@@ -1787,9 +1796,10 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
       }
     } else {
       const dim3 b(128);
-      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
-      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
-      else launch_timed(k_line_solve_q<T, 2, QD, false, 128, 0>, g, b, s, timing::LINE, solve_bytes, P);
+      const Bytes by{comp, solve_bytes};
+      if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, false, 128, 0>, g, b, s, timing::LINE, by, P);
+      else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, false, 128, 0>, g, b, s, timing::LINE, by, P);
+      else launch_timed(k_line_solve_q<T, 2, QD, false, 128, 0>, g, b, s, timing::LINE, by, P);
     }
   } else {
     const dim3 g((maxl + 63) / 64, P.njobs), b(128);
@@ -1799,11 +1809,11 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
       int maxn = 0;
       for (int j = 0; j < P.njobs; j++) maxn = std::max(maxn, P.job[j].L.n[0]);
       const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
-      launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, solve_bytes, P);
-      launch_timed(k_line_solve_x_apply<T>, dim3((unsigned)(nlb * npb), P.njobs), dim3(XT * XT), s, timing::LINE, 0.0,
-                   P, nlb);
-    } else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, solve_bytes, P);
-    else launch_timed(k_line_solve<T, 2>, g, b, s, timing::LINE, solve_bytes, P);
+      launch_timed(k_line_solve<T, 0>, g, b, s, timing::LINE, Bytes{comp, solve_bytes}, P);
+      launch_timed(k_line_solve_x_apply<T>, dim3((unsigned)(nlb * npb), P.njobs), dim3(XT * XT), s, timing::LINE,
+                   Bytes{0.0, 0.0}, P, nlb);
+    } else if (axis == 1) launch_timed(k_line_solve<T, 1>, g, b, s, timing::LINE, Bytes{comp, solve_bytes}, P);
+    else launch_timed(k_line_solve<T, 2>, g, b, s, timing::LINE, Bytes{comp, solve_bytes}, P);
   }
 }
 

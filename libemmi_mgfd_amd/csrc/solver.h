@@ -148,6 +148,11 @@ class Solver {
   // `lev` of pattern `pat`, with the frequency of slot 0, in the working
   // precision T of the solver
   template <typename T> void line_sweep_on(int pat, int lev, int axis, int iter, void* u, const void* f);
+  // average time of one launch (ms) and compulsory bytes of the level-0
+  // kernels of slot 0 in the cycle precision, each launched reps times:
+  // residual, restriction and prolongation (pattern 0), a full line sweep
+  // (four colours) along x, y, z; returns 6
+  int kernel_rates(int reps, double* ms, double* bytes);
 
  private:
   void init(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2,
@@ -160,6 +165,7 @@ class Solver {
   template <typename T> void smooth(int pat, int lev, int iter);
   template <typename T> void line_sweep(int pat, int lev, int axis, int iter);
   template <typename T> void point_sweep(int pat, int lev, int iter);
+  template <typename T> void kernel_rates_t(int reps, double* ms, double* bytes);
   template <typename T> bool coarsens(int pat, int lev);
   bool exact_coarsest(int pat) const;
   template <typename T> LineFactors& factors(Slot& S, int pat, int lev, int axis);

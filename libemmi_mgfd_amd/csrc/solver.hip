@@ -477,6 +477,53 @@ void Solver::smooth(int pat, int lev, int iter) {
 }
 
 // ---------------------------------------------------------------------------
+// kernel rates: the level-0 kernels of slot 0 timed one by one
+// ---------------------------------------------------------------------------
+template <typename T>
+void Solver::kernel_rates_t(int reps, double* ms, double* bytes) {
+  Slot& S = slots_[0];
+  const DevLevel<T>& L = hier<T>(S.med).pat[0][0];
+  const DevLevel<T>& C = hier<T>(S.med).pat[0][1];
+  const Level<T> V = L.view(S.omu), VC = C.view(S.omu);
+  const Transfer<T> X = C.transfer();
+  const double es = sizeof(typename cvec<T>::type), cs = sizeof(T);
+  const double ne = 3.0 * (double)L.nc, nc = (double)L.n[0] * L.n[1] * L.n[2], nec = 3.0 * (double)C.nc;
+  const double sweep = 3.0 * ne * es + 3.0 * nc * cs;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  for (int a = 0; a < 3; a++) factors<T>(S, 0, 0, a);
+  auto timed = [&](int k, double by, auto&& launch) {
+    launch();   // warm
+    HIPCHECK(hipEventRecord(e0, s_));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHECK(hipEventRecord(e1, s_));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    ms[k] = t / reps;
+    bytes[k] = by;
+  };
+  timed(0, ne * (3.0 * es) + 3.0 * nc * cs, [&] { launch_residual<T>(V, S.u[0], S.f[0], S.r[0], S.partial, s_); });
+  timed(1, (ne + nec) * es, [&] { launch_restrict<T>(V, VC, X, S.r[0], S.f[1], S.u[1], s_); });
+  timed(2, (2.0 * ne + nec) * es, [&] { launch_prolong<T>(V, VC, X, S.u[1], S.u[0], s_); });
+  for (int a = 0; a < 3; a++)
+    timed(3 + a, sweep, [&] { line_sweep_on<T>(0, 0, a, 0, S.u[0], S.f[0]); });
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+}
+
+int Solver::kernel_rates(int reps, double* ms, double* bytes) {
+  if (reps < 1) throw std::runtime_error("kernel_rates needs reps >= 1");
+  if (lmax_ < 2) throw std::runtime_error("kernel_rates needs a coarse level");
+  HIPCHECK(hipDeviceSynchronize());
+  if (p_.precision == 0) kernel_rates_t<double>(reps, ms, bytes);
+  else kernel_rates_t<float>(reps, ms, bytes);
+  HIPCHECK(hipStreamSynchronize(s_));
+  return 6;
+}
+
+// ---------------------------------------------------------------------------
 // convergence tests
 // ---------------------------------------------------------------------------
 void Solver::norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,

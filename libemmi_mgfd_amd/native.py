@@ -119,8 +119,13 @@ def lib() -> C.CDLL:
         L.emmi_set_line_kernel.restype = C.c_int
         L.emmi_set_line_kernel.argtypes = [C.c_int]
         L.emmi_timing_enable.argtypes = [C.c_int]
+        L.emmi_solver_kernel_rates.restype = C.c_int
+        L.emmi_solver_kernel_rates.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.c_int]
         L.emmi_timing_read.restype = C.c_int
         L.emmi_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double), C.c_int]
+        L.emmi_timing_read_layout.restype = C.c_int
+        L.emmi_timing_read_layout.argtypes = [C.POINTER(C.c_double), C.c_int]
         _LIB = L
     return _LIB
 
@@ -279,6 +284,18 @@ class Solver:
         n = lib().emmi_solver_history_slot(self.h, int(slot), buf, 4096)
         return np.array(buf[:min(n, 4096)])
 
+    KERNEL_RATES = ["residual", "restriction", "prolongation", "line_sweep_x", "line_sweep_y", "line_sweep_z"]
+
+    def kernel_rates(self, reps=20):
+        """{kernel: (ms per launch, compulsory bytes)} of the level-0 kernels
+        of slot 0 (emmi_solver_kernel_rates)."""
+        ms = (C.c_double * 8)()
+        by = (C.c_double * 8)()
+        n = lib().emmi_solver_kernel_rates(self.h, int(reps), ms, by, 8)
+        if n < 0:
+            raise NativeError(last_error())
+        return {k: (ms[i], by[i]) for i, k in enumerate(self.KERNEL_RATES[:n])}
+
     def levels(self):
         npat = C.c_int()
         lmax = lib().emmi_solver_levels(self.h, C.byref(npat))
@@ -331,9 +348,14 @@ TIMING_CLASSES = ["residual", "line_solve", "restriction", "prolongation", "othe
 
 
 def timing_read():
-    """{class: (total ms, launches, algorithmic bytes)} since timing_enable."""
+    """{class: (total ms, launches, compulsory bytes, layout bytes)} since
+    timing_enable: compulsory bytes read each input once and write each
+    output once at the launch's level shape; layout bytes add what the
+    kernels' own data layout moves (the line kernels' stored records)."""
     ms = (C.c_double * 8)()
     cnt = (C.c_long * 8)()
     by = (C.c_double * 8)()
+    lay = (C.c_double * 8)()
     n = lib().emmi_timing_read(ms, cnt, by, 8)
-    return {TIMING_CLASSES[i]: (ms[i], cnt[i], by[i]) for i in range(min(n, len(TIMING_CLASSES)))}
+    lib().emmi_timing_read_layout(lay, 8)
+    return {TIMING_CLASSES[i]: (ms[i], cnt[i], by[i], lay[i]) for i in range(min(n, len(TIMING_CLASSES)))}
