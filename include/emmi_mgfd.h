@@ -248,6 +248,17 @@ int emmi_extract_emf_data_dev(acq_t *acq, emf_t *emf, int ifreq, const void *x_d
  * reference's order, b += curl M runs on the device; bit-identical to the
  * host entry point. */
 int emmi_inject_adj_source_dev(acq_t *acq, emf_t *emf, int ifreq, void *x_dev, void *b_dev, void *stream);
+/* No counterpart in the reference: emmi_inject_adj_source_dev for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, same results bit
+ * for bit; the device tables of the call (entry indices and values, dual
+ * widths) go to a static device array of 2 MiB that each GPU holds for the
+ * life of the process (enough for about 900 receivers with all six
+ * channels).  A call whose tables exceed it returns -1 with an
+ * emmi_path_last_error() naming the sizes.  It allocates no device memory;
+ * calls that overlap on one GPU (other host threads, other streams) share
+ * that GPU's array and race. */
+int emmi_inject_adj_source_dev_noalloc(acq_t *acq, emf_t *emf, int ifreq, void *x_dev, void *b_dev, void *stream);
 /* extract_electric_field (src/inject_extract.c:477-635) on the device: E of
  * the device field vector x_dev (complex double, 3(n1+1)(n2+1)(n3+1)) on the
  * model's cell centres, written to e_dev as complex float [3][nz][ny][nx]

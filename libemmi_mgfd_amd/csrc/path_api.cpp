@@ -92,8 +92,12 @@ extern "C" int emmi_extract_emf_data_dev(acq_t* acq, emf_t* emf, int ifreq, cons
   }
 }
 
-// src/inject_extract.c:124-314
-extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, void* x_dev, void* b_dev, void* stream) {
+namespace {
+// src/inject_extract.c:124-314: the scatters accumulated on the host in the
+// reference's order, b += curl M on the device.  The device tables (entry
+// indices, values, dual widths) go to library-allocated scratch, or, with
+// noalloc, to the static scratch of this device.
+int adj_source(acq_t* acq, emf_t* emf, int ifreq, void* x_dev, void* b_dev, void* stream, bool noalloc) {
   try {
     hipStream_t s = (hipStream_t)stream;
     std::vector<long> bi, xi;
@@ -103,9 +107,28 @@ extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, voi
     const long n = 3L * (n1 + 1) * (n2 + 1) * (n3 + 1);
     for (long i : bi) if (i < 0 || i >= n) throw std::runtime_error("adjoint source entry outside the vector");
     for (long i : xi) if (i < 0 || i >= n) throw std::runtime_error("adjoint source entry outside the vector");
-    const size_t nb = bi.size(), nx = xi.size();
-    long* di = (long*)g_idx.get(sizeof(long) * (nb + nx + 1));
-    emmi_c128* dv = (emmi_c128*)g_val.get(sizeof(emmi_c128) * (nb + nx + 1));
+    const size_t nb = bi.size(), nx = xi.size(), nw = (size_t)n1 + n2 + n3 + 3;
+    auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bidx = pad(sizeof(long) * (nb + nx + 1)), bval = pad(sizeof(emmi_c128) * (nb + nx + 1));
+    const size_t bw = pad(sizeof(float) * nw);
+    long* di;
+    emmi_c128* dv;
+    float* dw;
+    if (noalloc) {
+      if (bidx + bval + bw > emmi::NOALLOC_SCRATCH_BYTES)
+        throw std::runtime_error("the adjoint source of " + std::to_string(acq->nrec) + " receivers needs " +
+                                 std::to_string(bidx + bval + bw) + " bytes of device tables; the static scratch " +
+                                 "holds " + std::to_string(emmi::NOALLOC_SCRATCH_BYTES) +
+                                 " (use emmi_inject_adj_source_dev)");
+      char* base = (char*)emmi::noalloc_scratch();
+      di = (long*)base;
+      dv = (emmi_c128*)(base + bidx);
+      dw = (float*)(base + bidx + bval);
+    } else {
+      di = (long*)g_idx.get(bidx);
+      dv = (emmi_c128*)g_val.get(bval);
+      dw = (float*)g_tab.get(bw);
+    }
     std::vector<long> idx(bi);
     idx.insert(idx.end(), xi.begin(), xi.end());
     std::vector<emmi_c128> val(bv);
@@ -116,7 +139,6 @@ extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, voi
     dd.insert(dd.end(), emf->d1, emf->d1 + n1 + 1);
     dd.insert(dd.end(), emf->d2, emf->d2 + n2 + 1);
     dd.insert(dd.end(), emf->d3, emf->d3 + n3 + 1);
-    float* dw = (float*)g_tab.get(sizeof(float) * dd.size());
     check(hipMemcpyAsync(dw, dd.data(), sizeof(float) * dd.size(), hipMemcpyHostToDevice, s), "h2d");
     emmi::launch_zero_c128(b_dev, n, s);
     emmi::launch_zero_c128(x_dev, n, s);
@@ -130,6 +152,18 @@ extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, voi
     g_path_err = e.what();
     return -1;
   }
+}
+}  // namespace
+
+// src/inject_extract.c:124-314
+extern "C" int emmi_inject_adj_source_dev(acq_t* acq, emf_t* emf, int ifreq, void* x_dev, void* b_dev, void* stream) {
+  return adj_source(acq, emf, ifreq, x_dev, b_dev, stream, false);
+}
+
+// the same with the device tables in the static scratch: no allocation
+extern "C" int emmi_inject_adj_source_dev_noalloc(acq_t* acq, emf_t* emf, int ifreq, void* x_dev, void* b_dev,
+                                                  void* stream) {
+  return adj_source(acq, emf, ifreq, x_dev, b_dev, stream, true);
 }
 
 namespace {

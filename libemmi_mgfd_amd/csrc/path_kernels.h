@@ -4,7 +4,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 namespace emmi {
+// Static device scratch of the entry point that allocates nothing
+// (emmi_inject_adj_source_dev_noalloc): a file-scope __device__ array of
+// NOALLOC_SCRATCH_BYTES in every GPU's memory for the life of the process.
+// noalloc_scratch() returns its address on the current device (looked up
+// once per device with hipGetSymbolAddress); throws on failure.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)2 << 20;
+void* noalloc_scratch();
+
 void launch_zero_c128(void* v, long n, hipStream_t s);
 void launch_scatter_set(void* v, const long* idx, const void* val, int n, hipStream_t s);
 void launch_gather(const void* v, const long* idx, void* out, int n, hipStream_t s);

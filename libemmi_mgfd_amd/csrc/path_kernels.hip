@@ -6,10 +6,37 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <stdexcept>
+#include <string>
 
 #include "path_kernels.h"
 
 namespace emmi {
+
+// The scratch of the entry point that allocates nothing.  In host code the
+// name `scratch` stands for the symbol, not for a device address: host code
+// uses only the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+void* noalloc_scratch() {
+  constexpr int MAXDEV = 64;
+  static void* addr[MAXDEV] = {};
+  static std::mutex mu;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= MAXDEV) throw std::runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress of the static scratch: ") + hipGetErrorString(e));
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
 
 __global__ void k_zero_c128(double2* __restrict__ v, long n) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;

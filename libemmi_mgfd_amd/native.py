@@ -84,6 +84,8 @@ def lib() -> C.CDLL:
         L.emmi_path_last_error.restype = C.c_char_p
         L.emmi_inject_adj_source_dev.restype = C.c_int
         L.emmi_inject_adj_source_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.emmi_inject_adj_source_dev_noalloc.restype = C.c_int
+        L.emmi_inject_adj_source_dev_noalloc.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_extract_electric_field_dev.restype = C.c_int
         L.emmi_extract_electric_field_dev.argtypes = [PA, PE, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.emmi_extract_magnetic_field_dev.restype = C.c_int
