@@ -19,7 +19,7 @@
 
 namespace emmi {
 
-constexpr int LINE_PACK_MAX = 8;
+constexpr int LINE_PACK_MAX = 16;
 
 // Value types of the line factor records (LineFac) and of the right-hand
 // side / substitution records (LineScr) for cycles of storage precision T.
@@ -46,6 +46,9 @@ struct LinePack {
   LineJob<T> job[LINE_PACK_MAX];
   int njobs;
 };
+// a pack is a kernel argument: it must stay within the 4 KiB of kernel
+// arguments a launch takes
+static_assert(sizeof(LinePack<double>) <= 4096 && sizeof(LinePack<float>) <= 4096, "line pack exceeds 4 KiB");
 
 // number of lines of a colour, split by perpendicular axis (low stride first)
 void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi);

@@ -20,7 +20,7 @@ struct Transfer {
 // is the base of the slot's geometry block (W, IW, H, X, D per axis, gstride
 // apart), sbar its edge medium and wr the base of its transfer weights
 // (wstride apart per axis).
-constexpr int MG_BATCH_MAX = 8;
+constexpr int MG_BATCH_MAX = 16;
 template <typename T>
 struct SlotVecs {
   const typename cvec<T>::type* in0[MG_BATCH_MAX];
@@ -33,6 +33,8 @@ struct SlotVecs {
   const double* wr[MG_BATCH_MAX];
   long gstride, wstride;
 };
+static_assert(sizeof(SlotVecs<double>) + 2 * sizeof(Level<double>) + sizeof(Transfer<double>) <= 4096,
+              "batched launch arguments exceed 4 KiB");
 
 template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s);

@@ -426,7 +426,7 @@ template <typename T>
 void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s) {
   using V = typename cvec<T>::type;
-  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("residual batch of 1..8 slots");
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("residual batch of 1..MG_BATCH_MAX slots");
   SlotVecs<T> B = {};
   slot_levels(B, Ls, n);
   for (int k = 0; k < n; k++) {
@@ -465,7 +465,7 @@ template <typename T>
 void launch_point_gs_batch(const Level<T>* Ls, int n, void* const* u, const void* const* f, const double* omu,
                            int iter, hipStream_t s) {
   using V = typename cvec<T>::type;
-  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("point smoother batch of 1..8 slots");
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("point smoother batch of 1..MG_BATCH_MAX slots");
   SlotVecs<T> B = {};
   slot_levels(B, Ls, n);
   const Level<T>& L = Ls[0];
@@ -494,7 +494,7 @@ template <typename T>
 void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* rf,
                            void* const* fc, void* const* uc, hipStream_t s) {
   using V = typename cvec<T>::type;
-  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("restriction batch of 1..8 slots");
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("restriction batch of 1..MG_BATCH_MAX slots");
   SlotVecs<T> B = {};
   slot_transfers(B, Xs, n);
   const Transfer<T>& X = Xs[0];
@@ -518,7 +518,7 @@ template <typename T>
 void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* uc,
                           void* const* uf, hipStream_t s) {
   using V = typename cvec<T>::type;
-  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("prolongation batch of 1..8 slots");
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("prolongation batch of 1..MG_BATCH_MAX slots");
   SlotVecs<T> B = {};
   slot_transfers(B, Xs, n);
   const Transfer<T>& X = Xs[0];

@@ -76,3 +76,37 @@ def test_config3_pairs_match_converged_reference():
             assert e_strong < 1e-5, (shot, f, name)
             worst = max(worst, e_strong)
     print(f"  worst {worst:.2e}", flush=True)
+
+
+def test_survey_packs_of_sixteen_equal_single_solves():
+    """18 right-hand sides (9 shots x 2 frequencies) in one solver: each
+    launch serves up to 16 of them, so the cycles run one full pack and one
+    of 2; every right-hand side still gets its single-solve iterates."""
+    s = cli_survey()
+    xs = np.linspace(-1500.0, 1500.0, 3)
+    s.sources = [(float(x), float(y), 450.0, 15.0 * k, 2.0) for k, (x, y) in
+                 enumerate((x, y) for y in (-300.0, 0.0, 300.0) for x in xs)]
+    s.table = [(i + 1, r + 1) for i in range(len(s.sources)) for r in range(len(s.receivers))]
+    s.freqs = [0.5, 1.25]
+    print("9 shots x 2 frequencies, one survey solver", flush=True)
+    run = survey.SurveyRun(s, precision=1, tol=1e-8, ncycle=80, max_slots=18)
+    assert len(run.chunks) == 1 and len(run.chunks[0].pairs) == 18
+    run.run()
+    L = native.lib()
+    for shot in (0, 4, 8):
+        sc = carriers.ShotCarriers(s, shot)
+        L.regrid_init(sc.acq, sc.emf)
+        g = sc.grid()
+        x1, x2, x3 = sc.x_arrays()
+        nb = 16 * sc.nvec
+        for f in range(2):
+            S = native.Solver((s.n1, s.n2, s.n3), x1, x2, x3, g["sigma11"], g["sigma22"], g["sigma33"],
+                              sc.omegas[f], precision=1, tol=1e-8, ncycle=80)
+            b = native.DeviceBuffer(nb)
+            x = native.DeviceBuffer(nb)
+            assert L.emmi_inject_source_dev(sc.acq, sc.emf, f, b.ptr, None) == 0
+            st = S.solve_device(b.ptr, x.ptr, b.ptr)
+            assert L.emmi_extract_emf_data_dev(sc.acq, sc.emf, f, x.ptr, b.ptr, None) == 0
+            r = run.results[(shot, f)]
+            assert r["status"] == 0 and r["cycles"] == st.cycles
+            np.testing.assert_array_equal(r["dcal"], sc.dcal[:, f, :])
