@@ -56,9 +56,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per line-sweep launch over its compulsory bytes, measured
 # with two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) on the
 # mixed+batched 128^3 step, each scaled by the same counter on the run's
-# device-to-device copies of known size (see the traffic summary of the line
-# sweep in profiles/); None until measured for the current kernels
-LINE_SWEEP_TRAFFIC_RATIO = None
+# 103 MB device-to-device copies (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00):
+# 85.20 MB per launch against 8.88 MB compulsory (the stored FP64 factor
+# records dominate)
+LINE_SWEEP_TRAFFIC_RATIO = 85.20 / 8.88
 
 
 def parse():
