@@ -1756,9 +1756,11 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     return;
   }
   // along x the quad-lane kernel's updates of u are scattered rows (see
-  // k_line_solve_x_apply): the finest FP32 x sweeps keep the one-lane kernel
+  // k_line_solve_x_apply): the finest FP32 x sweeps of several right-hand
+  // sides keep the one-lane kernel (3 slots at 128^3: 1% faster), a single
+  // right-hand side takes the quad-lane one (430 against 567 us per sweep)
   const bool quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>() &&
-                                           (axis != 0 || maxl <= 2048 || sizeof(T) == 8));
+                                           (axis != 0 || maxl <= 2048 || sizeof(T) == 8 || P.njobs == 1));
   const bool fuse = quad && maxl <= fuse_max_lines<T>();
   if (fuse) {
     // the substitution kernel computes its own right-hand sides
