@@ -69,7 +69,7 @@ def test_config2_mixed_matches_fp64_at_bench_tolerance():
         assert err.max() < 1e-5, name
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-10)])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-10)])
 def test_config2_receivers_match_converged_reference(precision, tol, line_kernel):
     s = configs.config2()
     sc = carriers.ShotCarriers(s)
