@@ -28,8 +28,9 @@ Further lines of the JSON object: per-kernel rates of the plain FP64
 level-0 kernels (``plain_kernels``); the batched survey (25 shots x 4
 frequencies, each frequency on its own multigrid size, ``survey``); one
 mode-2 gradient at survey scale (25 shots x 3 frequencies on 160x160x128,
-``gradient``); the reference's CPU path on a bounded sample
-(``cpu_baseline``).
+``gradient``); the rates of the initial-guess network's BF16 MFMA
+convolutions against the dense BF16 peak (``x0net_conv``); the reference's
+CPU path on a bounded sample (``cpu_baseline``).
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one
 GPU.
@@ -159,7 +160,11 @@ class ShotRun:
                     self._inject(f)
                 x0 = [p.ptr for p in self.x0]
                 extra = 1
+                # the solve from the guess continues the pattern sequence
+                # after the cycle that made its input
+                S.set_first_cycle(1)
             sts = S.solve_batch_device([b.ptr for b in self.b], [x.ptr for x in self.x], [b.ptr for b in self.b], x0)
+            S.set_first_cycle(0)
             for f in range(nf):
                 self._extract(f)
                 st = sts[f]
@@ -420,6 +425,10 @@ def main():
     if not a.no_kernels:
         out["plain_kernels"] = {"level": "128^3 level 0, complex FP64", **kernel_rates(survey, 0)}
         out["mixed_kernels"] = {"level": "128^3 level 0, FP32 storage", **kernel_rates(survey, 1)}
+        # the initial-guess network's BF16 MFMA convolutions, one training
+        # step (forward, input and weight gradients) on the 128^3 grid
+        from libemmi_mgfd_amd import x0net
+        out["x0net_conv"] = x0net.conv_rates((survey.n3, survey.n2, survey.n1))
     if not a.no_survey:
         out["survey"] = survey_leg()
         log("survey", out["survey"]["solves_per_s"])

@@ -216,6 +216,12 @@ int emmi_solver_history_slot(emmi_solver_t *s, int slot, double *out, int maxn);
 int emmi_solver_slots(emmi_solver_t *s);
 /* cycle limit of the following solves (the ncycle of the parameters) */
 int emmi_solver_set_ncycle(emmi_solver_t *s, int ncycle);
+/* No counterpart in the reference, whose solves always start at cycle 0
+ * (src/gmg.c:1911-1926): cycle k of the following solves uses the
+ * semi-coarsening pattern (first + k) % 3 instead of k % 3, so a solve that
+ * continues from the iterate of `first` earlier cycles keeps the reference's
+ * pattern sequence.  Default 0. */
+int emmi_solver_set_first_cycle(emmi_solver_t *s, int first);
 int emmi_solver_levels(emmi_solver_t *s, int *npatterns);
 void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
 
@@ -327,6 +333,11 @@ int emmi_x0_pack_bf16(const float *in, long nv, int C, int Cp, void *out, void *
  * builds the training targets from a solved field E and x1, with loss
  * weights 1 (dL) and wp (phasor) where |E| and |x1| exceed eref. */
 int emmi_x0_out_to_edges(const float *o, const void *x1, int n1, int n2, int n3, void *x0, void *stream);
+/* the same with each edge's correction averaged over the four cells that
+ * share the edge; cells with s11 < sig_air (float conductivities [n3][n2][n1],
+ * device, may be NULL) take no correction */
+int emmi_x0_out_to_edges_avg(const float *o, const void *x1, const float *s11, float sig_air, int n1, int n2, int n3,
+                             void *x0, void *stream);
 int emmi_x0_edges_to_target(const void *E, const void *x1, int n1, int n2, int n3, double eref, float wp, float *tgt,
                             float *mask, void *stream);
 /* input features [n1*n2*n3][32] from the medium, the source offset in skin

@@ -293,4 +293,73 @@ def config5(true_model: bool = True) -> Survey:
                   rh, rh.copy(), rv, srcs, recs, table, [0.25, 0.75, 1.25], 160, 160, 128, bathy=bathy)
 
 
-CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config5": config5, "tiny": tiny}
+CONFIG4_TRAIN = range(0, 2000)
+CONFIG4_TEST = range(2000, 2200)
+
+
+def config4(seed: int, n: int = 96) -> Survey:
+    """One random earth of the learned-initial-guess data set (seeds 0-1999
+    train, 2000-2199 test), drawn from numpy's default generator seeded with
+    ``seed``.
+
+    Input model 101x101x81 cells of 100/100/50 m, origin (-5050, -5050, 0):
+    sea water (0.3125 ohm-m) down to a seafloor at depth d0, uniform in
+    500-2000 m, plus a sinusoidal relief of amplitude up to 150 m (wavelengths
+    2-12 km, random phases); below it 3-6 flat layers whose interfaces lie
+    uniformly between d0 + 100 m and 4 km, each with rho_h log-uniform in
+    0.5-5 ohm-m and rho_v / rho_h uniform in 1-3; then 0-3 resistive boxes of
+    20-200 ohm-m (log-uniform), 0.5-5 km wide in x and y, 50-300 m thick, with
+    their centre at x, y uniform in +-4 km and 200-2000 m below d0 (only cells
+    below the seafloor take them).  One x-directed dipole 50 m above the
+    seafloor at x, y uniform in +-2 km; one frequency, log-uniform in
+    0.1-2 Hz; 31 inline receivers on the seafloor within 3 km of the source
+    (every 200 m).  Multigrid n^3 (96^3), stretched about the source."""
+    rng = np.random.default_rng(int(seed))
+    nx = ny = 101
+    nz = 81
+    dx = dy = 100.0
+    dz = 50.0
+    ox = oy = -5050.0
+    d0 = rng.uniform(500.0, 2000.0)
+    amp = rng.uniform(0.0, 150.0)
+    lx, ly = rng.uniform(2000.0, 12000.0, 2)
+    px, py = rng.uniform(0.0, 2.0 * np.pi, 2)
+
+    def seafloor(x, y):
+        return d0 + amp * np.sin(2.0 * np.pi * x / lx + px) * np.cos(2.0 * np.pi * y / ly + py)
+
+    xc, yc, zc = _centres(nx, dx, ox), _centres(ny, dy, oy), _centres(nz, dz, 0.0)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    sf = seafloor(X, Y)
+    nlay = int(rng.integers(3, 7))
+    tops = np.sort(rng.uniform(d0 + 100.0, 4000.0, nlay - 1))
+    lay_h = np.exp(rng.uniform(np.log(0.5), np.log(5.0), nlay))
+    lay_v = lay_h * rng.uniform(1.0, 3.0, nlay)
+    idx = np.searchsorted(tops, Z)
+    below = Z >= sf
+    rh = np.where(below, lay_h[idx], RHO_WATER)
+    rv = np.where(below, lay_v[idx], RHO_WATER)
+    for _ in range(int(rng.integers(0, 4))):
+        rho = float(np.exp(rng.uniform(np.log(20.0), np.log(200.0))))
+        wx, wy = rng.uniform(500.0, 5000.0, 2)
+        th = rng.uniform(50.0, 300.0)
+        cx, cy = rng.uniform(-4000.0, 4000.0, 2)
+        cz = d0 + rng.uniform(200.0, 2000.0)
+        m = (np.abs(X - cx) < 0.5 * wx) & (np.abs(Y - cy) < 0.5 * wy) & (np.abs(Z - cz) < 0.5 * th) & below
+        rh = np.where(m, rho, rh)
+        rv = np.where(m, rho, rv)
+    sx, sy = (float(v) for v in rng.uniform(-2000.0, 2000.0, 2))
+    freq = float(np.exp(rng.uniform(np.log(0.1), np.log(2.0))))
+    zs = float(seafloor(sx, sy)) - 50.0
+    recs = [(float(x), sy, float(seafloor(x, sy)), 0.0, 0.0) for x in sx + np.arange(-3000.0, 3000.0 + 1, 200.0)]
+    rh = rh.astype(np.float32)
+    bathy = seafloor(X[0], Y[0]).astype(np.float32)
+    s = Survey(f"config4_{seed}", nx, ny, nz, dx, dy, dz, ox, oy, 0.0, rh, rh.copy(), rv.astype(np.float32),
+               [(sx, sy, zs, 0.0, 0.0)], recs, [(1, i + 1) for i in range(len(recs))], [freq], n, n, n,
+               bathy=bathy)
+    s.extra["seed"] = int(seed)
+    return s
+
+
+CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config4": config4, "config5": config5,
+           "tiny": tiny}

@@ -201,6 +201,13 @@ int emmi_solver_set_ncycle(emmi_solver_t* h, int ncycle) {
   return 0;
 }
 
+int emmi_solver_set_first_cycle(emmi_solver_t* h, int first) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  if (first < 0) { set_err("negative cycle index"); return EMMI_ERROR; }
+  h->s->set_first_cycle(first % 3);
+  return 0;
+}
+
 int emmi_solver_history_slot(emmi_solver_t* h, int slot, double* out, int maxn) {
   if (!h || slot < 0 || slot >= (int)h->hist.size()) return 0;
   const std::vector<double>& v = h->hist[slot];

@@ -91,6 +91,10 @@ template <typename T> constexpr int ring_depth() { return sizeof(T) == 4 ? EMMI_
 #ifndef EMMI_LINE_FUSE_THREADS
 #define EMMI_LINE_FUSE_THREADS 512
 #endif
+// the same for lines of 64 nodes and more
+#ifndef EMMI_LINE_FUSE_THREADS_LONG
+#define EMMI_LINE_FUSE_THREADS_LONG 256
+#endif
 template <typename T> constexpr int fuse_max_lines() {
   return sizeof(T) == 4 ? EMMI_LINE_FUSE_MAX_LINES_F32 : EMMI_LINE_FUSE_MAX_LINES_F64;
 }
@@ -1279,9 +1283,13 @@ __device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restri
 }
 
 // fused launches with lines of NT nodes keep the block's line records in LDS
+// when they take at most EMMI_QUAD_LDS_KB KiB (build-time tunable)
+#ifndef EMMI_QUAD_LDS_KB
+#define EMMI_QUAD_LDS_KB 80
+#endif
 template <typename T>
 __host__ __device__ constexpr bool quad_lds_records(int nt) {
-  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= 80 * 1024;
+  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= EMMI_QUAD_LDS_KB * 1024;
 }
 
 // NT > 0: every job of the launch has lines of NT nodes; up to 64 nodes the
@@ -1698,7 +1706,7 @@ static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_
   // the 128-node lines keep their loop (not unrolled): a deeper ring pays
   // there in FP32 cycles (in FP64 it spills)
   constexpr int QD = NT == 128 && sizeof(T) == 4 ? 5 : EMMI_LINE_QUAD_RING;
-  constexpr int FT = NT >= 64 ? 256 : EMMI_LINE_FUSE_THREADS;
+  constexpr int FT = NT >= 64 ? EMMI_LINE_FUSE_THREADS_LONG : EMMI_LINE_FUSE_THREADS;
   const dim3 b(FT);
   if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
   else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);

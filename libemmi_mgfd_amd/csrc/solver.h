@@ -137,6 +137,9 @@ class Solver {
   void set_omega(int slot, float omega);
   // cycle limit of the following solves
   void set_ncycle(int ncycle) { p_.ncycle = ncycle; }
+  // cycle k of the following solves uses semi-coarsening pattern
+  // (first + k) % 3 (default 0, the reference's order)
+  void set_first_cycle(int first) { first_cycle_ = first; }
   const int* dims(int pat, int lev) const { return dims_[pat][lev].data(); }
 
   // kernel-level access for the element-wise checks (hierarchies of slot 0's grid)
@@ -198,6 +201,7 @@ class Solver {
   // state of the running batch
   std::vector<int> act_;      // active slots
   int icycle_ = 0;
+  int first_cycle_ = 0;
   bool r0_ready_ = false;     // r[0] holds f - A u of the level-0 test (FP64 form)
   bool rnorm0_in_cycle_ = false;  // rnorm0 is the cycle-0 in-cycle norm (FP64 V/W cycles)
   // captured cycle bodies, keyed by pattern and active-slot mask; valid for

@@ -838,7 +838,7 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
   const int kind = p_.cycleopt;
   rnorm0_in_cycle_ = p_.precision == 0 && kind != 2;
   for (icycle_ = 0; icycle_ < p_.ncycle && !act_.empty(); icycle_++) {
-    const int pat = p_.isemicoarsen ? icycle_ % 3 : 0;
+    const int pat = p_.isemicoarsen ? (icycle_ + first_cycle_) % 3 : 0;
     const bool test_first = p_.precision != 0 || (kind == 2 && coarsens<double>(pat, 0));
     if (test_first) {
       // the convergence test opens the cycle (src/gmg.c:1629-1630 for the

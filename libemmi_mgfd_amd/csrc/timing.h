@@ -7,7 +7,11 @@
 namespace emmi {
 namespace timing {
 
-enum Class { RESIDUAL = 0, LINE = 1, RESTRICT = 2, PROLONG = 3, OTHER = 4, LINE_RHS = 5, NCLASS = 6 };
+// CONV / CONV_DW: the initial-guess network's convolutions (forward and
+// input gradient / weight gradient); their "bytes" are algorithmic flops,
+// 2 * 27 * Cin * Cout per voxel
+enum Class { RESIDUAL = 0, LINE = 1, RESTRICT = 2, PROLONG = 3, OTHER = 4, LINE_RHS = 5, CONV = 6, CONV_DW = 7,
+             NCLASS = 8 };
 
 // Every launch carries two byte counts: its compulsory bytes (each input read
 // once and each output written once at its level's shape; a full line sweep,

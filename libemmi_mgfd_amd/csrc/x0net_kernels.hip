@@ -15,11 +15,13 @@
 // A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16] (j < 8) and receives
 // D[4(l/16)+i][l%16] (i < 4).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <algorithm>
 #include <cmath>
 
 #include "emmi_mgfd.h"
+#include "timing.h"
 
 namespace {
 
@@ -387,6 +389,48 @@ __global__ void k_out_to_edges(const float* __restrict__ o, const double2* __res
   x0[t] = v;
 }
 
+// The same with the correction of each edge averaged over the four cells
+// that share the edge, so the guess varies as smoothly as x1 between edges;
+// cells whose conductivity s11 is below sig_air (the air) take no correction
+// (s11 may be null: no cell is excluded).
+__global__ void k_out_to_edges_avg(const float* __restrict__ o, const double2* __restrict__ x1,
+                                   const float* __restrict__ s11, float sig_air, int n1, int n2, int n3,
+                                   double2* __restrict__ x0) {
+  const long nc = (long)(n1 + 1) * (n2 + 1) * (n3 + 1);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 3 * nc) return;
+  const int a = (int)(t / nc);
+  const long e = t % nc;
+  const int i = (int)(e % (n1 + 1)), j = (int)((e / (n1 + 1)) % (n2 + 1)), k = (int)(e / ((long)(n1 + 1) * (n2 + 1)));
+  const int p[3] = {i, j, k};
+  const int n[3] = {n1, n2, n3};
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  double2 v = make_double2(0.0, 0.0);
+  if (p[a] < n[a] && p[b] >= 1 && p[b] <= n[b] - 1 && p[c] >= 1 && p[c] <= n[c] - 1) {
+    double dL = 0.0, qr = 0.0, qi = 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      int q[3] = {p[0], p[1], p[2]};
+      q[b] -= s & 1;
+      q[c] -= s >> 1;
+      const long cell = ((long)q[2] * n2 + q[1]) * n1 + q[0];
+      if (s11 && s11[cell] < sig_air) continue;
+      dL += 0.25 * fmin(fmax((double)o[cell * 9 + 3 * a], -20.0), 20.0);
+      qr += 0.25 * (double)o[cell * 9 + 3 * a + 1];
+      qi += 0.25 * (double)o[cell * 9 + 3 * a + 2];
+    }
+    qr += 1.0;
+    const double nrm = sqrt(qr * qr + qi * qi);
+    const double2 u = x1[t];
+    if (nrm > 1e-6) {
+      const double sc = exp(dL) / nrm;
+      v.x = sc * (u.x * qr - u.y * qi);
+      v.y = sc * (u.x * qi + u.y * qr);
+    }
+  }
+  x0[t] = v;
+}
+
 // Training targets [V][9] for refining x1 towards the solved field E, and the
 // loss weights: 1 on dL, wp on the phasor channels, where both |E_a| and
 // |x1_a| exceed eref (elsewhere 0: the field is negligible there)
@@ -480,9 +524,18 @@ hipStream_t S(void* s) { return (hipStream_t)s; }
 template <int MT>
 void conv_fwd_mt(const bf16* x, int Z, int Y, int X, int Cin, const bf16* w, int Kpad, int Cout, const float* bias,
                  int epi, float slope, const bf16* act, bf16* ob, float* of, int cstore, hipStream_t s) {
-  const dim3 g(X / (16 * MT), (Y + 3) / 4, Z), b(256);
+  // the last tile of a row may be partial: loads and stores check x < X
+  const dim3 g((X + 16 * MT - 1) / (16 * MT), (Y + 3) / 4, Z), b(256);
+  hipEvent_t e0, e1;
+  emmi::timing::kernel_events(emmi::timing::CONV, 54.0 * Cin * Cout * (double)Z * Y * X, &e0, &e1);
   switch (Cout / 16) {
-#define CASE(NT) case NT: hipLaunchKernelGGL((k_conv_fwd<MT, NT>), g, b, 0, s, x, Z, Y, X, Cin, w, Kpad, bias, epi, slope, act, ob, of, cstore); break;
+#define CASE(NT)                                                                                                   \
+  case NT:                                                                                                         \
+    if (e0) hipExtLaunchKernelGGL((k_conv_fwd<MT, NT>), g, b, 0, s, e0, e1, 0, x, Z, Y, X, Cin, w, Kpad, bias, epi, \
+                                  slope, act, ob, of, cstore);                                                     \
+    else hipLaunchKernelGGL((k_conv_fwd<MT, NT>), g, b, 0, s, x, Z, Y, X, Cin, w, Kpad, bias, epi, slope, act, ob,  \
+                            of, cstore);                                                                           \
+    break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
   }
@@ -496,7 +549,7 @@ int emmi_x0_kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
 
 int emmi_x0_conv_fwd(const void* x, int Z, int Y, int X, int Cin, const void* w, int Cout, const float* bias, int epi,
                      float slope, const void* act, void* out_bf16, float* out_f32, int cout_store, void* stream) {
-  if (Cin % 16 || Cout % 16 || Cout > 128 || X % 16 || Cin <= 0) return -1;
+  if (Cin % 16 || Cout % 16 || Cout > 128 || Cin <= 0 || X <= 0 || Y <= 0 || Z <= 0) return -1;
   const int Kpad = emmi_x0_kpad(Cin);
   const bf16* xb = (const bf16*)x;
   const bf16* wb = (const bf16*)w;
@@ -511,7 +564,11 @@ int emmi_x0_conv_dw(const void* x, const void* dy, int Z, int Y, int X, int Cin,
   if (Cin % 16 || Cout % 16 || zslab < 1) return -1;
   const int nslab = (Z + zslab - 1) / zslab;
   const dim3 g(27 * (Cin / 16) * (Cout / 16), nslab), b(256);
-  hipLaunchKernelGGL(k_conv_dw, g, b, 0, S(stream), (const bf16*)x, (const bf16*)dy, Z, Y, X, Cin, Cout, zslab, part);
+  hipEvent_t e0, e1;
+  emmi::timing::kernel_events(emmi::timing::CONV_DW, 54.0 * Cin * Cout * (double)Z * Y * X, &e0, &e1);
+  if (e0) hipExtLaunchKernelGGL(k_conv_dw, g, b, 0, S(stream), e0, e1, 0, (const bf16*)x, (const bf16*)dy, Z, Y, X, Cin,
+                                Cout, zslab, part);
+  else hipLaunchKernelGGL(k_conv_dw, g, b, 0, S(stream), (const bf16*)x, (const bf16*)dy, Z, Y, X, Cin, Cout, zslab, part);
   const long n = 27L * Cin * Cout;
   hipLaunchKernelGGL(k_reduce_slabs, dim3(blocks(n, 256)), dim3(256), 0, S(stream), part, nslab, n, dw);
   if (db) hipLaunchKernelGGL(k_bias_grad, dim3(Cout), dim3(256), 0, S(stream), (const bf16*)dy, (long)Z * Y * X, Cout, db);
@@ -583,6 +640,14 @@ int emmi_x0_out_to_edges(const float* o, const void* x1, int n1, int n2, int n3,
   const long n = 3L * (n1 + 1) * (n2 + 1) * (n3 + 1);
   hipLaunchKernelGGL(k_out_to_edges, dim3(blocks(n, 256)), dim3(256), 0, S(stream), o, (const double2*)x1, n1, n2, n3,
                      (double2*)x0);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int emmi_x0_out_to_edges_avg(const float* o, const void* x1, const float* s11, float sig_air, int n1, int n2, int n3,
+                             void* x0, void* stream) {
+  const long n = 3L * (n1 + 1) * (n2 + 1) * (n3 + 1);
+  hipLaunchKernelGGL(k_out_to_edges_avg, dim3(blocks(n, 256)), dim3(256), 0, S(stream), o, (const double2*)x1, s11,
+                     sig_air, n1, n2, n3, (double2*)x0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -64,6 +64,8 @@ def lib() -> C.CDLL:
         L.emmi_solver_history_slot.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int]
         L.emmi_solver_set_ncycle.restype = C.c_int
         L.emmi_solver_set_ncycle.argtypes = [C.c_void_p, C.c_int]
+        L.emmi_solver_set_first_cycle.restype = C.c_int
+        L.emmi_solver_set_first_cycle.argtypes = [C.c_void_p, C.c_int]
         L.emmi_solver_slots.restype = C.c_int
         L.emmi_solver_slots.argtypes = [C.c_void_p]
         from . import carriers as K
@@ -277,6 +279,11 @@ class Solver:
         if lib().emmi_solver_set_ncycle(self.h, int(ncycle)):
             raise NativeError(last_error())
 
+    def set_first_cycle(self, first):
+        """Cycle k of the following solves uses pattern (first + k) % 3."""
+        if lib().emmi_solver_set_first_cycle(self.h, int(first)):
+            raise NativeError(last_error())
+
     def set_omega(self, omega, slot=0):
         if lib().emmi_solver_set_omega_slot(self.h, int(slot), np.float32(omega)):
             raise NativeError(last_error())
@@ -346,14 +353,16 @@ def timing_enable(on=True):
     lib().emmi_timing_enable(1 if on else 0)
 
 
-TIMING_CLASSES = ["residual", "line_solve", "restriction", "prolongation", "other", "line_rhs"]
+TIMING_CLASSES = ["residual", "line_solve", "restriction", "prolongation", "other", "line_rhs", "conv", "conv_dw"]
 
 
 def timing_read():
     """{class: (total ms, launches, compulsory bytes, layout bytes)} since
     timing_enable: compulsory bytes read each input once and write each
     output once at the launch's level shape; layout bytes add what the
-    kernels' own data layout moves (the line kernels' stored records)."""
+    kernels' own data layout moves (the line kernels' stored records).  For
+    the network's convolutions (conv: forward and input gradient; conv_dw:
+    weight gradient) the byte slots hold algorithmic flops."""
     ms = (C.c_double * 8)()
     cnt = (C.c_long * 8)()
     by = (C.c_double * 8)()

@@ -61,6 +61,7 @@ def _lib():
         "emmi_x0_adam": (i, [vp, vp, vp, vp, l, f, f, f, f, i, f, vp]),
         "emmi_x0_pack_bf16": (i, [vp, l, i, i, vp, vp]),
         "emmi_x0_out_to_edges": (i, [vp, vp, i, i, i, vp, vp]),
+        "emmi_x0_out_to_edges_avg": (i, [vp, vp, vp, f, i, i, i, vp, vp]),
         "emmi_x0_edges_to_target": (i, [vp, vp, i, i, i, d, f, vp, vp, vp]),
         "emmi_x0_features": (i, [vp] * 8 + [i, i, i, f, f, f, f, f, vp, d, vp, vp]),
         "emmi_x0_scale": (i, [vp, l, d, vp]),
@@ -169,15 +170,13 @@ class Conv:
 class UNet:
     def __init__(self, shape, seed=0, levels=LEVELS):
         """shape = (Z, Y, X) of the level-0 grid; each must be divisible by
-        2^(len(levels)-1) and X by 16 on every level."""
+        2^(len(levels)-1)."""
         self.shape = tuple(int(v) for v in shape)
         self.levels = tuple(levels)
         nl = len(levels)
         for v in self.shape:
             if v % (2 ** (nl - 1)):
                 raise ValueError("grid not divisible by the pooling depth")
-        if (self.shape[2] >> (nl - 1)) % 16:
-            raise ValueError("the x extent of the coarsest level must be a multiple of 16")
         rng = np.random.default_rng(seed)
         self.enc = []
         cin = CIN
@@ -370,3 +369,49 @@ class UNet:
             raise ValueError("weights are for another architecture")
         for i, cv in enumerate(self.convs):
             cv.set(d[f"w{i}"], d[f"b{i}"])
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
+
+
+def conv_rates(shape=(96, 96, 96), steps=20, seed=0):
+    """Training-step rates of the convolution kernels on random inputs of the
+    given grid: per class (conv: forward and input-gradient launches,
+    conv_dw: weight-gradient launches) the event-timed milliseconds per step,
+    the algorithmic TFLOP/s (2 * 27 * Cin * Cout per voxel) and the fraction
+    of the dense BF16 peak."""
+    net = UNet(shape, seed=seed)
+    rng = np.random.default_rng(seed)
+    nv = int(np.prod(shape))
+    feat = Buf.of(bf16_bits(rng.standard_normal((nv, CIN)).astype(np.float32) * 0.5))
+    tgt = Buf.of(rng.standard_normal((nv, COUT_STORE)).astype(np.float32))
+    mask = Buf.of(np.ones((nv, COUT_STORE), np.float32))
+
+    def step():
+        net.forward(feat)
+        net.loss_and_grad(tgt, mask, float(nv * COUT_STORE))
+        net.backward()
+        net.adam(lr=1e-4)
+
+    step()
+    native.lib().emmi_device_sync()
+    native.timing_enable(True)
+    for _ in range(steps):
+        step()
+    native.lib().emmi_device_sync()
+    kt = native.timing_read()
+    native.timing_enable(False)
+    out = {"shape": list(shape), "steps": steps}
+    for k in ("conv", "conv_dw"):
+        ms, n, flops, _ = kt[k]
+        tf = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        out[k] = {"ms_per_step": ms / steps, "launches_per_step": n / steps, "tflops": tf,
+                  "frac_of_bf16_peak": tf / MFMA_BF16_PEAK_TFLOPS}
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    shp = tuple(int(v) for v in sys.argv[1].split("x")) if len(sys.argv) > 1 else (96, 96, 96)
+    print(json.dumps(conv_rates(shp)), flush=True)

@@ -71,6 +71,7 @@ def solve_pair(net, grid, s11, s33, freqs, tol=1e-6):
 
     st0 = run(None)
     x0 = learned_guess(S, net, grid, ds11, ds33, freqs, run, xo)
+    S.set_first_cycle(1)   # continue the pattern sequence after x1's cycle
     st1 = run([p.ptr for p in x0])
     S.close()
     return [{"freq": float(f), "cycles_zero": a.cycles, "cycles_x0": 1 + b.cycles, "rejected": int(b.x0_rejected),
@@ -109,6 +110,48 @@ def train(net, samples, steps, lr0, log=print, val=None):
     return hist
 
 
+def train_config4(net, seeds, steps, lr0, chunk=16, buffer=384, log=print):
+    """Online training on the config-4 earths of ``seeds``: each chunk of
+    models is solved (x1 and the target field, x0data.solve_models), its
+    samples join a replay buffer of the last ``buffer`` samples (about 120 MB
+    each at 96^3), and the chunk's share of the ``steps`` Adam steps follows
+    on samples drawn from the buffer.  Returns the loss history and the
+    solver's cycle counts to the targets."""
+    from collections import deque
+    rng = np.random.default_rng(0)
+    buf = deque(maxlen=buffer)
+    seeds = list(seeds)
+    nchunks = max(1, (len(seeds) + chunk - 1) // chunk)
+    hist, cycles = [], []
+    it = 0
+    t0 = time.perf_counter()
+    for k, (ss, grids) in enumerate(x0data.config4_chunks(seeds, chunk)):
+        solved = x0data.solve_models(grids)
+        for ms in solved:
+            smp = x0data.sample_of(ms)
+            cycles.append(ms.stats.cycles)
+            if smp is None:
+                log(f"model {ms.grid.survey.extra['seed']}: solve status {ms.stats.status}, skipped")
+            else:
+                buf.append(smp)
+        for g in grids:
+            g.close()
+        del solved, grids
+        nsteps = round((k + 1) * steps / nchunks) - round(k * steps / nchunks)
+        for _ in range(nsteps):
+            sm = buf[int(rng.integers(len(buf)))]
+            lr = lr0 * 0.5 * (1 + math.cos(math.pi * it / steps)) + 0.02 * lr0
+            net.forward(sm.feat)
+            hist.append(net.loss_and_grad(sm.tgt, sm.mask, sm.count))
+            net.backward()
+            net.adam(lr=lr)
+            it += 1
+        log(f"chunk {k + 1}/{nchunks} seeds {ss[0]}-{ss[-1]}: buffer {len(buf)}, step {it}, "
+            f"loss {np.mean(hist[-max(nsteps, 1):]) if hist else float('nan'):.4f}, "
+            f"cycles to 1e-7 {np.mean(cycles[-len(ss):]):.1f}, {time.perf_counter() - t0:.0f}s")
+    return hist, cycles
+
+
 def main(argv=None):
     p = argparse.ArgumentParser()
     p.add_argument("--models", type=int, default=60)
@@ -119,10 +162,39 @@ def main(argv=None):
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--out", default="x0net_weights.npz")
     p.add_argument("--init", default=None, help="start from these weights")
+    p.add_argument("--config4", action="store_true",
+                   help="train on the config-4 earths (96^3, one grid per model, seeds 0..models-1) and "
+                        "evaluate on the test seeds (x0eval)")
+    p.add_argument("--chunk", type=int, default=16, help="config-4 models per batched solve")
+    p.add_argument("--buffer", type=int, default=384, help="config-4 replay buffer (samples)")
+    p.add_argument("--eval-out", default=None, help="config-4 evaluation report (JSON)")
     a = p.parse_args(argv)
 
     def log(m):
         print(m, flush=True)
+
+    if a.config4:
+        from . import configs, x0eval
+        seeds = list(configs.CONFIG4_TRAIN)[:a.models]
+        net = x0net.UNet((96, 96, 96), seed=a.seed)
+        if a.init:
+            net.load(np.load(a.init))
+        t0 = time.perf_counter()
+        hist, cycles = train_config4(net, seeds, a.steps, a.lr, chunk=a.chunk, buffer=a.buffer, log=log)
+        state = net.state()
+        state["train_loss"] = np.array(hist, np.float32)
+        np.savez_compressed(a.out, **state)
+        log(f"saved {a.out} after {time.perf_counter() - t0:.0f}s")
+        test = list(configs.CONFIG4_TEST)[:a.eval_models]
+        rep = x0eval.evaluate(net, test, chunk=a.chunk, log=log)
+        rep["training"] = {"models": len(seeds), "steps": a.steps, "lr": a.lr, "buffer": a.buffer,
+                           "final_loss": float(np.mean(hist[-200:])) if hist else None,
+                           "target_cycles_mean": float(np.mean(cycles)) if cycles else None}
+        if a.eval_out:
+            with open(a.eval_out, "w") as f:
+                json.dump(rep, f, indent=1)
+        log(json.dumps(rep["summary"]))
+        return 0
 
     grid = x0data.Grid()
     t0 = time.perf_counter()

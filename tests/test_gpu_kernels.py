@@ -197,11 +197,12 @@ def test_curl_fp64(tiny):
     assert rel(S.check_op(4, 0, 0, 0, 0, u), L0.curl_h(omu, u)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-11), (1, 1e-8)])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-11), (1, 1e-11)])
 def test_solve_matches_oracle(tiny, precision, tol, line_kernel):
-    """Converged fields agree: both solvers run to a tight relative residual
-    (1e-11 in FP64; 1e-8 with FP32 cycles, whose refinement stalls near the
-    FP32 resolution of the near-singular air blocks)."""
+    """Converged fields agree: both solvers run to relative residual 1e-11
+    (FP64 cycles, and FP32-storage cycles inside FP64 refinement), H
+    everywhere and E on the edges at and below the sea surface (in the air
+    the gradient part of E is barely constrained and converges last)."""
     s, g, n, L0 = tiny
     rw = float(min(s.rho11.min(), s.rho22.min(), s.rho33.min()))
     mg = O.mg_struct(g, rw)
@@ -219,6 +220,15 @@ def test_solve_matches_oracle(tiny, precision, tol, line_kernel):
         print("  E rel", rel(E, E_ref), "H rel", rel(H, H_ref))
         assert st.status == 0, (st.status, st.cycles)
         assert rel(H, H_ref) < 100 * tol
+        x3 = np.asarray(g["x3"], np.float64)
+        zc = 0.5 * (x3[:-1] + x3[1:])
+        m = np.zeros((3, n[2] + 1, n[1] + 1, n[0] + 1), bool)
+        m[0] = m[1] = (x3 >= 0)[:, None, None]
+        m[2, :-1] = (zc >= 0)[:, None, None]
+        m = m.reshape(-1)
+        e_sea = float(np.abs(E - E_ref)[m].max() / np.abs(E_ref)[m].max())
+        print("  E rel below the sea surface", e_sea)
+        assert e_sea < 1e4 * tol
 
 
 @pytest.mark.parametrize("it", [0, 1])

@@ -242,3 +242,29 @@ def test_good_initial_guess_is_kept(tiny, precision):
     assert st1.x0_rejected == 0 and st1.x0_rnorm < 1e-6 * st1.rnorm0
     assert st1.status == 0 and st1.cycles <= 1 < st0.cycles
     assert relmax(E1, E_ref, below_air(g, n)) < 1e-7
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_continuation_from_first_cycle_iterate(tiny, precision):
+    """A solve from the iterate after one cycle, started at cycle index 1
+    (Solver.set_first_cycle: the pattern sequence 1, 2, 0, ... continues),
+    reproduces the zero-guess solve: one cycle fewer and the same fields and
+    residual history from cycle 1 on (to rounding: the continued solve
+    recomputes the residual norm of its guess)."""
+    s, g, n, L0, b, E_ref, _ = tiny
+    S = solver(s, g, precision=precision, tol=1e-8, ncycle=80)
+    E0, H0, st0 = S.solve(b)
+    h0 = S.history()
+    S.set_ncycle(1)
+    x1, _, _ = S.solve(b)
+    S.set_ncycle(80)
+    S.set_first_cycle(1)
+    E1, H1, st1 = S.solve(b, x0=x1)
+    h1 = S.history()
+    S.set_first_cycle(0)
+    print(f"precision {precision}: zero guess {st0.cycles} cycles, continued 1 + {st1.cycles}; "
+          f"E max rel diff {relmax(E1, E0):.2e}", flush=True)
+    assert st0.status == 0 and st1.status == 0 and st1.x0_rejected == 0
+    assert st1.cycles + 1 == st0.cycles
+    np.testing.assert_allclose(h1, h0[1:len(h1) + 1], rtol=1e-10)
+    assert relmax(E1, E0) < 1e-10 and relmax(H1, H0) < 1e-10

@@ -44,7 +44,9 @@ def L():
     return x0net._lib()
 
 
-SHAPES = [((4, 4, 16), 16, 16), ((4, 8, 32), 32, 48), ((2, 4, 64), 16, 112), ((3, 5, 16), 48, 16)]
+# the last two have rows whose length is not a multiple of the 16-voxel tile
+SHAPES = [((4, 4, 16), 16, 16), ((4, 8, 32), 32, 48), ((2, 4, 64), 16, 112), ((3, 5, 16), 48, 16),
+          ((3, 4, 12), 16, 32), ((2, 3, 24), 32, 16)]
 
 
 @pytest.mark.parametrize("shape,cin,cout", SHAPES)
@@ -157,9 +159,11 @@ def test_mse_and_adam(L):
     assert rel(tp.get(np.float32), ref) < 1e-6
 
 
-def test_unet_gradients_match_oracle():
+@pytest.mark.parametrize("shape", [(8, 8, 64), (8, 8, 24)])
+def test_unet_gradients_match_oracle(shape):
     """Forward output and every parameter gradient of a 3-level U-Net
-    (16/32/48 channels) on an 8x8x64 grid against the float64 oracle run on
+    (16/32/48 channels) on an 8x8x64 grid, and on 8x8x24 whose rows end in
+    partial tiles, against the float64 oracle run on
     the same BF16 weights and inputs.  With the oracle rounding to BF16 where
     the kernels store BF16, the remaining differences are FP32 accumulation
     order and the rounding flips it causes, which accumulate along the
@@ -167,7 +171,6 @@ def test_unet_gradients_match_oracle():
     weight gradient, last in the chain), against O(1) for an indexing error.
     Without that emulation the plain float64 network differs by up to 6%."""
     from libemmi_mgfd_amd import x0net
-    shape = (8, 8, 64)
     levels = (16, 32, 48)
     net = x0net.UNet(shape, seed=5, levels=levels)
     rng = np.random.default_rng(6)
@@ -230,3 +233,37 @@ def test_unet_training_reduces_loss():
         net.adam(lr=3e-3)
     print("losses", losses[0], losses[-1])
     assert losses[-1] < 0.8 * losses[0]
+
+
+def test_config4_learned_guess_gain_reproduces_report():
+    """What the trained network gains, on the config-4 test earths: the first
+    16 test seeds (2000-2015, each a random earth on its own 96^3 multigrid)
+    re-evaluated with the committed weights (libemmi_mgfd_amd/
+    x0net_weights_96.npz) give the committed report's cycle counts exactly
+    (zero guess; one cycle then a solve from x1; one cycle, the network, then
+    a solve from its guess), and the solve continued from x1 costs exactly
+    the zero-guess solve's cycles.  The report's summary over all 200 test
+    seeds is printed: it is what the network saves (a negative number means
+    it costs cycles)."""
+    import json
+    import os
+    from libemmi_mgfd_amd import configs, x0eval, x0net
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rep = json.load(open(os.path.join(here, "libemmi_mgfd_amd", "x0net_eval_96.json")))
+    d = np.load(os.path.join(here, "libemmi_mgfd_amd", "x0net_weights_96.npz"))
+    net = x0net.UNet((96, 96, 96), levels=tuple(int(v) for v in d["levels"]))
+    net.load(d)
+    seeds = list(configs.CONFIG4_TEST)[:16]
+    print("config4 test seeds 2000-2015 on 96^3, reconstructions", rep["recons"], flush=True)
+    now = x0eval.evaluate(net, seeds, chunk=16, recons=tuple(rep["recons"]), first_cycle=rep["first_cycle"],
+                          log=lambda m: print(m, flush=True))
+    keys = ["cycles_zero", "cycles_x1", "status"] + [f"{k}_{rc}" for rc in rep["recons"]
+                                                     for k in ("cycles_learned", "rejected")]
+    for a, b in zip(now["rows"], rep["rows"][:16]):
+        assert a["seed"] == b["seed"]
+        for k in keys:
+            assert a[k] == b[k], (a["seed"], k, a[k], b[k])
+        assert a["cycles_x1"] == a["cycles_zero"]
+    s = rep["summary"]
+    print(json.dumps({k: s[k] for k in s if k != "wall_s"}), flush=True)
+    assert s["models"] == 200 and s["all_converged"]
