@@ -75,19 +75,25 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, dou
   if (t < npts) {
     int i, j, k;
     ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
-    cxd v;
-    v = ld<T>(f, t);
-    if (interior<0>(L.n, i, j, k)) v -= apply_row<T, 0>(L, u, i, j, k, t);
-    st<T>(r, t, v);
-    acc += (double)v.re * v.re + (double)v.im * v.im;
-    v = ld<T>(f, L.nc + t);
-    if (interior<1>(L.n, i, j, k)) v -= apply_row<T, 1>(L, u, j, k, i, t);
-    st<T>(r, L.nc + t, v);
-    acc += (double)v.re * v.re + (double)v.im * v.im;
-    v = ld<T>(f, 2 * L.nc + t);
-    if (interior<2>(L.n, i, j, k)) v -= apply_row<T, 2>(L, u, k, i, j, t);
-    st<T>(r, 2 * L.nc + t, v);
-    acc += (double)v.re * v.re + (double)v.im * v.im;
+    const bool in0 = interior<0>(L.n, i, j, k), in1 = interior<1>(L.n, i, j, k), in2 = interior<2>(L.n, i, j, k);
+    cxd v0 = ld<T>(f, t), v1 = ld<T>(f, L.nc + t), v2 = ld<T>(f, 2 * L.nc + t);
+    if (in0 && in1 && in2) {
+      // all three rows in one block with the stores after them: the rows
+      // share 15 of their 39 stencil values, and with no store in between the
+      // compiler loads each of the 24 distinct values once
+      v0 -= apply_row<T, 0>(L, u, i, j, k, t);
+      v1 -= apply_row<T, 1>(L, u, j, k, i, t);
+      v2 -= apply_row<T, 2>(L, u, k, i, j, t);
+    } else {
+      if (in0) v0 -= apply_row<T, 0>(L, u, i, j, k, t);
+      if (in1) v1 -= apply_row<T, 1>(L, u, j, k, i, t);
+      if (in2) v2 -= apply_row<T, 2>(L, u, k, i, j, t);
+    }
+    st<T>(r, t, v0);
+    st<T>(r, L.nc + t, v1);
+    st<T>(r, 2 * L.nc + t, v2);
+    acc = ((double)v0.re * v0.re + (double)v0.im * v0.im) + ((double)v1.re * v1.re + (double)v1.im * v1.im) +
+          ((double)v2.re * v2.re + (double)v2.im * v2.im);
   }
   if (partial) block_sum_store(acc, partial, blk);
 }
@@ -196,8 +202,10 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // ---------------------------------------------------------------------------
 // restriction, gather form: one thread per coarse index, three components;
 // uc, when given, is the coarse correction, zeroed at the same indices (the
-// coarse solve starts from zero)
-template <typename T>
+// coarse solve starts from zero).  SCB has bit a set when axis a is
+// coarsened (sc[a] == 2), so every gather loop has a fixed trip count and
+// unrolls.
+template <typename T, int SCB>
 __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Transfer<T> X, SlotVecs<T> V) {
   slot_transfer(X, V, blockIdx.y);
   const typename cvec<T>::type* __restrict__ rf = V.in0[blockIdx.y];
@@ -207,38 +215,57 @@ __global__ void __launch_bounds__(256) k_restrict(Level<T> F, Level<T> Cl, Trans
   if (t >= Cl.nc) return;
   int P[3];
   ijk_of(t, Cl.es[1], Cl.n[1] + 1, P[0], P[1], P[2]);
+  cxd acc3[3];
 #pragma unroll
   for (int A = 0; A < 3; A++) {
     const int Bx = (A + 1) % 3, Cx = (A + 2) % 3;
+    const bool ca = (SCB >> A) & 1, cb = (SCB >> Bx) & 1, cc = (SCB >> Cx) & 1;
     cxd acc(0, 0);
     const bool inside = P[A] < Cl.n[A] && P[Bx] >= 1 && P[Bx] <= Cl.n[Bx] - 1 && P[Cx] >= 1 && P[Cx] <= Cl.n[Cx] - 1;
     if (inside) {
-      // fine indices and weights along each axis
-      int qa[2], na = 1;
-      if (X.sc[A] == 2) { qa[0] = 2 * P[A]; qa[1] = 2 * P[A] + 1; na = 2; } else { qa[0] = P[A]; }
-      int qb[3], nbq = 0; double wb[3];
-      int qc[3], ncq = 0; double wc[3];
-      auto collect = [&](int ax, int Pv, int* q, double* w, int& nq) {
-        if (X.sc[ax] == 2) {
-          q[nq] = 2 * Pv - 1; w[nq] = X.wr[ax][2 * Pv - 1]; nq++;
-          q[nq] = 2 * Pv;     w[nq] = 1.0 - (double)X.wr[ax][2 * Pv]; nq++;
-          q[nq] = 2 * Pv + 1; w[nq] = 1.0 - (double)X.wr[ax][2 * Pv + 1]; nq++;
-        } else {
-          q[nq] = Pv; w[nq] = 1.0 - (double)X.wr[ax][Pv]; nq++;
-        }
-      };
-      collect(Bx, P[Bx], qb, wb, nbq);
-      collect(Cx, P[Cx], qc, wc, ncq);
-      for (int ia = 0; ia < na; ia++)
-        for (int ib = 0; ib < nbq; ib++)
-          for (int ic = 0; ic < ncq; ic++) {
+      // fine indices and weights along each axis: 2 fine edges along a
+      // coarsened edge axis, 3 fine nodes (weights w, 1 - w, 1 - w) across a
+      // coarsened axis, 1 otherwise
+      const int na = ca ? 2 : 1, nbq = cb ? 3 : 1, ncq = cc ? 3 : 1;
+      int qb[3], qc[3];
+      double wb[3], wc[3];
+      if (cb) {
+        qb[0] = 2 * P[Bx] - 1; wb[0] = X.wr[Bx][2 * P[Bx] - 1];
+        qb[1] = 2 * P[Bx];     wb[1] = 1.0 - (double)X.wr[Bx][2 * P[Bx]];
+        qb[2] = 2 * P[Bx] + 1; wb[2] = 1.0 - (double)X.wr[Bx][2 * P[Bx] + 1];
+      } else {
+        qb[0] = P[Bx]; wb[0] = 1.0 - (double)X.wr[Bx][P[Bx]];
+      }
+      if (cc) {
+        qc[0] = 2 * P[Cx] - 1; wc[0] = X.wr[Cx][2 * P[Cx] - 1];
+        qc[1] = 2 * P[Cx];     wc[1] = 1.0 - (double)X.wr[Cx][2 * P[Cx]];
+        qc[2] = 2 * P[Cx] + 1; wc[2] = 1.0 - (double)X.wr[Cx][2 * P[Cx] + 1];
+      } else {
+        qc[0] = P[Cx]; wc[0] = 1.0 - (double)X.wr[Cx][P[Cx]];
+      }
+      const int qa0 = ca ? 2 * P[A] : P[A];
+#pragma unroll
+      for (int ia = 0; ia < 2; ia++) {
+        if (ia >= na) break;
+#pragma unroll
+        for (int ib = 0; ib < 3; ib++) {
+          if (ib >= nbq) break;
+#pragma unroll
+          for (int ic = 0; ic < 3; ic++) {
+            if (ic >= ncq) break;
             int q[3];
-            q[A] = qa[ia]; q[Bx] = qb[ib]; q[Cx] = qc[ic];
+            q[A] = qa0 + ia; q[Bx] = qb[ib]; q[Cx] = qc[ic];
             const long o = A * F.nc + q[0] + q[1] * F.es[1] + q[2] * F.es[2];
             acc += ld<T>(rf, o) * (wb[ib] * wc[ic]);
           }
+        }
+      }
     }
-    st<T>(fc, A * Cl.nc + t, acc);
+    acc3[A] = acc;
+  }
+#pragma unroll
+  for (int A = 0; A < 3; A++) {
+    st<T>(fc, A * Cl.nc + t, acc3[A]);
     if (uc) st<T>(uc, A * Cl.nc + t, cxd(0, 0));
   }
 }
@@ -503,8 +530,25 @@ void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<
     B.out0[k] = (V*)fc[k];
     B.out1[k] = uc ? (V*)uc[k] : nullptr;
   }
-  timing::Scope ts(timing::RESTRICT, s, n * 3.0 * (F.nc + C.nc) * sizeof(typename cvec<T>::type));
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(C.nc, 256), n), dim3(256), 0, s, F, C, X, B);
+  // compulsory bytes: the fine residual read, the coarse right-hand side
+  // written and, when the kernel also zeroes the coarse correction, that
+  // vector written
+  timing::Scope ts(timing::RESTRICT, s,
+                   n * 3.0 * (F.nc + (uc ? 2.0 : 1.0) * C.nc) * sizeof(typename cvec<T>::type));
+  const dim3 g(nblk(C.nc, 256), n), b(256);
+  int scb = 0;
+  for (int a = 0; a < 3; a++) {
+    if (X.sc[a] != 1 && X.sc[a] != 2) throw std::runtime_error("restriction: coarsening factor other than 1 or 2");
+    if (X.sc[a] == 2) scb |= 1 << a;
+  }
+  for (int k = 1; k < n; k++)
+    for (int a = 0; a < 3; a++)
+      if (Xs[k].sc[a] != X.sc[a]) throw std::runtime_error("restriction: slots with different coarsening");
+  switch (scb) {
+#define CASE(M) case M: hipLaunchKernelGGL((k_restrict<T, M>), g, b, 0, s, F, C, X, B); break;
+    CASE(0) CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
+#undef CASE
+  }
 }
 
 template <typename T>

@@ -505,7 +505,9 @@ void Solver::kernel_rates_t(int reps, double* ms, double* bytes) {
     bytes[k] = by;
   };
   timed(0, ne * (3.0 * es) + 3.0 * nc * cs, [&] { launch_residual<T>(V, S.u[0], S.f[0], S.r[0], S.partial, s_); });
-  timed(1, (ne + nec) * es, [&] { launch_restrict<T>(V, VC, X, S.r[0], S.f[1], S.u[1], s_); });
+  // restriction: fine residual read, coarse right-hand side and the zeroed
+  // coarse correction written
+  timed(1, (ne + 2.0 * nec) * es, [&] { launch_restrict<T>(V, VC, X, S.r[0], S.f[1], S.u[1], s_); });
   timed(2, (2.0 * ne + nec) * es, [&] { launch_prolong<T>(V, VC, X, S.u[1], S.u[0], s_); });
   for (int a = 0; a < 3; a++)
     timed(3 + a, sweep, [&] { line_sweep_on<T>(0, 0, a, 0, S.u[0], S.f[0]); });
