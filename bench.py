@@ -16,6 +16,11 @@ region (``setup_s``), and every variant keeps one right-hand-side slot per
 frequency, so the line factors and captured cycle graphs of the warmup steps
 serve the timed steps of every variant alike.  Data are synthetic.
 
+Before its timed steps each variant runs the autotuner on its own launch
+shapes (``Solver.autotune``, part of ``setup_s``): every line-sweep shape
+gets the substitution kernel measured fastest (``line_autotune`` lists the
+table; ``--no-autotune`` keeps the built-in rules).
+
 Variants: ``plain`` (complex FP64 cycles, one solver per frequency, solved
 one after another), ``mixed`` (FP32-storage cycles inside FP64 refinement,
 one solver per frequency), ``mixed_batched`` (the three frequencies as one
@@ -75,6 +80,8 @@ def parse():
     p.add_argument("--no-gradient", action="store_true", help="skip the survey-scale gradient leg")
     p.add_argument("--no-kernels", action="store_true", help="skip the per-kernel rates")
     p.add_argument("--cpu-cycles", type=int, default=2, help="F-cycles of the bounded CPU sample")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the built-in rules for the line-sweep kernels instead of timing them per launch shape")
     p.add_argument("--no-graphs", action="store_true",
                    help="launch every kernel eagerly instead of replaying captured cycle graphs")
     p.add_argument("--verbose", action="store_true")
@@ -91,7 +98,7 @@ class ShotRun:
     solve; otherwise one single-slot solver per frequency, solved one after
     another."""
 
-    def __init__(self, survey, precision, batched=False, shot=0, graphs=True, learned=False):
+    def __init__(self, survey, precision, batched=False, shot=0, graphs=True, learned=False, autotune=True):
         from libemmi_mgfd_amd import carriers, native
         self.native = native
         self.batched = batched
@@ -115,6 +122,11 @@ class ShotRun:
             self.solvers = [native.Solver(self.n, x1, x2, x3, *sig, list(self.sc.omegas[:nf]), **kw)]
         else:
             self.solvers = [native.Solver(self.n, x1, x2, x3, *sig, self.sc.omegas[f], **kw) for f in range(nf)]
+        if autotune:
+            # time the line-sweep kernels on this variant's launch shapes
+            # (the table is process-wide: shapes already tuned are skipped)
+            for S in self.solvers:
+                S.autotune()
         nb = 16 * self.sc.nvec
         self.b = [native.DeviceBuffer(nb) for _ in range(nf)]
         self.x = [native.DeviceBuffer(nb) for _ in range(nf)]
@@ -186,13 +198,13 @@ VARIANTS = {"plain": (0, False, False), "mixed": (1, False, False), "batched": (
             "mixed_batched": (1, True, False), "mixed_batched_learned": (1, True, True)}
 
 
-def time_variant(survey, variant, steps, warmup, graphs=True):
+def time_variant(survey, variant, steps, warmup, graphs=True, autotune=True):
     """Warmup, then `steps` timed steps (cycle bodies replayed as HIP graphs);
     then one more step runs eagerly with per-launch HIP events for the
     kernel-level figures (the same kernels with the same arguments)."""
     from libemmi_mgfd_amd import native
     precision, batched, learned = VARIANTS[variant]
-    run = ShotRun(survey, precision, batched, graphs=graphs, learned=learned)
+    run = ShotRun(survey, precision, batched, graphs=graphs, learned=learned, autotune=autotune)
     for _ in range(warmup):
         run.solve_all()
     native.lib().emmi_device_sync()
@@ -231,12 +243,13 @@ def kernel_rates(survey, precision):
     return out
 
 
-def survey_leg():
+def survey_leg(autotune=True):
     """Batched survey: 25 shots x 4 frequencies, each frequency on its own
     multigrid size, FP32 cycles in FP64 refinement, rel-res 1e-6."""
     from libemmi_mgfd_amd import configs, survey
     s = configs.config3()
-    run = survey.SurveyRun(s, grids=configs.CONFIG3_GRIDS, precision=1, tol=1e-6, ncycle=60, max_slots=25)
+    run = survey.SurveyRun(s, grids=configs.CONFIG3_GRIDS, precision=1, tol=1e-6, ncycle=60, max_slots=25,
+                           autotune=autotune)
     out = run.run(repeats=1)
     st = sorted({r["status"] for r in run.results.values()})
     cy = [r["cycles"] for r in run.results.values()]
@@ -352,7 +365,7 @@ def main():
             if not os.path.exists(x0guess.WEIGHTS):
                 log(f"bench.py: {x0guess.WEIGHTS} missing, variant {v} skipped")
                 continue
-        res[v] = time_variant(survey, v, a.steps, a.warmup, graphs=not a.no_graphs)
+        res[v] = time_variant(survey, v, a.steps, a.warmup, graphs=not a.no_graphs, autotune=not a.no_autotune)
         log(v, {k: res[v][k] for k in ("solves_per_s", "ms_per_step", "cycles", "status", "rel_res")})
     # receiver data of the signal channels (inline Ex, Hy; inline Ey and Hx
     # vanish by symmetry) against the reference's converged FP64 solve
@@ -413,6 +426,17 @@ def main():
                                      for c, t in v["kernels"].items() if t[1]}}
                      for k, v in res.items()},
     }
+    if not a.no_autotune:
+        tab = native.line_tuning()
+        out["line_autotune"] = {
+            "shapes": len(tab),
+            "changed": sum(1 for t in tab if t["us"] < t["us_rule"] * 0.999),
+            "us_saved_per_launch_sum": sum(t["us_rule"] - t["us"] for t in tab),
+            "choices": {c: sum(1 for t in tab if t["choice"] == c) for c in native.LINE_CHOICES},
+            "table": [[t["esize"], t["axis"], t["n"], t["lines"], t["jobs"], t["choice"], round(t["us"], 1),
+                       round(t["us_rule"], 1)] for t in tab],
+            "columns": ["value bytes", "axis", "line cells", "lines per colour", "rhs per launch", "choice",
+                        "us", "us of the rule"]}
     if "plain" in res:
         for k in res:
             if k != "plain":
@@ -430,7 +454,7 @@ def main():
         from libemmi_mgfd_amd import x0net
         out["x0net_conv"] = x0net.conv_rates((survey.n3, survey.n2, survey.n1))
     if not a.no_survey:
-        out["survey"] = survey_leg()
+        out["survey"] = survey_leg(autotune=not a.no_autotune)
         log("survey", out["survey"]["solves_per_s"])
     if not a.no_gradient:
         out["gradient"] = gradient_leg()

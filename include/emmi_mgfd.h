@@ -222,6 +222,27 @@ int emmi_solver_set_ncycle(emmi_solver_t *s, int ncycle);
  * continues from the iterate of `first` earlier cycles keeps the reference's
  * pattern sequence.  Default 0. */
 int emmi_solver_set_first_cycle(emmi_solver_t *s, int first);
+/* Autotuner (no counterpart in the reference).  For every line-sweep launch
+ * shape a solve of all the solver's slots issues -- precision, axis, line
+ * length, lines per colour, right-hand sides per launch -- that the
+ * process-wide table does not hold yet, times each substitution kernel
+ * (one lane per line half; four lanes per line half with the right-hand
+ * sides computed in the same launch; four lanes with a separate
+ * right-hand-side launch) reps times on the slots' own vectors and records
+ * the fastest; later launches of that shape use it (unless
+ * emmi_set_line_kernel forces a kernel).  Overwrites the solver's vectors
+ * (the next solve sets them) and drops its captured cycle graphs.  Returns
+ * the number of shapes tuned or EMMI_ERROR.  The kernels differ in rounding
+ * order, so solves of one right-hand side in different pack sizes may differ
+ * in the last bits once the table holds entries. */
+int emmi_solver_autotune(emmi_solver_t *s, int reps);
+/* the table: rows[6k..6k+5] = value size (4 or 8), axis, line length (cells),
+ * lines per colour, right-hand sides per launch, choice (0 one lane, 1 quad
+ * fused, 2 quad separate); us[2k] = microseconds per launch of the choice,
+ * us[2k+1] of the built-in rule.  Returns the number of entries (may exceed
+ * maxn) */
+int emmi_line_tuning(int *rows, double *us, int maxn);
+void emmi_line_tuning_clear(void);
 int emmi_solver_levels(emmi_solver_t *s, int *npatterns);
 void emmi_solver_dims(emmi_solver_t *s, int pattern, int lev, int *n123);
 

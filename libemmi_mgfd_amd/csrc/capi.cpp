@@ -201,6 +201,20 @@ int emmi_solver_set_ncycle(emmi_solver_t* h, int ncycle) {
   return 0;
 }
 
+int emmi_solver_autotune(emmi_solver_t* h, int reps) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    return h->s->autotune(reps);
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
+int emmi_line_tuning(int* rows, double* us, int maxn) { return emmi::line_choices(rows, us, maxn); }
+
+void emmi_line_tuning_clear(void) { emmi::clear_line_choices(); }
+
 int emmi_solver_set_first_cycle(emmi_solver_t* h, int first) {
   if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
   if (first < 0) { set_err("negative cycle index"); return EMMI_ERROR; }
@@ -382,6 +396,7 @@ struct LegacyState {
   emf_t* emf = nullptr;
   float omega = 0;
   emmi_params_t p;
+  bool autotune = false;
   bool init = false;
 };
 LegacyState g_legacy;
@@ -406,6 +421,7 @@ extern "C" void gmg_init(emf_t* emf, int ifreq) {
   g_legacy.p.coarse_sweeps = par_int("coarse_sweeps", 2);
   g_legacy.p.verb = emf->verb;
   g_legacy.p.graphs = par_int("graphs", 1);
+  g_legacy.autotune = par_int("autotune", 0) != 0;
   g_legacy.init = true;
 }
 
@@ -417,6 +433,7 @@ extern "C" void gmg_apply(int n, emmi_c128* b, emmi_c128* x) {
   emmi_solver_t* s = emmi_solver_create(e->n1, e->n2, e->n3, e->x1, e->x2, e->x3, &e->sigma11[0][0][0],
                                         &e->sigma22[0][0][0], &e->sigma33[0][0][0], 0, g_legacy.omega, &g_legacy.p);
   if (!s) fatal(emmi_last_error());
+  if (g_legacy.autotune && emmi_solver_autotune(s, 5) < 0) fatal(emmi_last_error());
   void *db = emmi_device_alloc(sizeof(emmi_c128) * nv), *dx = emmi_device_alloc(sizeof(emmi_c128) * nv);
   if (!db || !dx) fatal("device allocation failed");
   emmi_memcpy_h2d(db, b, sizeof(emmi_c128) * nv);

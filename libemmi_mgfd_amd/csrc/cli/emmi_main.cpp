@@ -27,7 +27,8 @@
 // cycles, or FP32 cycles inside FP64 refinement, default 1; either stops at
 // the same FP64 residual test), graphs=0|1, coarse_sweeps=N, nproc=N,
 // batch_shots=N (modes 1 and 2: the shots solved together in one survey
-// solve; default 24 / nfreq).
+// solve; default 24 / nfreq), autotune=0|1 (time the line-sweep kernels on
+// each new launch shape and use the fastest, default 0).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -160,6 +161,8 @@ struct ShotGroup {
     solver = emmi_solver_create_survey(e.n1, e.n2, e.n3, ns, x1.data(), x2.data(), x3.data(), s11.data(),
                                        s22.data(), s33.data(), 0, med.data(), om.data(), ns * nf, &p);
     if (!solver) emmi_cli_err("solver: %s", emmi_last_error());
+    if (par_int("autotune", 0) && emmi_solver_autotune(solver, 5) < 0)
+      emmi_cli_err("autotune: %s", emmi_last_error());
     for (int k = 0; k < ns * nf; ++k) {
       b.push_back(emmi_device_alloc(sizeof(emmi_c128) * nv));
       x.push_back(emmi_device_alloc(sizeof(emmi_c128) * nv));

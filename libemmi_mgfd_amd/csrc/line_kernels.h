@@ -64,13 +64,48 @@ inline void line_job_colour(LineJob<T>& J, int a, int colour) {
   line_colour_counts(J.L.n, a, colour, &J.nlo, &J.nhi);
 }
 
-// substitution kernel of later launches: 0 by line count, 1 one lane per
-// line half, 2 four lanes per line half; returns the previous mode or -1
+// substitution kernel of later launches: 0 by line count (or by the
+// autotuner's table where it has the launch's shape), 1 one lane per line
+// half, 2 four lanes per line half; returns the previous mode or -1
 int set_line_kernel(int mode);
+
+// Autotuning: a launch shape (value size of the cycle precision, axis, line
+// length, lines per job, jobs per pack) and the substitution choice measured
+// fastest for it.  LINE_RULE: the built-in rule by line count.
+// LINE_RULE_ONLY (forced launches only): the rule, ignoring the table.
+enum LineChoice {
+  LINE_RULE_ONLY = -2, LINE_RULE = -1, LINE_ONE_LANE = 0, LINE_QUAD_FUSED = 1, LINE_QUAD_SEPARATE = 2, LINE_NCHOICE = 3
+};
+struct LineShape {
+  int esize, axis, n;
+  long nl;
+  int njobs;
+  bool operator<(const LineShape& o) const {
+    if (esize != o.esize) return esize < o.esize;
+    if (axis != o.axis) return axis < o.axis;
+    if (n != o.n) return n < o.n;
+    if (nl != o.nl) return nl < o.nl;
+    return njobs < o.njobs;
+  }
+};
+template <typename T>
+LineShape line_shape(const LinePack<T>& P, int axis);
+// single-line launches (cyclic reduction) have no choice to tune
+template <typename T>
+bool line_pcr_shape(const LinePack<T>& P, int axis);
+void set_line_choice(const LineShape& k, int choice, double us_choice, double us_rule);
+bool has_line_choice(const LineShape& k);
+void clear_line_choices();
+// rows of (esize, axis, n, nl, njobs, choice) and (us of the choice, us of the
+// rule); returns the number of entries
+int line_choices(int* out, double* us, int maxn);
 
 template <typename T>
 void launch_line_factor(const LinePack<T>& P, int axis, hipStream_t s);
 template <typename T>
 void launch_line_solve(const LinePack<T>& P, int axis, hipStream_t s);
+// the same with the substitution choice forced (LINE_RULE: the rule alone)
+template <typename T>
+void launch_line_solve_as(const LinePack<T>& P, int axis, int choice, hipStream_t s);
 
 }  // namespace emmi

@@ -32,6 +32,10 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <map>
+#include <mutex>
 #include "stencil.h"
 #include "timing.h"
 
@@ -1663,6 +1667,63 @@ int set_line_kernel(int mode) {
   return prev;
 }
 
+// The autotuner's table: the substitution choice measured fastest for a
+// launch shape.  Process-wide (every solver on the device shares it) and
+// consulted only in the automatic mode (set_line_kernel(0)).
+struct TunedLine {
+  int choice;
+  double us_choice, us_rule;
+};
+static std::map<LineShape, TunedLine> g_tuned;
+static std::mutex g_tuned_mu;
+
+template <typename T>
+LineShape line_shape(const LinePack<T>& P, int axis) {
+  LineShape k{(int)sizeof(T), axis, 0, 0, P.njobs};
+  for (int j = 0; j < P.njobs; j++) {
+    k.n = std::max(k.n, P.job[j].L.n[axis]);
+    k.nl = std::max(k.nl, (long)P.job[j].nlo * P.job[j].nhi);
+  }
+  return k;
+}
+
+static int tuned_choice(const LineShape& k) {
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  const auto it = g_tuned.find(k);
+  return it == g_tuned.end() ? LINE_RULE : it->second.choice;
+}
+
+void set_line_choice(const LineShape& k, int choice, double us_choice, double us_rule) {
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  g_tuned[k] = TunedLine{choice, us_choice, us_rule};
+}
+
+bool has_line_choice(const LineShape& k) {
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  return g_tuned.count(k) != 0;
+}
+
+void clear_line_choices() {
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  g_tuned.clear();
+}
+
+int line_choices(int* out, double* us, int maxn) {
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  int i = 0;
+  for (const auto& kv : g_tuned) {
+    if (i < maxn) {
+      const LineShape& k = kv.first;
+      const int row[7] = {k.esize, k.axis, k.n, (int)std::min<long>(k.nl, INT32_MAX), k.njobs, kv.second.choice, 0};
+      for (int c = 0; c < 6; c++) out[6 * i + c] = row[c];
+      us[2 * i] = kv.second.us_choice;
+      us[2 * i + 1] = kv.second.us_rule;
+    }
+    i++;
+  }
+  return i;
+}
+
 static size_t chunks_of(int values, size_t esize) {
   const size_t v = 16 / esize;
   return (values + v - 1) / v;
@@ -1714,7 +1775,7 @@ static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_
 }
 
 template <typename T>
-static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s) {
+static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t s, int choice = LINE_RULE) {
   int maxl = 0;
   long maxnodes = 0;
   double rhs_bytes = 0, solve_bytes = 0, rec_bytes = 0, comp = 0;
@@ -1767,9 +1828,16 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
   // k_line_solve_x_apply): the finest FP32 x sweeps of several right-hand
   // sides keep the one-lane kernel (3 slots at 128^3: 1% faster), a single
   // right-hand side takes the quad-lane one (430 against 567 us per sweep)
-  const bool quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>() &&
-                                           (axis != 0 || maxl <= 2048 || sizeof(T) == 8 || P.njobs == 1));
-  const bool fuse = quad && maxl <= fuse_max_lines<T>();
+  if (choice == LINE_RULE && g_line_kernel == 0) choice = tuned_choice(line_shape(P, axis));
+  bool quad, fuse;
+  if (choice == LINE_ONE_LANE || choice == LINE_QUAD_FUSED || choice == LINE_QUAD_SEPARATE) {
+    quad = choice != LINE_ONE_LANE;
+    fuse = choice == LINE_QUAD_FUSED;
+  } else {
+    quad = g_line_kernel == 2 || (g_line_kernel == 0 && maxl <= quad_max_lines<T>() &&
+                                  (axis != 0 || maxl <= 2048 || sizeof(T) == 8 || P.njobs == 1));
+    fuse = quad && maxl <= fuse_max_lines<T>();
+  }
   if (fuse) {
     // the substitution kernel computes its own right-hand sides
   } else if (axis == 0) {
@@ -1831,10 +1899,25 @@ template <typename T>
 void launch_line_factor(const LinePack<T>& P, int axis, hipStream_t s) { launch_pack<T>(P, axis, true, s); }
 template <typename T>
 void launch_line_solve(const LinePack<T>& P, int axis, hipStream_t s) { launch_pack<T>(P, axis, false, s); }
+template <typename T>
+void launch_line_solve_as(const LinePack<T>& P, int axis, int choice, hipStream_t s) {
+  launch_pack<T>(P, axis, false, s, choice);
+}
+template <typename T>
+bool line_pcr_shape(const LinePack<T>& P, int axis) {
+  const LineShape k = line_shape(P, axis);
+  return k.nl == 1 && k.n <= PCR_MAX_NODES;
+}
 
 template void launch_line_factor<double>(const LinePack<double>&, int, hipStream_t);
 template void launch_line_factor<float>(const LinePack<float>&, int, hipStream_t);
 template void launch_line_solve<double>(const LinePack<double>&, int, hipStream_t);
 template void launch_line_solve<float>(const LinePack<float>&, int, hipStream_t);
+template void launch_line_solve_as<double>(const LinePack<double>&, int, int, hipStream_t);
+template void launch_line_solve_as<float>(const LinePack<float>&, int, int, hipStream_t);
+template LineShape line_shape<double>(const LinePack<double>&, int);
+template LineShape line_shape<float>(const LinePack<float>&, int);
+template bool line_pcr_shape<double>(const LinePack<double>&, int);
+template bool line_pcr_shape<float>(const LinePack<float>&, int);
 
 }  // namespace emmi

@@ -156,8 +156,18 @@ class Solver {
   // residual, restriction and prolongation (pattern 0), a full line sweep
   // (four colours) along x, y, z; returns 6
   int kernel_rates(int reps, double* ms, double* bytes);
+  // Autotuning of the line sweeps: for every launch shape a solve of all
+  // slots issues (levels, axes, colours, packs of up to LINE_PACK_MAX
+  // slots) and that the process-wide table does not hold yet, times each
+  // substitution choice (one-lane, quad-lane fused, quad-lane with separate
+  // right-hand sides) reps times on the slots' own level vectors and records
+  // the fastest.  The vectors' contents are overwritten (the next solve sets
+  // them again); captured cycle graphs are dropped.  Returns the number of
+  // shapes tuned.
+  int autotune(int reps);
 
  private:
+  template <typename T> int autotune_t(int reps);
   void init(int n1, int n2, int n3, int nmed, const float* const* x1, const float* const* x2,
             const float* const* x3, const float* const* sig11, const float* const* sig22,
             const float* const* sig33, int sig_on_device, const int* slot_med, const float* omegas, int nslots);

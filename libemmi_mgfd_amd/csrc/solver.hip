@@ -477,6 +477,84 @@ void Solver::smooth(int pat, int lev, int iter) {
 }
 
 // ---------------------------------------------------------------------------
+// autotuning of the line-sweep launches
+// ---------------------------------------------------------------------------
+template <typename T>
+int Solver::autotune_t(int reps) {
+  if (reps < 1) reps = 1;
+  std::vector<int> all(slots_.size());
+  for (size_t k = 0; k < slots_.size(); k++) all[k] = (int)k;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  int tuned = 0;
+  // microseconds per launch: the best of three rounds of reps launches
+  // (launches of a few microseconds vary by more than the choices differ)
+  auto time_of = [&](const LinePack<T>& P, int axis, int choice) {
+    launch_line_solve_as<T>(P, axis, choice, s_);   // warm (instruction cache, TLB)
+    double best = 0;
+    for (int round = 0; round < 3; round++) {
+      HIPCHECK(hipEventRecord(e0, s_));
+      for (int r = 0; r < reps; r++) launch_line_solve_as<T>(P, axis, choice, s_);
+      HIPCHECK(hipEventRecord(e1, s_));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      const double us = 1e3 * (double)ms / reps;
+      if (round == 0 || us < best) best = us;
+    }
+    return best;
+  };
+  for (int pat = 0; pat < npat_; pat++) {
+    for (int lev = 0; lev + 1 < lmax_ && p_.isemicoarsen; lev++) {
+      for (int axis = 0; axis < 3; axis++) {
+        if (scs_[pat][lev + 1][axis] != 2) continue;
+        std::vector<const LineFactors*> F(all.size());
+        for (size_t i = 0; i < all.size(); i++) F[i] = &factors<T>(slots_[all[i]], pat, lev, axis);
+        for (int colour = 0; colour < 4; colour++) {
+          for (size_t i0 = 0; i0 < all.size(); i0 += LINE_PACK_MAX) {
+            LinePack<T> P;
+            P.njobs = (int)std::min<size_t>(LINE_PACK_MAX, all.size() - i0);
+            for (int j = 0; j < P.njobs; j++) {
+              Slot& S = slots_[all[i0 + j]];
+              LineJob<T>& J = P.job[j];
+              J.L = hier<T>(S.med).pat[pat][lev].view(S.omu);
+              J.u = (typename cvec<T>::type*)S.u[lev];
+              J.f = (const typename cvec<T>::type*)S.f[lev];
+              J.fac = (typename LineFac<T>::type*)F[i0 + j]->fac[colour];
+              J.scr = (typename LineScr<T>::type*)S.lscr;
+              line_job_colour(J, axis, colour);
+            }
+            if (line_pcr_shape<T>(P, axis)) continue;
+            const LineShape key = line_shape<T>(P, axis);
+            if (key.nl < 1 || has_line_choice(key)) continue;
+            double best = 0;
+            int bc = LINE_RULE;
+            for (int c = 0; c < LINE_NCHOICE; c++) {
+              const double us = time_of(P, axis, c);
+              if (bc == LINE_RULE || us < best) { best = us; bc = c; }
+            }
+            set_line_choice(key, bc, best, time_of(P, axis, LINE_RULE_ONLY));
+            tuned++;
+          }
+        }
+      }
+    }
+  }
+  HIPCHECK(hipEventDestroy(e0));
+  HIPCHECK(hipEventDestroy(e1));
+  return tuned;
+}
+
+int Solver::autotune(int reps) {
+  HIPCHECK(hipDeviceSynchronize());
+  drop_graphs();
+  const int n = p_.precision == 0 ? autotune_t<double>(reps) : autotune_t<float>(reps);
+  HIPCHECK(hipStreamSynchronize(s_));
+  return n;
+}
+
+// ---------------------------------------------------------------------------
 // kernel rates: the level-0 kernels of slot 0 timed one by one
 // ---------------------------------------------------------------------------
 template <typename T>

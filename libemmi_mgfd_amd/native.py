@@ -66,6 +66,12 @@ def lib() -> C.CDLL:
         L.emmi_solver_set_ncycle.argtypes = [C.c_void_p, C.c_int]
         L.emmi_solver_set_first_cycle.restype = C.c_int
         L.emmi_solver_set_first_cycle.argtypes = [C.c_void_p, C.c_int]
+        L.emmi_solver_autotune.restype = C.c_int
+        L.emmi_solver_autotune.argtypes = [C.c_void_p, C.c_int]
+        L.emmi_line_tuning.restype = C.c_int
+        L.emmi_line_tuning.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int]
+        L.emmi_line_tuning_clear.restype = None
+        L.emmi_line_tuning_clear.argtypes = []
         L.emmi_solver_slots.restype = C.c_int
         L.emmi_solver_slots.argtypes = [C.c_void_p]
         from . import carriers as K
@@ -279,6 +285,15 @@ class Solver:
         if lib().emmi_solver_set_ncycle(self.h, int(ncycle)):
             raise NativeError(last_error())
 
+    def autotune(self, reps=5):
+        """Times the line-sweep kernels on every launch shape of this solver
+        not yet in the process-wide table and records the fastest
+        (emmi_solver_autotune); returns the number of shapes tuned."""
+        n = lib().emmi_solver_autotune(self.h, int(reps))
+        if n < 0:
+            raise NativeError(last_error())
+        return n
+
     def set_first_cycle(self, first):
         """Cycle k of the following solves uses pattern (first + k) % 3."""
         if lib().emmi_solver_set_first_cycle(self.h, int(first)):
@@ -351,6 +366,27 @@ class Solver:
 
 def timing_enable(on=True):
     lib().emmi_timing_enable(1 if on else 0)
+
+
+LINE_CHOICES = ["one_lane", "quad_fused", "quad_separate"]
+
+
+def line_tuning():
+    """The autotuner's table: one dict per launch shape."""
+    n = lib().emmi_line_tuning(None, None, 0)
+    rows = (C.c_int * (6 * max(n, 1)))()
+    us = (C.c_double * (2 * max(n, 1)))()
+    n = min(n, lib().emmi_line_tuning(rows, us, n))
+    out = []
+    for k in range(n):
+        r = rows[6 * k:6 * k + 6]
+        out.append({"esize": r[0], "axis": r[1], "n": r[2], "lines": r[3], "jobs": r[4],
+                    "choice": LINE_CHOICES[r[5]], "us": us[2 * k], "us_rule": us[2 * k + 1]})
+    return out
+
+
+def line_tuning_clear():
+    lib().emmi_line_tuning_clear()
 
 
 TIMING_CLASSES = ["residual", "line_solve", "restriction", "prolongation", "other", "line_rhs", "conv", "conv_dw"]

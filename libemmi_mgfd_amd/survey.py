@@ -14,6 +14,9 @@ right-hand side retires at its own convergence test, so it gets exactly the
 iterates it would get alone.  Source injection and receiver extraction run on
 the device (emmi_inject_source_dev, emmi_extract_emf_data_dev).
 
+With ``autotune`` each chunk's solver first times the line-sweep kernels on
+its launch shapes (emmi_solver_autotune, counted in ``setup_s``).
+
 Groups are solved one after another, each in chunks of at most ``max_slots``
 right-hand sides, so that the device memory of one chunk (about 2.5 GB per
 128^3 right-hand side in mixed precision: factor records, vectors of all
@@ -46,7 +49,7 @@ class SurveyRun:
 
     def __init__(self, survey: configs.Survey, grids: Optional[Dict[float, Sequence[int]]] = None,
                  pairs: Optional[Sequence[Tuple[int, int]]] = None, precision: int = 1, tol: float = 1e-6,
-                 ncycle: int = 60, max_slots: int = 25, graphs: bool = True):
+                 ncycle: int = 60, max_slots: int = 25, graphs: bool = True, autotune: bool = False):
         self.survey = survey
         self.kw = dict(precision=precision, tol=tol, ncycle=ncycle, graphs=1 if graphs else 0)
         if grids is None:
@@ -58,6 +61,7 @@ class SurveyRun:
         if not 1 <= max_slots <= 64:
             raise ValueError("max_slots must be 1..64")
         self.max_slots = max_slots
+        self.autotune = autotune
         # group by multigrid size, keep the survey's frequency order
         groups: Dict[Tuple[int, int, int], List[Tuple[int, int]]] = {}
         for s, f in self.pairs:
@@ -96,6 +100,8 @@ class SurveyRun:
         fidx = [ch.group_freqs.index(self.survey.freqs[f]) for _, f in ch.pairs]
         om = [ch.carriers[s].omegas[fi] for (s, _), fi in zip(ch.pairs, fidx)]
         solver = native.Solver.survey(ch.n, grids, med, om, **self.kw)
+        if self.autotune:
+            solver.autotune()
         nb = 16 * 3 * (ch.n[0] + 1) * (ch.n[1] + 1) * (ch.n[2] + 1)
         bufs = {"b": [native.DeviceBuffer(nb) for _ in ch.pairs], "x": [native.DeviceBuffer(nb) for _ in ch.pairs]}
         native.lib().emmi_device_sync()
