@@ -172,11 +172,7 @@ class ShotRun:
                     self._inject(f)
                 x0 = [p.ptr for p in self.x0]
                 extra = 1
-                # the solve from the guess continues the pattern sequence
-                # after the cycle that made its input
-                S.set_first_cycle(1)
             sts = S.solve_batch_device([b.ptr for b in self.b], [x.ptr for x in self.x], [b.ptr for b in self.b], x0)
-            S.set_first_cycle(0)
             for f in range(nf):
                 self._extract(f)
                 st = sts[f]

@@ -60,7 +60,7 @@ def edges_of(net_out, grid, x1, recon, s11=None):
     return x0
 
 
-def evaluate_chunk(net, grids, tol=1e-6, ncycle=60, recons=("avg_air",), first_cycle=1):
+def evaluate_chunk(net, grids, tol=1e-6, ncycle=60, recons=("avg_air",), first_cycle=0):
     """Cycles of the zero, x1 and learned paths (one per reconstruction in
     ``recons``) for one chunk of models; the solves from x1 or a guess
     continue the semi-coarsening pattern sequence at ``first_cycle``."""
@@ -143,7 +143,7 @@ def summarize(rows, seeds, tol, recons, times):
     return out
 
 
-def evaluate(net, seeds, chunk=16, tol=1e-6, log=print, recons=("avg_air",), first_cycle=1):
+def evaluate(net, seeds, chunk=16, tol=1e-6, log=print, recons=("avg_air",), first_cycle=0):
     """Report (dict) over the config-4 models ``seeds``."""
     rows, times = [], {}
     for ss, grids in x0data.config4_chunks(seeds, chunk):
@@ -168,7 +168,9 @@ def main(argv=None):
     p.add_argument("--chunk", type=int, default=16)
     p.add_argument("--out", default=None)
     p.add_argument("--recons", default="avg_air", help="comma list of cell, avg, avg_air")
-    p.add_argument("--first-cycle", type=int, default=1)
+    p.add_argument("--first-cycle", type=int, default=0,
+                   help="pattern index of the first cycle of the solves from x1 or a guess (1 continues the "
+                        "sequence after x1's cycle; 0, the default, restarts it as a zero-guess solve does)")
     a = p.parse_args(argv)
     d = np.load(a.weights)
     net = x0net.UNet((96, 96, 96), levels=tuple(int(v) for v in d["levels"]))

@@ -2,10 +2,13 @@
 
 The network (libemmi_mgfd_amd.x0net) refines x1, the field after one
 multigrid cycle from zero, towards the converged field; the solve then
-iterates from the refined field to its tolerance (continuing the
-semi-coarsening pattern sequence, Solver.set_first_cycle(1)), and the solver
-discards the guess before its first cycle, restarting from zero, when the
-guess's residual ||f - A x0|| is not below the zero guess's ||f||.  The
+iterates from the refined field to its tolerance, and the solver discards
+the guess before its first cycle, restarting from zero, when the guess's
+residual ||f - A x0|| is not below the zero guess's ||f||.  The solve from
+the guess starts the semi-coarsening pattern sequence afresh (cycle 0): the
+right-hand sides of a batch share one sequence, and a rejected guess then
+restarts exactly as the zero-guess solve (continuing at pattern 1, as
+Solver.set_first_cycle(1) allows, would suit accepted guesses only).  The
 weights are trained for the multigrid of the 128^3 survey
 (libemmi_mgfd_amd.x0train); other grid shapes are refused.
 """

@@ -71,7 +71,6 @@ def solve_pair(net, grid, s11, s33, freqs, tol=1e-6):
 
     st0 = run(None)
     x0 = learned_guess(S, net, grid, ds11, ds33, freqs, run, xo)
-    S.set_first_cycle(1)   # continue the pattern sequence after x1's cycle
     st1 = run([p.ptr for p in x0])
     S.close()
     return [{"freq": float(f), "cycles_zero": a.cycles, "cycles_x0": 1 + b.cycles, "rejected": int(b.x0_rejected),

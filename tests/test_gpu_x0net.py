@@ -241,8 +241,7 @@ def test_config4_learned_guess_gain_reproduces_report():
     re-evaluated with the committed weights (libemmi_mgfd_amd/
     x0net_weights_96.npz) give the committed report's cycle counts exactly
     (zero guess; one cycle then a solve from x1; one cycle, the network, then
-    a solve from its guess), and the solve continued from x1 costs exactly
-    the zero-guess solve's cycles.  The report's summary over all 200 test
+    a solve from its guess).  The report's summary over all 200 test
     seeds is printed: it is what the network saves (a negative number means
     it costs cycles)."""
     import json
@@ -263,7 +262,10 @@ def test_config4_learned_guess_gain_reproduces_report():
         assert a["seed"] == b["seed"]
         for k in keys:
             assert a[k] == b[k], (a["seed"], k, a[k], b[k])
-        assert a["cycles_x1"] == a["cycles_zero"]
+        if rep["first_cycle"] == 1:
+            # continuing the pattern sequence after x1's cycle reproduces the
+            # zero-guess solve exactly
+            assert a["cycles_x1"] == a["cycles_zero"]
     s = rep["summary"]
     print(json.dumps({k: s[k] for k in s if k != "wall_s"}), flush=True)
     assert s["models"] == 200 and s["all_converged"]
