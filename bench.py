@@ -25,7 +25,8 @@ Variants: ``plain`` (complex FP64 cycles, one solver per frequency, solved
 one after another), ``mixed`` (FP32-storage cycles inside FP64 refinement,
 one solver per frequency), ``mixed_batched`` (the three frequencies as one
 batched solve) and ``mixed_batched_learned`` (the same from the network's
-initial guess, charged the cycle and the network that make it).  ``value``
+initial guess, blended by the solver with the one-cycle field it refines,
+charged the cycle and the network that make it).  ``value``
 is the fastest variant whose solves all converged (status 0) with finite
 receiver data.
 
@@ -170,9 +171,15 @@ class ShotRun:
                 S.set_ncycle(60)
                 for f in range(nf):
                     self._inject(f)
+                # the solver blends each guess with x1 (the multiple of the
+                # network's correction that minimises the residual), so the
+                # solve from the blend continues x1's pattern sequence
+                S.blend_guess([b.ptr for b in self.b], [x.ptr for x in self.x], [p.ptr for p in self.x0])
+                S.set_first_cycle(1)
                 x0 = [p.ptr for p in self.x0]
                 extra = 1
             sts = S.solve_batch_device([b.ptr for b in self.b], [x.ptr for x in self.x], [b.ptr for b in self.b], x0)
+            S.set_first_cycle(0)
             for f in range(nf):
                 self._extract(f)
                 st = sts[f]

@@ -222,6 +222,16 @@ int emmi_solver_set_ncycle(emmi_solver_t *s, int ncycle);
  * continues from the iterate of `first` earlier cycles keeps the reference's
  * pattern sequence.  Default 0. */
 int emmi_solver_set_first_cycle(emmi_solver_t *s, int first);
+/* No counterpart in the reference: before a solve from guesses, blend each
+ * guess with a multigrid iterate of the same right-hand side.  For slots
+ * 0..n-1 (b: source terms as for emmi_solver_solve_batch; x1: iterates, e.g.
+ * after one cycle from zero; xg: guesses, overwritten), xg <- x1 + beta
+ * (xg - x1) with the complex beta that minimises ||f - A xg||, so the blend's
+ * residual is never above x1's and the solver's rule never rejects it;
+ * beta = 0 when the guess changes nothing or a norm is not finite.  beta
+ * receives 2n doubles (re, im) or may be NULL.  Returns 0 or EMMI_ERROR. */
+int emmi_solver_blend_guess(emmi_solver_t *s, int n, const void *const *b, const void *const *x1, void *const *xg,
+                            double *beta);
 /* Autotuner (no counterpart in the reference).  For every line-sweep launch
  * shape a solve of all the solver's slots issues -- precision, axis, line
  * length, lines per colour, right-hand sides per launch -- that the

@@ -211,6 +211,18 @@ int emmi_solver_autotune(emmi_solver_t* h, int reps) {
   }
 }
 
+int emmi_solver_blend_guess(emmi_solver_t* h, int n, const void* const* b, const void* const* x1, void* const* xg,
+                            double* beta) {
+  if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  try {
+    h->s->blend_guess(n, b, x1, xg, beta);
+    return 0;
+  } catch (const std::exception& e) {
+    set_err(e.what());
+    return EMMI_ERROR;
+  }
+}
+
 int emmi_line_tuning(int* rows, double* us, int maxn) { return emmi::line_choices(rows, us, maxn); }
 
 void emmi_line_tuning_clear(void) { emmi::clear_line_choices(); }

@@ -71,6 +71,12 @@ template <typename T>
 void launch_scale_rhs(const Level<T>& L, const double* D0, const double* D1, const double* D2, const void* b, void* f, hipStream_t s);
 template <typename T>
 void launch_curl_h(const Level<T>& L, const void* u, void* h, hipStream_t s);
+// out3 = (re, im of d^H r1, |d|^2) with d = r1 - rg over n complex doubles;
+// part holds 3 * blend_blocks(n) doubles
+void launch_blend_dots(const void* r1, const void* rg, long n, double* part, double* out3, hipStream_t s);
+int blend_blocks(long n);
+// xg = x1 + beta (xg - x1) over n complex doubles
+void launch_blend(void* xg, const void* x1, double br, double bi, long n, hipStream_t s);
 void launch_demote(const void* hi, void* lo, long n, hipStream_t s);
 void launch_promote_add(void* hi, const void* lo, long n, hipStream_t s);
 void launch_sigma_vol0(int n1, int n2, int n3, const double* W0, const double* W1, const double* W2, const float* sig,

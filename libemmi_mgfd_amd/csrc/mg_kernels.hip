@@ -361,6 +361,65 @@ __global__ void k_promote_add(double2* __restrict__ hi, const float2* __restrict
   if (t < n) { const float2 v = lo[t]; double2 h = hi[t]; h.x += v.x; h.y += v.y; hi[t] = h; }
 }
 
+// Blending a guess with a multigrid iterate (Solver::blend_guess): per block
+// the partial sums of d^H r1 (complex) and |d|^2 with d = r1 - rg, the
+// residuals of the iterate and of the guess; in a second pass the
+// fixed-order sum of the partials
+__global__ void __launch_bounds__(256) k_blend_dots(const double2* __restrict__ r1, const double2* __restrict__ rg,
+                                                    long n, double* __restrict__ part) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double re = 0, im = 0, dd = 0;
+  if (t < n) {
+    const double2 a = r1[t], g = rg[t];
+    const double dr = a.x - g.x, di = a.y - g.y;
+    re = dr * a.x + di * a.y;   // conj(d) r1
+    im = dr * a.y - di * a.x;
+    dd = dr * dr + di * di;
+  }
+  __shared__ double red[3][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_down(re, off, 64);
+    im += __shfl_down(im, off, 64);
+    dd += __shfl_down(dd, off, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { red[0][w] = re; red[1][w] = im; red[2][w] = dd; }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) v += red[threadIdx.x][k];
+    part[3 * blockIdx.x + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_sum3(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+  __shared__ double red[3][4];
+  double v[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+    for (int c = 0; c < 3; c++) v[c] += part[3 * i + c];
+  for (int c = 0; c < 3; c++)
+    for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int c = 0; c < 3; c++) red[c][w] = v[c];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double s = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[threadIdx.x][k];
+    out[threadIdx.x] = s;
+  }
+}
+
+// xg = x1 + beta (xg - x1)
+__global__ void k_blend(double2* __restrict__ xg, const double2* __restrict__ x1, double br, double bi, long n) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) {
+    const double2 a = x1[t], g = xg[t];
+    const double dr = g.x - a.x, di = g.y - a.y;
+    xg[t] = make_double2(a.x + br * dr - bi * di, a.y + br * di + bi * dr);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // medium setup (src/gmg.c:1791-1858)
 // ---------------------------------------------------------------------------
@@ -603,6 +662,16 @@ void launch_demote(const void* hi, void* lo, long n, hipStream_t s) {
 void launch_promote_add(void* hi, const void* lo, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_promote_add, dim3(nblk(n, 256)), dim3(256), 0, s, (double2*)hi, (const float2*)lo, n);
 }
+void launch_blend_dots(const void* r1, const void* rg, long n, double* part, double* out3, hipStream_t s) {
+  const unsigned nb = nblk(n, 256);
+  hipLaunchKernelGGL(k_blend_dots, dim3(nb), dim3(256), 0, s, (const double2*)r1, (const double2*)rg, n, part);
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, part, (int)nb, out3);
+}
+int blend_blocks(long n) { return (int)nblk(n, 256); }
+void launch_blend(void* xg, const void* x1, double br, double bi, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_blend, dim3(nblk(n, 256)), dim3(256), 0, s, (double2*)xg, (const double2*)x1, br, bi, n);
+}
+
 
 void launch_sigma_vol0(int n1, int n2, int n3, const double* W0, const double* W1, const double* W2, const float* sig,
                        double* out, hipStream_t s) {

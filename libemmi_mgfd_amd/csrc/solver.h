@@ -165,6 +165,13 @@ class Solver {
   // them again); captured cycle graphs are dropped.  Returns the number of
   // shapes tuned.
   int autotune(int reps);
+  // Blends guesses with multigrid iterates before a solve from them: for
+  // slots 0..n-1 (b: source terms as for solve_batch, x1: an iterate, xg: a
+  // guess, overwritten) xg <- x1 + beta (xg - x1) with the complex beta that
+  // minimises ||f - A xg|| (f = b scaled by the dual volumes), so the blend's
+  // residual is never above x1's; beta = 0 when xg - x1 leaves the residual
+  // unchanged or a norm is not finite.  beta: 2n doubles (re, im).
+  void blend_guess(int n, const void* const* b, const void* const* x1, void* const* xg, double* beta);
 
  private:
   template <typename T> int autotune_t(int reps);
@@ -206,6 +213,8 @@ class Solver {
   std::vector<Slot> slots_;
   size_t lscr_bytes_ = 0;
   double* dnorm_ = nullptr;   // one per slot
+  double* blend_part_ = nullptr;   // blend_guess: partial sums and their total (allocated on first use)
+  double* blend_dots_ = nullptr;
   double* hnorm_ = nullptr;   // pinned host, one per slot
   std::vector<void*> owned_;
   // state of the running batch

@@ -546,6 +546,42 @@ int Solver::autotune_t(int reps) {
   return tuned;
 }
 
+void Solver::blend_guess(int n, const void* const* b, const void* const* x1, void* const* xg, double* beta) {
+  if (n < 0 || n > (int)slots_.size()) throw std::runtime_error("blend_guess: slot count out of range");
+  // b, x1 and xg may come from kernels on other streams
+  HIPCHECK(hipDeviceSynchronize());
+  const long nv = 3 * nc0_;
+  if (!blend_part_) {
+    blend_part_ = (double*)dalloc(sizeof(double) * 3 * (size_t)blend_blocks(nv), owned_);
+    blend_dots_ = (double*)dalloc(sizeof(double) * 3, owned_);
+  }
+  double *part = blend_part_, *dots = blend_dots_;
+  for (int k = 0; k < n; k++) {
+    Slot& S = slots_[k];
+    const DevLevel<double>& L0 = meds_[S.med].l0_64;
+    launch_scale_rhs<double>(L0.view(S.omu), L0.D(0), L0.D(1), L0.D(2), b[k], S.f64, s_);
+    // residuals of the iterate (r64) and of the guess (u64, free before a solve)
+    launch_residual<double>(L0.view(S.omu), x1[k], S.f64, S.r64, nullptr, s_);
+    launch_residual<double>(L0.view(S.omu), xg[k], S.f64, S.u64, nullptr, s_);
+    launch_blend_dots(S.r64, S.u64, nv, part, dots, s_);
+    double h[3];
+    HIPCHECK(hipMemcpyAsync(h, dots, sizeof(h), hipMemcpyDeviceToHost, s_));
+    HIPCHECK(hipStreamSynchronize(s_));
+    double br = 0, bi = 0;
+    if (std::isfinite(h[0]) && std::isfinite(h[1]) && std::isfinite(h[2]) && h[2] > 0) {
+      br = h[0] / h[2];
+      bi = h[1] / h[2];
+    }
+    launch_blend(xg[k], x1[k], br, bi, nv, s_);
+    if (beta) {
+      beta[2 * k] = br;
+      beta[2 * k + 1] = bi;
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(s_));
+  HIPCHECK(hipGetLastError());
+}
+
 int Solver::autotune(int reps) {
   HIPCHECK(hipDeviceSynchronize());
   drop_graphs();

@@ -66,6 +66,9 @@ def lib() -> C.CDLL:
         L.emmi_solver_set_ncycle.argtypes = [C.c_void_p, C.c_int]
         L.emmi_solver_set_first_cycle.restype = C.c_int
         L.emmi_solver_set_first_cycle.argtypes = [C.c_void_p, C.c_int]
+        L.emmi_solver_blend_guess.restype = C.c_int
+        L.emmi_solver_blend_guess.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.POINTER(C.c_double)]
         L.emmi_solver_autotune.restype = C.c_int
         L.emmi_solver_autotune.argtypes = [C.c_void_p, C.c_int]
         L.emmi_line_tuning.restype = C.c_int
@@ -293,6 +296,16 @@ class Solver:
         if n < 0:
             raise NativeError(last_error())
         return n
+
+    def blend_guess(self, b_ptrs, x1_ptrs, xg_ptrs):
+        """xg <- x1 + beta (xg - x1) per slot with the beta that minimises the
+        residual (emmi_solver_blend_guess); returns the betas (complex)."""
+        n = len(b_ptrs)
+        P = C.c_void_p * n
+        beta = (C.c_double * (2 * max(n, 1)))()
+        if lib().emmi_solver_blend_guess(self.h, n, P(*b_ptrs), P(*x1_ptrs), P(*xg_ptrs), beta):
+            raise NativeError(last_error())
+        return [complex(beta[2 * k], beta[2 * k + 1]) for k in range(n)]
 
     def set_first_cycle(self, first):
         """Cycle k of the following solves uses pattern (first + k) % 3."""

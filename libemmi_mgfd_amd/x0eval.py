@@ -17,6 +17,11 @@ solve that builds its line factors:
   guess whose residual is not below the zero guess's ||f|| is dropped and
   the slot starts from zero (``rejected``); the cycle that made x1 is charged
   either way.
+* ``learned`` with a ``*_blend`` reconstruction: the same, but before the
+  solve the solver blends the guess with x1, x1 + beta (guess - x1) with the
+  beta that minimises the residual (emmi_solver_blend_guess), and the solve
+  continues x1's pattern sequence (the blend's residual is never above x1's,
+  so it is never rejected).
 
 Cycle counts are per right-hand side (each retires at its own convergence
 test); ``saved`` = zero - learned.  Wall times are per chunk (all its
@@ -38,14 +43,15 @@ WEIGHTS_96 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "x0net_wei
 
 
 SIG_AIR = 1e-6   # S/m: cells below it are air (1e-8 in these surveys)
-RECONS = ("cell", "avg", "avg_air")
+RECONS = ("cell", "avg", "avg_air", "cell_blend", "avg_air_blend")
 
 
 def edges_of(net_out, grid, x1, recon, s11=None):
     """Device guess (complex double edge vector) from the network output:
     ``cell`` takes each edge's correction from one cell, ``avg`` averages it
     over the four cells that share the edge, ``avg_air`` also leaves the air
-    cells uncorrected."""
+    cells uncorrected.  (The ``*_blend`` paths of the evaluation build one of
+    these and then let the solver blend it with x1.)"""
     x0 = native.DeviceBuffer(16 * grid.sc.nvec)
     n1, n2, n3 = grid.n
     L = x0net._lib()
@@ -100,15 +106,21 @@ def evaluate_chunk(net, grids, tol=1e-6, ncycle=60, recons=("avg_air",), first_c
     run(None, x1, 0, 1)
     stx = run([x.ptr for x in x1], xo, first_cycle)
     times["x1_s"] = time.perf_counter() - t
-    stl = {}
+    stl, betas = {}, {}
     for rc in recons:
         t = time.perf_counter()
         run(None, x1, 0, 1)
+        blend = rc.endswith("_blend")
         guesses = []
         for g, (a, b), x in zip(grids, ds, x1):
             y = net.forward(x0data.features(g, a, b, g.freq, x))
-            guesses.append(edges_of(y, g, x, rc, a))
-        stl[rc] = run([p.ptr for p in guesses], xo, first_cycle)
+            guesses.append(edges_of(y, g, x, rc[:-len("_blend")] if blend else rc, a))
+        if blend:
+            # the solver takes the multiple of the network's correction of x1
+            # that minimises the residual; the blend is never rejected, so the
+            # solve continues x1's pattern sequence
+            betas[rc] = S.blend_guess([b.ptr for b in bs], [x.ptr for x in x1], [p.ptr for p in guesses])
+        stl[rc] = run([p.ptr for p in guesses], xo, 1 if blend else first_cycle)
         times[f"learned_{rc}_s"] = time.perf_counter() - t
     S.close()
     rows = []
@@ -121,6 +133,8 @@ def evaluate_chunk(net, grids, tol=1e-6, ncycle=60, recons=("avg_air",), first_c
             r[f"cycles_learned_{rc}"] = 1 + int(c.cycles)
             r[f"rejected_{rc}"] = int(c.x0_rejected)
             r[f"guess_rel_residual_{rc}"] = float(c.x0_rnorm / c.rnorm0)
+            if rc in betas:
+                r[f"beta_{rc}"] = [betas[rc][k].real, betas[rc][k].imag]
             status.append(int(c.status))
         r["status"] = status
         rows.append(r)

@@ -268,3 +268,39 @@ def test_continuation_from_first_cycle_iterate(tiny, precision):
     assert st1.cycles + 1 == st0.cycles
     np.testing.assert_allclose(h1, h0[1:len(h1) + 1], rtol=1e-10)
     assert relmax(E1, E0) < 1e-10 and relmax(H1, H0) < 1e-10
+
+
+def test_blend_guess_never_raises_the_residual(tiny):
+    """emmi_solver_blend_guess: x1 + beta (xg - x1) with the residual-minimising
+    beta.  With xg the converged field, beta is 1 to rounding and the blend's
+    residual is the converged one; with xg = x1 plus noise larger than the
+    field, the blend's residual stays at or below x1's; in both cases the
+    solver accepts the blend and a solve from it converges."""
+    s, g, n, L0, b, E_ref, _ = tiny
+    S = solver(s, g, precision=1, tol=1e-8, ncycle=80)
+    nb = 16 * S.nvec
+    db = native.DeviceBuffer(nb).upload(b)
+    dx1 = native.DeviceBuffer(nb)
+    dh = native.DeviceBuffer(nb)
+    S.set_ncycle(1)
+    S.solve_device(db.ptr, dx1.ptr, dh.ptr)
+    S.set_ncycle(80)
+    x1 = dx1.download(np.complex128)
+    dout = native.DeviceBuffer(nb)
+    # residual of x1 itself, as the solver reports it for a guess
+    st_x1 = S.solve_device(db.ptr, dout.ptr, dh.ptr, dx1.ptr)
+    rng = np.random.default_rng(3)
+    noise = np.abs(E_ref).max() * (rng.standard_normal(E_ref.size) + 1j * rng.standard_normal(E_ref.size))
+    noise = np.where(interior_mask(n).reshape(-1), noise, 0)
+    for name, xg in (("converged field", E_ref), ("x1 plus noise", x1 + noise)):
+        dg = native.DeviceBuffer(nb).upload(np.ascontiguousarray(xg, np.complex128))
+        beta, = S.blend_guess([db.ptr], [dx1.ptr], [dg.ptr])
+        S.set_first_cycle(1)
+        st = S.solve_device(db.ptr, dout.ptr, dh.ptr, dg.ptr)
+        S.set_first_cycle(0)
+        print(f"{name}: beta {beta:.3e}, blend residual {st.x0_rnorm / st.rnorm0:.3e} ||f|| "
+              f"(x1: {st_x1.x0_rnorm / st_x1.rnorm0:.3e}), cycles {st.cycles}", flush=True)
+        assert st.x0_rejected == 0 and st.status == 0
+        assert st.x0_rnorm <= st_x1.x0_rnorm * (1 + 1e-12)
+        if name == "converged field":
+            assert abs(beta - 1) < 1e-6 and st.x0_rnorm < 1e-6 * st.rnorm0 and st.cycles <= 1
