@@ -9,8 +9,15 @@ HIP kernels, Adam with cosine decay, one sample per step.  Evaluation: held-out
 models solved to relative residual 1e-6 from a zero guess and from the
 network's prediction, which refines the field after one multigrid cycle (FP32
 cycles in FP64 refinement, all three frequencies batched); the learned path
-is charged that extra cycle.  The solver judges the prediction and falls back
-to zero when it loses.
+is charged that extra cycle.  The solver judges the prediction before its
+first cycle and falls back to zero when its residual is not below ||f||.
+
+    python -m libemmi_mgfd_amd.x0train --config4 --models 2000 --steps 12000 \
+        --lr 1e-3 --out x0net_weights_96.npz --eval-out x0net_eval_96.json
+
+trains on the config-4 earths instead (each on its own 96^3 multigrid, one
+frequency each, online from a replay buffer; train_config4) and evaluates on
+the test seeds with libemmi_mgfd_amd.x0eval.
 """
 from __future__ import annotations
 
