@@ -136,27 +136,33 @@ __global__ void __launch_bounds__(256) k_conv_dw(const bf16* __restrict__ x, con
   const int ci0 = ((t / nco) % nci) * 16, co0 = (t % nco) * 16;
   const int dz = tap / 9 - 1, dyy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
   const int z0 = blockIdx.y * zslab, z1 = min(Z, z0 + zslab);
-  const long nv = (long)(z1 - z0) * Y * X;
+  // voxels of a slab fit 32-bit indices (the launcher checks the volume)
+  const int nv = (z1 - z0) * Y * X;
   const int col = lane & 15, kq = (lane >> 4) * 8;
   f32x4 acc = {0, 0, 0, 0};
-  for (long v0 = (long)wave * 32; v0 < nv; v0 += 128) {
+  for (int v0 = wave * 32; v0 < nv; v0 += 128) {
     bf16x8 a, b;
+    // coordinates of the lane's first voxel, then stepped along x with carry
+    // (one division per 8 elements instead of three 64-bit ones per element)
+    int v = v0 + kq;
+    int xx = v % X, q = v / X;
+    int yy = q % Y, zz = z0 + q / Y;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const long v = v0 + kq + j;
       float av = 0.f, bv = 0.f;
-      if (v < nv) {
-        const int xx = (int)(v % X);
-        const int yy = (int)((v / X) % Y);
-        const int zz = z0 + (int)(v / ((long)X * Y));
-        const long g = ((long)zz * Y + yy) * X + xx;
-        av = bf2f(dy[g * Cout + co0 + col]);
+      if (v + j < nv) {
+        const int g = (zz * Y + yy) * X + xx;
+        av = bf2f(dy[(long)g * Cout + co0 + col]);
         const int sz = zz + dz, sy = yy + dyy, sx = xx + dx;
         if (sz >= 0 && sz < Z && sy >= 0 && sy < Y && sx >= 0 && sx < X)
-          bv = bf2f(x[(((long)sz * Y + sy) * X + sx) * Cin + ci0 + col]);
+          bv = bf2f(x[((long)(sz * Y + sy) * X + sx) * Cin + ci0 + col]);
       }
       a[j] = (__bf16)av;
       b[j] = (__bf16)bv;
+      if (++xx == X) {
+        xx = 0;
+        if (++yy == Y) { yy = 0; ++zz; }
+      }
     }
     // A = dy^T (rows co, cols voxels), B = x_tap (rows voxels, cols ci)
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
@@ -562,6 +568,7 @@ int emmi_x0_conv_fwd(const void* x, int Z, int Y, int X, int Cin, const void* w,
 int emmi_x0_conv_dw(const void* x, const void* dy, int Z, int Y, int X, int Cin, int Cout, float* part, int zslab,
                     float* dw, float* db, void* stream) {
   if (Cin % 16 || Cout % 16 || zslab < 1) return -1;
+  if ((long)Z * Y * X >= (1L << 31)) return -1;   // the kernel indexes voxels in 32 bits
   const int nslab = (Z + zslab - 1) / zslab;
   const dim3 g(27 * (Cin / 16) * (Cout / 16), nslab), b(256);
   hipEvent_t e0, e1;
