@@ -214,6 +214,10 @@ int emmi_solver_autotune(emmi_solver_t* h, int reps) {
 int emmi_solver_blend_guess(emmi_solver_t* h, int n, const void* const* b, const void* const* x1, void* const* xg,
                             double* beta) {
   if (!h || !h->s) { set_err("null solver"); return EMMI_ERROR; }
+  if (n < 0 || n > h->s->nslots()) { set_err("blend_guess: slot count out of range"); return EMMI_ERROR; }
+  if (n > 0 && (!b || !x1 || !xg)) { set_err("blend_guess: null vector table"); return EMMI_ERROR; }
+  for (int k = 0; k < n; k++)
+    if (!b[k] || !x1[k] || !xg[k]) { set_err("blend_guess: null device vector"); return EMMI_ERROR; }
   try {
     h->s->blend_guess(n, b, x1, xg, beta);
     return 0;
