@@ -95,7 +95,81 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, dou
     acc = ((double)v0.re * v0.re + (double)v0.im * v0.im) + ((double)v1.re * v1.re + (double)v1.im * v1.im) +
           ((double)v2.re * v2.re + (double)v2.im * v2.im);
   }
-  if (partial) block_sum_store(acc, partial, blk);
+  if (partial) block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
+}
+
+// The convergence test that opens a step of FP64 iterative refinement, for
+// the slots of one level-0 shape (grid y = slot).  With ADD the step before
+// left its FP32 correction c: every stencil value is read as the updated
+// iterate u + (double)c, and the point's own edges of that sum are written to
+// unext (another vector: the other points still read u and c).  The residual
+// r = f - A u' is demoted into r32, the next FP32 cycle's right-hand side, and
+// its |r|^2 partial sums go to partial[slot * gridDim.x + block].  The values
+// are those of a residual of the updated FP64 iterate followed by a
+// demotion (k_promote_add, k_residual<double>, k_demote).
+template <bool ADD>
+__global__ void __launch_bounds__(256) k_refine_test(Level<double> L, RefineVecs V, double* partial) {
+  const int sl = blockIdx.y;
+  {
+    const double* g = V.geo[sl];
+    const long gs = V.gstride;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      L.W[a] = g + (5 * a + 0) * gs;
+      L.IW[a] = g + (5 * a + 1) * gs;
+      L.H[a] = g + (5 * a + 2) * gs;
+      L.X[a] = g + (5 * a + 3) * gs;
+    }
+    L.sbar = V.sbar[sl];
+    L.omu = V.omu[sl];
+  }
+  const double2* __restrict__ u = V.u[sl];
+  const float2* __restrict__ c = V.c[sl];
+  const double2* __restrict__ f = V.f[sl];
+  const long npts = L.nc;
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long t = blk * blockDim.x + threadIdx.x;
+  double acc = 0;
+  if (t < npts) {
+    auto get = [u, c](long x) -> cxd {
+      if constexpr (ADD) {
+        double2 h = u[x];
+        const float2 v = c[x];
+        h.x += v.x;
+        h.y += v.y;
+        return cxd(h.x, h.y);
+      } else {
+        return ld<double>(u, x);
+      }
+    };
+    int i, j, k;
+    ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
+    const bool in0 = interior<0>(L.n, i, j, k), in1 = interior<1>(L.n, i, j, k), in2 = interior<2>(L.n, i, j, k);
+    cxd v0 = ld<double>(f, t), v1 = ld<double>(f, L.nc + t), v2 = ld<double>(f, 2 * L.nc + t);
+    if (in0 && in1 && in2) {
+      v0 -= apply_row_get<double, 0>(L, get, i, j, k, t);
+      v1 -= apply_row_get<double, 1>(L, get, j, k, i, t);
+      v2 -= apply_row_get<double, 2>(L, get, k, i, j, t);
+    } else {
+      if (in0) v0 -= apply_row_get<double, 0>(L, get, i, j, k, t);
+      if (in1) v1 -= apply_row_get<double, 1>(L, get, j, k, i, t);
+      if (in2) v2 -= apply_row_get<double, 2>(L, get, k, i, j, t);
+    }
+    if constexpr (ADD) {
+      double2* __restrict__ un = V.unext[sl];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const cxd e = get(q * L.nc + t);
+        un[q * L.nc + t] = make_double2(e.re, e.im);
+      }
+    }
+    float2* __restrict__ r32 = V.r32[sl];
+    r32[t] = make_float2((float)v0.re, (float)v0.im);
+    r32[L.nc + t] = make_float2((float)v1.re, (float)v1.im);
+    r32[2 * L.nc + t] = make_float2((float)v2.re, (float)v2.im);
+    acc = (v0.re * v0.re + v0.im * v0.im) + (v1.re * v1.re + v1.im * v1.im) + (v2.re * v2.re + v2.im * v2.im);
+  }
+  block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,7 +254,10 @@ __global__ void __launch_bounds__(128) k_point_gs(Level<T> L, SlotVecs<T> V, int
   for (int m = 0; m < 6; m++) st<T>(u, off[m], ld<T>(u, off[m]) + r[m]);
 }
 
+// block b sums the n partials of slot b (partial + b n) into out[b]
 __global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+  partial += (long)blockIdx.x * n;
+  out += blockIdx.x;
   double v = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
   __shared__ double red[16];
@@ -510,7 +587,7 @@ static void slot_transfers(SlotVecs<T>& B, const Transfer<T>* Xs, int n) {
 
 template <typename T>
 void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
-                           const double* omu, hipStream_t s) {
+                           const double* omu, hipStream_t s, double* partial) {
   using V = typename cvec<T>::type;
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("residual batch of 1..MG_BATCH_MAX slots");
   SlotVecs<T> B = {};
@@ -524,7 +601,39 @@ void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, cons
   const Level<T>& L = Ls[0];
   const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
   timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (3 * sz + sr));
-  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256), n), dim3(256), 0, s, L, B, (double*)nullptr);
+  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256), n), dim3(256), 0, s, L, B, partial);
+}
+
+void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* c,
+                        const void* const* f, void* const* unext, void* const* r32, double* partial, hipStream_t s) {
+  if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("refinement test of 1..MG_BATCH_MAX slots");
+  if (!partial) throw std::runtime_error("refinement test without partial sums");
+  RefineVecs V = {};
+  V.gstride = Ls[0].IW[0] - Ls[0].W[0];
+  for (int k = 0; k < n; k++) {
+    for (int a = 0; a < 3; a++)
+      if (Ls[k].n[a] != Ls[0].n[a]) throw std::runtime_error("batched levels differ in shape");
+    if (Ls[k].IW[0] - Ls[k].W[0] != V.gstride) throw std::runtime_error("batched levels differ in layout");
+    if (!u[k] || !f[k] || !r32[k] || (c && (!c[k] || !unext[k])))
+      throw std::runtime_error("refinement test: null vector");
+    V.geo[k] = Ls[k].W[0];
+    V.sbar[k] = Ls[k].sbar;
+    V.omu[k] = Ls[k].omu;
+    V.u[k] = (const double2*)u[k];
+    V.f[k] = (const double2*)f[k];
+    V.r32[k] = (float2*)r32[k];
+    if (c) {
+      V.c[k] = (const float2*)c[k];
+      V.unext[k] = (double2*)unext[k];
+    }
+  }
+  const Level<double>& L = Ls[0];
+  // u, f, sbar read and r32 written; with the update also c read and unext
+  // written
+  timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (16.0 + 16.0 + 8.0 + 8.0 + (c ? 8.0 + 16.0 : 0.0)));
+  const dim3 g(nblk(L.nc, 256), n), b(256);
+  if (c) hipLaunchKernelGGL(k_refine_test<true>, g, b, 0, s, L, V, partial);
+  else hipLaunchKernelGGL(k_refine_test<false>, g, b, 0, s, L, V, partial);
 }
 
 template <typename T>
@@ -545,6 +654,10 @@ int residual_blocks(long nc) { return (int)nblk(nc, 256); }
 
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, n, out);
+}
+
+void launch_sum_partials_batch(const double* partial, int n, int nslots, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(nslots), dim3(256), 0, s, partial, n, out);
 }
 
 template <typename T>
@@ -692,7 +805,7 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 #define EMMI_INSTANTIATE(T)                                                                                    \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
   template void launch_residual_batch<T>(const Level<T>*, int, const void* const*, const void* const*,         \
-                                         void* const*, const double*, hipStream_t);                            \
+                                         void* const*, const double*, hipStream_t, double*);                   \
   template void launch_restrict_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>*, int,            \
                                          const void* const*, void* const*, void* const*, hipStream_t);         \
   template void launch_prolong_batch<T>(const Level<T>&, const Level<T>&, const Transfer<T>*, int,             \

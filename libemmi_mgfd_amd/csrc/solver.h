@@ -193,15 +193,19 @@ class Solver {
   template <typename T> bool f_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool w_cycle(int pat, int lev, bool norm_here, bool zero_guess);
   template <typename T> bool coarse_correction(int pat, int lev, bool norm_here, bool zero_guess, int kind);
-  // level-0 FP64 residual norms of the active slots (r64 <- f64 - A u64);
-  // retires the converged ones and returns true when none is left
+  // level-0 FP64 residual norms of the active slots (see norms); retires
+  // the converged ones and returns true when none is left
   bool check_active(int pat);
   // one cycle after its convergence test, as a replayed graph when possible
   void cycle_body(int pat);
   void cycle_body_eager(int pat);
   void drop_graphs();
-  void norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,
-             std::vector<double>& out);
+  // ||f64 - A u64|| of the slots `which`, batched.  FP64 cycles leave the
+  // residual in r64 (= r[0]).  FP32-storage cycles leave it demoted in f[0]
+  // (the next cycle's right-hand side); with `add` each slot's pending FP32
+  // correction u[0] is added to u64 first (into r64's buffer, which then
+  // becomes u64: u64 and r64 swap).
+  void norms(const std::vector<int>& which, bool add, std::vector<double>& out);
 
   int n_[3];
   long nc0_;
@@ -213,6 +217,7 @@ class Solver {
   std::vector<Slot> slots_;
   size_t lscr_bytes_ = 0;
   double* dnorm_ = nullptr;   // one per slot
+  double* part_all_ = nullptr;   // residual-norm partials of MG_BATCH_MAX slots
   double* blend_part_ = nullptr;   // blend_guess: partial sums and their total (allocated on first use)
   double* blend_dots_ = nullptr;
   double* hnorm_ = nullptr;   // pinned host, one per slot
@@ -223,6 +228,7 @@ class Solver {
   int first_cycle_ = 0;
   bool r0_ready_ = false;     // r[0] holds f - A u of the level-0 test (FP64 form)
   bool rnorm0_in_cycle_ = false;  // rnorm0 is the cycle-0 in-cycle norm (FP64 V/W cycles)
+  bool pending_ = false;      // FP32 cycles: the active slots' u[0] holds a correction not yet in u64
   // captured cycle bodies, keyed by pattern and active-slot mask; valid for
   // the slots' frequencies at capture time
   std::map<std::pair<int, unsigned long long>, hipGraphExec_t> graphs_;

@@ -255,6 +255,7 @@ void Solver::init(int n1, int n2, int n3, int nmed, const float* const* x1, cons
     S.partial = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
   }
   dnorm_ = (double*)dalloc(sizeof(double) * nslots, owned_);
+  part_all_ = (double*)dalloc(sizeof(double) * MG_BATCH_MAX * residual_blocks(nc0_), owned_);
   HIPCHECK(hipHostMalloc((void**)&hnorm_, sizeof(double) * nslots, hipHostMallocDefault));
   HIPCHECK(hipStreamSynchronize(s_));
 }
@@ -550,6 +551,7 @@ void Solver::blend_guess(int n, const void* const* b, const void* const* x1, voi
   if (n < 0 || n > (int)slots_.size()) throw std::runtime_error("blend_guess: slot count out of range");
   // b, x1 and xg may come from kernels on other streams
   HIPCHECK(hipDeviceSynchronize());
+  (void)hipGetLastError();   // as in solve_batch: the final check reports this call's launches
   const long nv = 3 * nc0_;
   if (!blend_part_) {
     blend_part_ = (double*)dalloc(sizeof(double) * 3 * (size_t)blend_blocks(nv), owned_);
@@ -642,31 +644,47 @@ int Solver::kernel_rates(int reps, double* ms, double* bytes) {
 // ---------------------------------------------------------------------------
 // convergence tests
 // ---------------------------------------------------------------------------
-void Solver::norms(const std::vector<int>& which, const void* const* u, const void* const* f, void* const* r,
-                   std::vector<double>& out) {
-  for (size_t i = 0; i < which.size(); i++) {
-    Slot& S = slots_[which[i]];
-    const DevLevel<double>& L0 = meds_[S.med].l0_64;
-    launch_residual<double>(L0.view(S.omu), u[i], f[i], r[i], S.partial, s_);
-    launch_sum_partials(S.partial, residual_blocks(L0.nc), dnorm_ + which[i], s_);
+void Solver::norms(const std::vector<int>& which, bool add, std::vector<double>& out) {
+  const int nb = residual_blocks(nc0_);
+  for (size_t i0 = 0; i0 < which.size(); i0 += MG_BATCH_MAX) {
+    const int n = (int)std::min<size_t>(MG_BATCH_MAX, which.size() - i0);
+    const void* u[MG_BATCH_MAX];
+    const void* f[MG_BATCH_MAX];
+    const void* c[MG_BATCH_MAX];
+    void* r[MG_BATCH_MAX];
+    void* r32[MG_BATCH_MAX];
+    double omu[MG_BATCH_MAX];
+    Level<double> Ls[MG_BATCH_MAX];
+    for (int j = 0; j < n; j++) {
+      Slot& S = slots_[which[i0 + j]];
+      Ls[j] = meds_[S.med].l0_64.view(S.omu);
+      u[j] = S.u64;
+      f[j] = S.f64;
+      r[j] = S.r64;
+      c[j] = S.u[0];
+      r32[j] = S.f[0];
+      omu[j] = S.omu;
+    }
+    if (p_.precision == 0) {
+      launch_residual_batch<double>(Ls, n, u, f, r, omu, s_, part_all_);
+    } else {
+      launch_refine_test(Ls, n, u, add ? c : nullptr, f, r, r32, part_all_, s_);
+      if (add)
+        for (int j = 0; j < n; j++) std::swap(slots_[which[i0 + j]].u64, slots_[which[i0 + j]].r64);
+    }
+    launch_sum_partials_batch(part_all_, nb, n, dnorm_ + i0, s_);
   }
-  HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double) * slots_.size(), hipMemcpyDeviceToHost, s_));
+  HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double) * which.size(), hipMemcpyDeviceToHost, s_));
   HIPCHECK(hipStreamSynchronize(s_));
   out.resize(which.size());
-  for (size_t i = 0; i < which.size(); i++) out[i] = std::sqrt(hnorm_[which[i]]);
+  for (size_t i = 0; i < which.size(); i++) out[i] = std::sqrt(hnorm_[i]);
 }
 
 bool Solver::check_active(int pat) {
   (void)pat;
-  std::vector<const void*> u, f;
-  std::vector<void*> r;
-  for (int k : act_) {
-    u.push_back(slots_[k].u64);
-    f.push_back(slots_[k].f64);
-    r.push_back(slots_[k].r64);
-  }
   std::vector<double> rn;
-  norms(act_, u.data(), f.data(), r.data(), rn);
+  norms(act_, pending_, rn);
+  pending_ = false;
   std::vector<int> keep;
   for (size_t i = 0; i < act_.size(); i++) {
     Slot& S = slots_[act_[i]];
@@ -824,16 +842,13 @@ void Solver::cycle_body_eager(int pat) {
     r0_ready_ = false;
     return;
   }
-  // refinement: the test left r64 = f - A u64; FP32-storage cycle on it from
-  // zero, FP64 update
-  for (int k : act_) {
-    launch_demote(slots_[k].r64, slots_[k].f[0], nv, s_);
-    launch_zero<float>(slots_[k].u[0], nv, s_);
-  }
+  // refinement: the test left f[0] = f - A u64 in FP32; FP32-storage cycle
+  // on it from zero.  Its correction u[0] is added to u64 by the next test
+  // (or after the last cycle).
+  for (int k : act_) launch_zero<float>(slots_[k].u[0], nv, s_);
   if (kind == 1) v_cycle<float>(pat, 0, false, true);
   else if (kind == 3) w_cycle<float>(pat, 0, false, true);
   else f_cycle<float>(pat, 0, false, true);
-  for (int k : act_) launch_promote_add(slots_[k].u64, slots_[k].u[0], nv, s_);
 }
 
 void Solver::drop_graphs() {
@@ -875,6 +890,8 @@ void Solver::cycle_body(int pat) {
     HIPCHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     cycle_body_eager(pat);
     HIPCHECK(hipStreamEndCapture(s_, &g));
+    // kernel launches report their errors here, not by a return code
+    check_hip(hipGetLastError(), "a kernel launch while capturing a cycle body");
     HIPCHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     HIPCHECK(hipGraphDestroy(g));
     it = graphs_.emplace(key, ge).first;
@@ -894,6 +911,10 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
                         SolveStats* st) {
   if (n < 1 || n > nslots()) throw std::runtime_error("batch size out of range of the solver's slots");
   const long nv = 3 * nc0_;
+  // The launch-error check at the end of the solve reads the thread's last
+  // HIP error; clear what calls outside the solver left there (unchecked
+  // frees and queries of other objects) so that it reports this solve's.
+  (void)hipGetLastError();
   // The solver's stream does not order itself after other streams: wait for
   // the work the caller queued elsewhere (b and x0 may still be in flight).
   HIPCHECK(hipDeviceSynchronize());
@@ -909,13 +930,11 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
     launch_zero<double>(S.u64, nv, s_);
     act_.push_back(k);
   }
+  pending_ = false;
   // ||f||: the reference's rnorm0 (its first residual, taken at u = 0)
   {
-    std::vector<const void*> u, f;
-    std::vector<void*> r;
-    for (int k : act_) { u.push_back(slots_[k].u64); f.push_back(slots_[k].f64); r.push_back(slots_[k].r64); }
     std::vector<double> rn;
-    norms(act_, u.data(), f.data(), r.data(), rn);
+    norms(act_, false, rn);
     for (size_t i = 0; i < act_.size(); i++) {
       slots_[act_[i]].rnorm0 = rn[i];
       slots_[act_[i]].st->rnorm0 = rn[i];
@@ -927,19 +946,14 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
   // to ||f|| either way.
   {
     std::vector<int> g;
-    std::vector<const void*> u, f;
-    std::vector<void*> r;
     for (int k = 0; k < n; k++)
       if (x0 && x0[k]) {
         HIPCHECK(hipMemcpyAsync(slots_[k].u64, x0[k], sizeof(double2) * nv, hipMemcpyDeviceToDevice, s_));
         g.push_back(k);
-        u.push_back(slots_[k].u64);
-        f.push_back(slots_[k].f64);
-        r.push_back(slots_[k].r64);
       }
     if (!g.empty()) {
       std::vector<double> rn;
-      norms(g, u.data(), f.data(), r.data(), rn);
+      norms(g, false, rn);
       for (size_t i = 0; i < g.size(); i++) {
         Slot& S = slots_[g[i]];
         S.st->x0_rnorm = rn[i];
@@ -961,6 +975,7 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
       // F-cycle); the rest of the cycle runs as one body
       if (check_active(pat)) break;
       cycle_body(pat);
+      pending_ = p_.precision != 0;
       continue;
     }
     // FP64 V/W cycles test after the pre-smoothing (src/gmg.c:1599,1656)
@@ -976,6 +991,10 @@ int Solver::solve_batch(int n, const void* const* b, void* const* x, void* const
       if (done) break;
     }
   }
+  // slots that ran out of cycles still hold their last cycle's correction
+  if (pending_)
+    for (int k : act_) launch_promote_add(slots_[k].u64, slots_[k].u[0], nv, s_);
+  pending_ = false;
   for (int k = 0; k < n; k++) {
     Slot& S = slots_[k];
     SolveStats* s = S.st;

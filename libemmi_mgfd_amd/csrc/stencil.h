@@ -16,23 +16,37 @@ namespace emmi {
 // ---------------------------------------------------------------------------
 // operator row
 // ---------------------------------------------------------------------------
+// a x - b y per component with one rounded product: fma(a, x, -(b y))
+__device__ __forceinline__ cxd fms2(const cxd& a, double x, const cxd& b, double y) {
+  return cxd(fma(a.re, x, -(b.re * y)), fma(a.im, x, -(b.im * y)));
+}
+
+// get(x) returns the value at flat index x of the edge vector (component
+// blocks L.nc apart); o is the flat index of node p.  Every product is an
+// explicit fma or stands alone, so the compiler has nothing left to contract
+// and each kernel that calls this (whatever its loads) rounds alike.
+template <typename T, int A, typename Get>
+__device__ __forceinline__ cxd apply_row_get(const Level<T>& L, const Get& get, int pa, int pb, int pc, long o) {
+  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  const long sa = L.es[A], sb = L.es[B], sc = L.es[C];
+  const long oa = A * L.nc + o, ob = B * L.nc + o, oc = C * L.nc + o;
+  const double iWa = L.IW[A][pa], iWb = L.IW[B][pb], iWbm = L.IW[B][pb - 1];
+  const double iWc = L.IW[C][pc], iWcm = L.IW[C][pc - 1];
+  const cxd ea = get(oa);
+  const cxd c1 = fms2(get(ob + sa) - get(ob), iWa, get(oa + sb) - ea, iWb);
+  const cxd c2 = fms2(get(ob + sa - sb) - get(ob - sb), iWa, ea - get(oa - sb), iWbm);
+  const cxd c3 = fms2(get(oa + sc) - ea, iWc, get(oc + sa) - get(oc), iWa);
+  const cxd c4 = fms2(ea - get(oa - sc), iWcm, get(oc + sa - sc) - get(oc - sc), iWa);
+  const cxd q = fms2(c1 - c2, (double)L.H[C][pc], c3 - c4, (double)L.H[B][pb]);
+  // W_a q - i omu sbar ea
+  const double wa = L.W[A][pa], s = L.omu * (double)L.sbar[oa];
+  return cxd(fma(q.re, wa, ea.im * s), fma(q.im, wa, -(ea.re * s)));
+}
+
 template <typename T, int A>
 __device__ __forceinline__ cxd apply_row(const Level<T>& L, const typename cvec<T>::type* __restrict__ u,
                                            int pa, int pb, int pc, long o) {
-  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
-  const long sa = L.es[A], sb = L.es[B], sc = L.es[C];
-  const typename cvec<T>::type* ua = u + A * L.nc + o;
-  const typename cvec<T>::type* ub = u + B * L.nc + o;
-  const typename cvec<T>::type* uc = u + C * L.nc + o;
-  const double iWa = L.IW[A][pa], iWb = L.IW[B][pb], iWbm = L.IW[B][pb - 1];
-  const double iWc = L.IW[C][pc], iWcm = L.IW[C][pc - 1];
-  const cxd ea = ld<T>(ua, 0);
-  const cxd c1 = (ld<T>(ub, sa) - ld<T>(ub, 0)) * iWa - (ld<T>(ua, sb) - ea) * iWb;
-  const cxd c2 = (ld<T>(ub, sa - sb) - ld<T>(ub, -sb)) * iWa - (ea - ld<T>(ua, -sb)) * iWbm;
-  const cxd c3 = (ld<T>(ua, sc) - ea) * iWc - (ld<T>(uc, sa) - ld<T>(uc, 0)) * iWa;
-  const cxd c4 = (ea - ld<T>(ua, -sc)) * iWcm - (ld<T>(uc, sa - sc) - ld<T>(uc, -sc)) * iWa;
-  cxd r = ((c1 - c2) * (double)L.H[C][pc] - (c3 - c4) * (double)L.H[B][pb]) * (double)L.W[A][pa];
-  return r + mul_minus_i(ea, L.omu * (double)L.sbar[A * L.nc + o]);
+  return apply_row_get<T, A>(L, [u](long x) { return ld<T>(u, x); }, pa, pb, pc, o);
 }
 
 template <int A>
