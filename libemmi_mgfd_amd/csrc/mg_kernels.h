@@ -58,13 +58,13 @@ void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, d
 // For n slots of level-0 shape (Ls[k]: slot k's FP64 level 0): with c (a
 // table of pending FP32 corrections) non-null, unext_k = u_k + c_k and
 // r = f_k - A unext_k, else r = f_k - A u_k; r is written demoted to FP32 into
-// r32_k and the |r|^2 partials of slot k go to partial + k residual_blocks(nc).
+// r32_k and the |r|^2 partials of slot k go to partial + k residual_blocks<double>(n).
 // One launch.
 void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* c,
                         const void* const* f, void* const* unext, void* const* r32, double* partial, hipStream_t s);
 // r_k = f_k - A(omu_k) u_k for n slots of levels of one shape (Ls[k]: slot
 // k's level), one launch
-// (partial, when non-null: slot k's |r|^2 partials at partial + k residual_blocks(nc))
+// (partial, when non-null: slot k's |r|^2 partials at partial + k residual_blocks<T>(n))
 template <typename T>
 void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s, double* partial = nullptr);
@@ -82,7 +82,11 @@ void launch_restrict_batch(const Level<T>& F, const Level<T>& C, const Transfer<
 template <typename T>
 void launch_prolong_batch(const Level<T>& F, const Level<T>& C, const Transfer<T>* Xs, int n, const void* const* uc,
                           void* const* uf, hipStream_t s);
-int residual_blocks(long nc);
+// partial sums per slot of a residual launch over a level of shape n with
+// storage T, and an upper bound for either storage
+template <typename T>
+int residual_blocks(const int* n);
+int residual_blocks_max(const int* n);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s);
 // out[k] = the sum of the n partials at partial + k n, k < nslots (one launch)
 void launch_sum_partials_batch(const double* partial, int n, int nslots, double* out, hipStream_t s);

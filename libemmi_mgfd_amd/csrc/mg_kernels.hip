@@ -14,6 +14,8 @@
 // the reference's m/(d d) coefficient form with invmur*vol = W_a W_b W_c.
 // Only interior edges carry an equation; boundary-tangential edges are PEC
 // zeros and are never written.
+#include <algorithm>
+#include <climits>
 #include <stdexcept>
 
 #include "emmi_dev.h"
@@ -60,42 +62,243 @@ __device__ __forceinline__ void block_sum_store(double v, double* partial, long 
   }
 }
 
-// r = f - A u on interior edges, r = f elsewhere; optional |r|^2 partials
-template <typename T>
+// ---------------------------------------------------------------------------
+// Residual r = f - A u (r = f on the edges that carry no equation), with
+// optional |r|^2 partial sums per block.
+//
+// A block of 256 threads owns 256 consecutive points q = i + es1 j of the
+// (i, j) plane and a chunk of KC planes; each thread marches its column
+// (i, j) up through k.  Of the 24 stencil values of a point's three rows,
+// the nine that lie in the plane below or in the thread's own column are
+// carried in registers from the previous step, so a step loads 15 (the
+// values of plane k+1 are loaded as raw storage values and widened when
+// used).  Blocks go to the XCDs in contiguous runs (xcd_block), so the rows
+// of neighbouring strips that a strip reads for its j +- 1 neighbours sit in
+// the same L2.  With NTS (FP64 levels larger than the 256 MB last-level
+// cache) f, sbar and r are streamed with nontemporal loads and stores, which
+// leaves the cache to u.  The arithmetic is the stencil row of stencil.h term
+// by term (explicit fma), so every residual kernel rounds alike.
+// ---------------------------------------------------------------------------
+// The level geometry of slot sl of a batched launch
+template <typename T, typename Args>
+__device__ __forceinline__ void slot_geo(Level<T>& L, const Args& V, int sl) {
+  const double* g = V.geo[sl];
+  const long gs = V.gstride;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    L.W[a] = g + (5 * a + 0) * gs;
+    L.IW[a] = g + (5 * a + 1) * gs;
+    L.H[a] = g + (5 * a + 2) * gs;
+    L.X[a] = g + (5 * a + 3) * gs;
+  }
+  L.sbar = V.sbar[sl];
+  L.omu = V.omu[sl];
+}
+
+template <typename V, bool NT>
+__device__ __forceinline__ V ld_stream(const V* p, int i) {
+  if constexpr (NT) {
+    using E = decltype(V().x);
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    const v2 a = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p) + i);
+    V v;
+    v.x = a.x;
+    v.y = a.y;
+    return v;
+  } else {
+    return p[i];
+  }
+}
+template <typename E, bool NT>
+__device__ __forceinline__ E ld_stream_s(const E* p, int i) {
+  if constexpr (NT) return __builtin_nontemporal_load(p + i);
+  else return p[i];
+}
+template <typename V, bool NT>
+__device__ __forceinline__ void st_stream(V* p, int i, double re, double im) {
+  using E = decltype(V().x);
+  if constexpr (NT) {
+    typedef E v2 __attribute__((ext_vector_type(2)));
+    v2 a;
+    a.x = (E)re;
+    a.y = (E)im;
+    __builtin_nontemporal_store(a, reinterpret_cast<v2*>(p) + i);
+  } else {
+    V v;
+    v.x = (E)re;
+    v.y = (E)im;
+    p[i] = v;
+  }
+}
+
+template <typename R> __device__ __forceinline__ cxd cxv(const R& v) { return cxd((double)v.x, (double)v.y); }
+
+// The stencil values of a step that are not carried (raw storage values)
+template <typename R, typename T>
+struct MarchPlane {
+  R xpj, xmj, xpjmi, ypi, ymi, ypimj, z0, zpi, zmi, zpj, zmj, xp, xpmi, yp, ypmj;
+  typename cvec<T>::type f0, f1, f2;
+  T s0, s1, s2;
+};
+
+// March of one thread over planes k0..k1-1 of column q.  src(x) returns the
+// raw value at flat index x of the edge vector (components nc apart);
+// sink(t, v0, v1, v2, ex, ey, ez) receives the residual of point t and the
+// point's own three values of the vector the stencil read.
+template <typename T, bool NTS, typename Src, typename Sink>
+__device__ __forceinline__ void residual_march(const Level<T>& L, int q, int k0, int k1, const Src& src,
+                                               const typename cvec<T>::type* __restrict__ f, Sink& sink) {
+  using R = decltype(src(0));
+  const int e1 = (int)L.es[1], e2 = (int)L.es[2], nc = (int)L.nc;
+  const int i = q % e1, j = q / e1;
+  // only these neighbours can fall outside the vector (they then belong to
+  // rows without an equation): E_x at t - 1, t - e1, t - e2 and E_z at
+  // t + 1, t + e1 (the other offsets stay inside the three components)
+  auto load = [&](MarchPlane<R, T>& v, int t) {
+    v.xpj = src(t + e1);
+    v.xmj = src(max(t - e1, 0));
+    v.xpjmi = src(t + e1 - 1);
+    v.ypi = src(nc + t + 1);
+    v.ymi = src(nc + t - 1);
+    v.ypimj = src(nc + t + 1 - e1);
+    v.z0 = src(2 * nc + t);
+    v.zpi = src(2 * nc + min(t + 1, nc - 1));
+    v.zmi = src(2 * nc + t - 1);
+    v.zpj = src(2 * nc + min(t + e1, nc - 1));
+    v.zmj = src(2 * nc + t - e1);
+    v.xp = src(t + e2);
+    v.xpmi = src(t + e2 - 1);
+    v.yp = src(nc + t + e2);
+    v.ypmj = src(nc + t + e2 - e1);
+    v.f0 = ld_stream<typename cvec<T>::type, NTS>(f, t);
+    v.f1 = ld_stream<typename cvec<T>::type, NTS>(f, nc + t);
+    v.f2 = ld_stream<typename cvec<T>::type, NTS>(f, 2 * nc + t);
+    v.s0 = ld_stream_s<T, NTS>(L.sbar, t);
+    v.s1 = ld_stream_s<T, NTS>(L.sbar, nc + t);
+    v.s2 = ld_stream_s<T, NTS>(L.sbar, 2 * nc + t);
+  };
+  int t = q + k0 * e2;
+  // carried: E_x, E_y at (i, j, k) and (i-1, j, k) / (i, j-1, k); plane k-1
+  cxd xk = cxv(src(t)), xmi = cxv(src(max(t - 1, 0))), yk = cxv(src(nc + t)), ymj = cxv(src(nc + t - e1));
+  cxd xm = cxv(src(max(t - e2, 0))), ym = cxv(src(nc + t - e2)), zm = cxv(src(2 * nc + t - e2)),
+      zpim = cxv(src(2 * nc + t + 1 - e2)), zpjm = cxv(src(2 * nc + t + e1 - e2));
+  const double iW0 = L.IW[0][i], iW0m = i > 0 ? L.IW[0][i - 1] : 0.0;
+  const double iW1 = L.IW[1][j], iW1m = j > 0 ? L.IW[1][j - 1] : 0.0;
+  const double H0 = L.H[0][i], H1 = L.H[1][j], W0 = L.W[0][i], W1 = L.W[1][j];
+  const bool ii = i >= 1 && i <= L.n[0] - 1, jj = j >= 1 && j <= L.n[1] - 1;
+  MarchPlane<R, T> cur;
+  load(cur, t);
+  for (int k = k0; k < k1; k++, t += e2) {
+    cxd v0 = cxv(cur.f0), v1 = cxv(cur.f1), v2 = cxv(cur.f2);
+    const bool kk = k >= 1 && k <= L.n[2] - 1;
+    const bool in0 = i < L.n[0] && jj && kk, in1 = j < L.n[1] && kk && ii, in2 = k < L.n[2] && ii && jj;
+    const cxd z0 = cxv(cur.z0), zpi = cxv(cur.zpi), zpj = cxv(cur.zpj);
+    const cxd xp = cxv(cur.xp), xpmi = cxv(cur.xpmi), yp = cxv(cur.yp), ypmj = cxv(cur.ypmj);
+    if (in0 || in1 || in2) {
+      const cxd xpj = cxv(cur.xpj), xmj = cxv(cur.xmj), xpjmi = cxv(cur.xpjmi);
+      const cxd ypi = cxv(cur.ypi), ymi = cxv(cur.ymi), ypimj = cxv(cur.ypimj);
+      const cxd zmi = cxv(cur.zmi), zmj = cxv(cur.zmj);
+      const double iW2 = L.IW[2][k], iW2m = k > 0 ? L.IW[2][k - 1] : 0.0, H2 = L.H[2][k], W2 = L.W[2][k];
+      const double s0 = L.omu * (double)cur.s0, s1 = L.omu * (double)cur.s1, s2 = L.omu * (double)cur.s2;
+      // rows as apply_row_get<T, A> with (pa, pb, pc) = (i, j, k), (j, k, i), (k, i, j)
+      if (in0) {
+        const cxd ea = xk;
+        const cxd c1 = fms2(ypi - yk, iW0, xpj - ea, iW1);
+        const cxd c2 = fms2(ypimj - ymj, iW0, ea - xmj, iW1m);
+        const cxd c3 = fms2(xp - ea, iW2, zpi - z0, iW0);
+        const cxd c4 = fms2(ea - xm, iW2m, zpim - zm, iW0);
+        const cxd qq = fms2(c1 - c2, H2, c3 - c4, H1);
+        v0 -= cxd(fma(qq.re, W0, ea.im * s0), fma(qq.im, W0, -(ea.re * s0)));
+      }
+      if (in1) {
+        const cxd ea = yk;
+        const cxd c1 = fms2(zpj - z0, iW1, yp - ea, iW2);
+        const cxd c2 = fms2(zpjm - zm, iW1, ea - ym, iW2m);
+        const cxd c3 = fms2(ypi - ea, iW0, xpj - xk, iW1);
+        const cxd c4 = fms2(ea - ymi, iW0m, xpjmi - xmi, iW1);
+        const cxd qq = fms2(c1 - c2, H0, c3 - c4, H2);
+        v1 -= cxd(fma(qq.re, W1, ea.im * s1), fma(qq.im, W1, -(ea.re * s1)));
+      }
+      if (in2) {
+        const cxd ea = z0;
+        const cxd c1 = fms2(xp - xk, iW2, zpi - ea, iW0);
+        const cxd c2 = fms2(xpmi - xmi, iW2, ea - zmi, iW0m);
+        const cxd c3 = fms2(zpj - ea, iW1, yp - yk, iW2);
+        const cxd c4 = fms2(ea - zmj, iW1m, ypmj - ymj, iW2);
+        const cxd qq = fms2(c1 - c2, H1, c3 - c4, H0);
+        v2 -= cxd(fma(qq.re, W2, ea.im * s2), fma(qq.im, W2, -(ea.re * s2)));
+      }
+    }
+    sink(t, v0, v1, v2, xk, yk, z0);
+    xm = xk; ym = yk; zm = z0; zpim = zpi; zpjm = zpj;
+    xk = xp; xmi = xpmi; yk = yp; ymj = ypmj;
+    if (k + 1 < k1) load(cur, t + e2);
+  }
+}
+
+// planes per thread of the residual march on levels of at least 2^20
+// points in FP64 / FP32 storage and on smaller levels (build-time tunables),
+// and whether it streams f, sbar, r past the cache (see k_residual)
+#ifndef EMMI_MARCH_KC_BIG64
+#define EMMI_MARCH_KC_BIG64 5
+#endif
+#ifndef EMMI_MARCH_KC_BIG32
+#define EMMI_MARCH_KC_BIG32 6
+#endif
+#ifndef EMMI_MARCH_KC_SMALL
+#define EMMI_MARCH_KC_SMALL 3
+#endif
+template <typename T> __host__ __device__ constexpr int march_kc(bool big) {
+  return big ? (sizeof(T) == 8 ? EMMI_MARCH_KC_BIG64 : EMMI_MARCH_KC_BIG32) : EMMI_MARCH_KC_SMALL;
+}
+static inline bool march_big(long nc) { return nc >= (1L << 20); }
+
+// block of a march launch: strip and plane chunk of the XCD-ordered block
+// index; false when the thread's plane point lies beyond the plane
+__device__ __forceinline__ bool march_block(int nstrips, int kc, int n3p1, int e2, int& q, int& k0, int& k1, long& blk) {
+  blk = xcd_block(blockIdx.x, gridDim.x);
+  const int strip = (int)(blk % nstrips), ch = (int)(blk / nstrips);
+  q = strip * 256 + (int)threadIdx.x;
+  k0 = ch * kc;
+  k1 = min(k0 + kc, n3p1);
+  return q < e2;
+}
+
+// r = f - A u for the slots of a batched launch (grid y = slot)
+// (PART: the launch asks for |r|^2 partial sums)
+// one slot's march: the vectors come in as restrict parameters, so that the
+// compiler may order the next plane's loads before this plane's stores
+template <typename T, int KC, bool NTS, bool PART>
+__device__ __forceinline__ double residual_slot(const Level<T>& L, const typename cvec<T>::type* __restrict__ u,
+                                                const typename cvec<T>::type* __restrict__ f,
+                                                typename cvec<T>::type* __restrict__ r, int q, int k0, int k1) {
+  const int nc = (int)L.nc;
+  double acc = 0;
+  auto src = [u](int x) { return u[x]; };
+  auto sink = [&](int t, const cxd& v0, const cxd& v1, const cxd& v2, const cxd&, const cxd&, const cxd&) {
+    st_stream<typename cvec<T>::type, NTS>(r, t, v0.re, v0.im);
+    st_stream<typename cvec<T>::type, NTS>(r, nc + t, v1.re, v1.im);
+    st_stream<typename cvec<T>::type, NTS>(r, 2 * nc + t, v2.re, v2.im);
+    if constexpr (PART)
+      acc += (v0.re * v0.re + v0.im * v0.im) + (v1.re * v1.re + v1.im * v1.im) + (v2.re * v2.re + v2.im * v2.im);
+  };
+  residual_march<T, NTS>(L, q, k0, k1, src, f, sink);
+  return acc;
+}
+
+// r = f - A u for the slots of a batched launch (grid y = slot)
+// (PART: the launch asks for |r|^2 partial sums)
+template <typename T, int KC, bool NTS, bool PART>
 __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, double* partial) {
   const int sl = blockIdx.y;
-  slot_level(L, V, sl);
-  const typename cvec<T>::type* __restrict__ u = V.in0[sl];
-  const typename cvec<T>::type* __restrict__ f = V.in1[sl];
-  typename cvec<T>::type* __restrict__ r = V.out0[sl];
-  const long npts = L.nc;
-  const long blk = xcd_block(blockIdx.x, gridDim.x);
-  const long t = blk * blockDim.x + threadIdx.x;
-  double acc = 0;
-  if (t < npts) {
-    int i, j, k;
-    ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
-    const bool in0 = interior<0>(L.n, i, j, k), in1 = interior<1>(L.n, i, j, k), in2 = interior<2>(L.n, i, j, k);
-    cxd v0 = ld<T>(f, t), v1 = ld<T>(f, L.nc + t), v2 = ld<T>(f, 2 * L.nc + t);
-    if (in0 && in1 && in2) {
-      // all three rows in one block with the stores after them: the rows
-      // share 15 of their 39 stencil values, and with no store in between the
-      // compiler loads each of the 24 distinct values once
-      v0 -= apply_row<T, 0>(L, u, i, j, k, t);
-      v1 -= apply_row<T, 1>(L, u, j, k, i, t);
-      v2 -= apply_row<T, 2>(L, u, k, i, j, t);
-    } else {
-      if (in0) v0 -= apply_row<T, 0>(L, u, i, j, k, t);
-      if (in1) v1 -= apply_row<T, 1>(L, u, j, k, i, t);
-      if (in2) v2 -= apply_row<T, 2>(L, u, k, i, j, t);
-    }
-    st<T>(r, t, v0);
-    st<T>(r, L.nc + t, v1);
-    st<T>(r, 2 * L.nc + t, v2);
-    acc = ((double)v0.re * v0.re + (double)v0.im * v0.im) + ((double)v1.re * v1.re + (double)v1.im * v1.im) +
-          ((double)v2.re * v2.re + (double)v2.im * v2.im);
-  }
-  if (partial) block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
+  slot_geo(L, V, sl);
+  const int nstrips = (int)((L.es[2] + 255) / 256);
+  int q, k0, k1;
+  long blk;
+  const bool own = march_block(nstrips, KC, L.n[2] + 1, (int)L.es[2], q, k0, k1, blk);
+  if (!PART && !own) return;   // no block-wide sum follows
+  const double acc = own ? residual_slot<T, KC, NTS, PART>(L, V.in0[sl], V.in1[sl], V.out0[sl], q, k0, k1) : 0.0;
+  if constexpr (PART) block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
 }
 
 // The convergence test that opens a step of FP64 iterative refinement, for
@@ -107,68 +310,47 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, dou
 // its |r|^2 partial sums go to partial[slot * gridDim.x + block].  The values
 // are those of a residual of the updated FP64 iterate followed by a
 // demotion (k_promote_add, k_residual<double>, k_demote).
-template <bool ADD>
+template <bool ADD, int KC, bool NTS>
+__device__ __forceinline__ double refine_slot(const Level<double>& L, const double2* __restrict__ u,
+                                              const float2* __restrict__ c, const double2* __restrict__ f,
+                                              double2* __restrict__ un, float2* __restrict__ r32, int q, int k0,
+                                              int k1) {
+  const int nc = (int)L.nc;
+  double acc = 0;
+  auto src = [u, c](int x) -> double2 {
+    double2 h = u[x];
+    if constexpr (ADD) {
+      const float2 v = c[x];
+      h.x += v.x;
+      h.y += v.y;
+    }
+    return h;
+  };
+  auto sink = [&](int t, const cxd& v0, const cxd& v1, const cxd& v2, const cxd& ex, const cxd& ey, const cxd& ez) {
+    if constexpr (ADD) {
+      st_stream<double2, NTS>(un, t, ex.re, ex.im);
+      st_stream<double2, NTS>(un, nc + t, ey.re, ey.im);
+      st_stream<double2, NTS>(un, 2 * nc + t, ez.re, ez.im);
+    }
+    st_stream<float2, NTS>(r32, t, v0.re, v0.im);
+    st_stream<float2, NTS>(r32, nc + t, v1.re, v1.im);
+    st_stream<float2, NTS>(r32, 2 * nc + t, v2.re, v2.im);
+    acc += (v0.re * v0.re + v0.im * v0.im) + (v1.re * v1.re + v1.im * v1.im) + (v2.re * v2.re + v2.im * v2.im);
+  };
+  residual_march<double, NTS>(L, q, k0, k1, src, f, sink);
+  return acc;
+}
+
+template <bool ADD, int KC, bool NTS>
 __global__ void __launch_bounds__(256) k_refine_test(Level<double> L, RefineVecs V, double* partial) {
   const int sl = blockIdx.y;
-  {
-    const double* g = V.geo[sl];
-    const long gs = V.gstride;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      L.W[a] = g + (5 * a + 0) * gs;
-      L.IW[a] = g + (5 * a + 1) * gs;
-      L.H[a] = g + (5 * a + 2) * gs;
-      L.X[a] = g + (5 * a + 3) * gs;
-    }
-    L.sbar = V.sbar[sl];
-    L.omu = V.omu[sl];
-  }
-  const double2* __restrict__ u = V.u[sl];
-  const float2* __restrict__ c = V.c[sl];
-  const double2* __restrict__ f = V.f[sl];
-  const long npts = L.nc;
-  const long blk = xcd_block(blockIdx.x, gridDim.x);
-  const long t = blk * blockDim.x + threadIdx.x;
-  double acc = 0;
-  if (t < npts) {
-    auto get = [u, c](long x) -> cxd {
-      if constexpr (ADD) {
-        double2 h = u[x];
-        const float2 v = c[x];
-        h.x += v.x;
-        h.y += v.y;
-        return cxd(h.x, h.y);
-      } else {
-        return ld<double>(u, x);
-      }
-    };
-    int i, j, k;
-    ijk_of(t, L.es[1], L.n[1] + 1, i, j, k);
-    const bool in0 = interior<0>(L.n, i, j, k), in1 = interior<1>(L.n, i, j, k), in2 = interior<2>(L.n, i, j, k);
-    cxd v0 = ld<double>(f, t), v1 = ld<double>(f, L.nc + t), v2 = ld<double>(f, 2 * L.nc + t);
-    if (in0 && in1 && in2) {
-      v0 -= apply_row_get<double, 0>(L, get, i, j, k, t);
-      v1 -= apply_row_get<double, 1>(L, get, j, k, i, t);
-      v2 -= apply_row_get<double, 2>(L, get, k, i, j, t);
-    } else {
-      if (in0) v0 -= apply_row_get<double, 0>(L, get, i, j, k, t);
-      if (in1) v1 -= apply_row_get<double, 1>(L, get, j, k, i, t);
-      if (in2) v2 -= apply_row_get<double, 2>(L, get, k, i, j, t);
-    }
-    if constexpr (ADD) {
-      double2* __restrict__ un = V.unext[sl];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const cxd e = get(q * L.nc + t);
-        un[q * L.nc + t] = make_double2(e.re, e.im);
-      }
-    }
-    float2* __restrict__ r32 = V.r32[sl];
-    r32[t] = make_float2((float)v0.re, (float)v0.im);
-    r32[L.nc + t] = make_float2((float)v1.re, (float)v1.im);
-    r32[2 * L.nc + t] = make_float2((float)v2.re, (float)v2.im);
-    acc = (v0.re * v0.re + v0.im * v0.im) + (v1.re * v1.re + v1.im * v1.im) + (v2.re * v2.re + v2.im * v2.im);
-  }
+  slot_geo(L, V, sl);
+  const int nstrips = (int)((L.es[2] + 255) / 256);
+  int q, k0, k1;
+  long blk;
+  const bool own = march_block(nstrips, KC, L.n[2] + 1, (int)L.es[2], q, k0, k1, blk);
+  const double acc =
+      own ? refine_slot<ADD, KC, NTS>(L, V.u[sl], V.c[sl], V.f[sl], V.unext[sl], V.r32[sl], q, k0, k1) : 0.0;
   block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
 }
 
@@ -585,6 +767,38 @@ static void slot_transfers(SlotVecs<T>& B, const Transfer<T>* Xs, int n) {
   }
 }
 
+// grid of a residual march over a level of shape n (one slot)
+static unsigned march_blocks(const int* n, int kc) {
+  const long e2 = (long)(n[0] + 1) * (n[1] + 1), nc = e2 * (n[2] + 1);
+  if (3 * nc >= (long)INT_MAX) throw std::runtime_error("residual: level too large for 32-bit indices");
+  return (unsigned)(((e2 + 255) / 256) * ((n[2] + 1 + kc - 1) / kc));
+}
+
+template <typename T>
+int residual_blocks(const int* n) {
+  const long nc = (long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+  return (int)march_blocks(n, march_kc<T>(march_big(nc)));
+}
+
+int residual_blocks_max(const int* n) {
+  return (int)march_blocks(n, std::min(EMMI_MARCH_KC_SMALL, std::min(EMMI_MARCH_KC_BIG64, EMMI_MARCH_KC_BIG32)));
+}
+
+template <typename T>
+static void launch_residual_kernel(const Level<T>& L, const SlotVecs<T>& B, int n, double* partial, hipStream_t s) {
+  const bool big = march_big(L.nc);
+  const dim3 g(march_blocks(L.n, march_kc<T>(big)), n), b(256);
+  constexpr int KS = march_kc<T>(false), KB = march_kc<T>(true);
+  constexpr bool NB = sizeof(T) == 8;
+  if (partial) {
+    if (!big) hipLaunchKernelGGL((k_residual<T, KS, false, true>), g, b, 0, s, L, B, partial);
+    else hipLaunchKernelGGL((k_residual<T, KB, NB, true>), g, b, 0, s, L, B, partial);
+  } else {
+    if (!big) hipLaunchKernelGGL((k_residual<T, KS, false, false>), g, b, 0, s, L, B, partial);
+    else hipLaunchKernelGGL((k_residual<T, KB, NB, false>), g, b, 0, s, L, B, partial);
+  }
+}
+
 template <typename T>
 void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,
                            const double* omu, hipStream_t s, double* partial) {
@@ -601,7 +815,7 @@ void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, cons
   const Level<T>& L = Ls[0];
   const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
   timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (3 * sz + sr));
-  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256), n), dim3(256), 0, s, L, B, partial);
+  launch_residual_kernel<T>(L, B, n, partial, s);
 }
 
 void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* c,
@@ -631,9 +845,17 @@ void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, co
   // u, f, sbar read and r32 written; with the update also c read and unext
   // written
   timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (16.0 + 16.0 + 8.0 + 8.0 + (c ? 8.0 + 16.0 : 0.0)));
-  const dim3 g(nblk(L.nc, 256), n), b(256);
-  if (c) hipLaunchKernelGGL(k_refine_test<true>, g, b, 0, s, L, V, partial);
-  else hipLaunchKernelGGL(k_refine_test<false>, g, b, 0, s, L, V, partial);
+  const bool big = march_big(L.nc);
+  const dim3 g(march_blocks(L.n, march_kc<double>(big)), n), b(256);
+  if (big) {
+    constexpr int KC = march_kc<double>(true);
+    if (c) hipLaunchKernelGGL((k_refine_test<true, KC, true>), g, b, 0, s, L, V, partial);
+    else hipLaunchKernelGGL((k_refine_test<false, KC, true>), g, b, 0, s, L, V, partial);
+  } else {
+    constexpr int KC = march_kc<double>(false);
+    if (c) hipLaunchKernelGGL((k_refine_test<true, KC, false>), g, b, 0, s, L, V, partial);
+    else hipLaunchKernelGGL((k_refine_test<false, KC, false>), g, b, 0, s, L, V, partial);
+  }
 }
 
 template <typename T>
@@ -647,10 +869,8 @@ void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, d
   B.omu[0] = L.omu;
   const double sz = sizeof(typename cvec<T>::type), sr = sizeof(T);
   timing::Scope ts(timing::RESIDUAL, s, 3.0 * L.nc * (3 * sz + sr));
-  hipLaunchKernelGGL(k_residual<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, B, partial);
+  launch_residual_kernel<T>(L, B, 1, partial, s);
 }
-
-int residual_blocks(long nc) { return (int)nblk(nc, 256); }
 
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, n, out);
@@ -803,6 +1023,7 @@ void launch_edge_sigma(int n1, int n2, int n3, const double* s11, const double* 
 }
 
 #define EMMI_INSTANTIATE(T)                                                                                    \
+  template int residual_blocks<T>(const int*);                                                                 \
   template void launch_residual<T>(const Level<T>&, const void*, const void*, void*, double*, hipStream_t);   \
   template void launch_residual_batch<T>(const Level<T>*, int, const void* const*, const void* const*,         \
                                          void* const*, const double*, hipStream_t, double*);                   \

@@ -252,10 +252,10 @@ void Solver::init(int n1, int n2, int n3, int nmed, const float* const* x1, cons
     }
     S.facs.assign((size_t)npat_ * lmax_ * 3, LineFactors());
     S.lscr = dalloc(std::max<size_t>(16, lscr_bytes_), owned_);
-    S.partial = (double*)dalloc(sizeof(double) * residual_blocks(nc0_), owned_);
+    S.partial = (double*)dalloc(sizeof(double) * residual_blocks_max(n_), owned_);
   }
   dnorm_ = (double*)dalloc(sizeof(double) * nslots, owned_);
-  part_all_ = (double*)dalloc(sizeof(double) * MG_BATCH_MAX * residual_blocks(nc0_), owned_);
+  part_all_ = (double*)dalloc(sizeof(double) * MG_BATCH_MAX * residual_blocks_max(n_), owned_);
   HIPCHECK(hipHostMalloc((void**)&hnorm_, sizeof(double) * nslots, hipHostMallocDefault));
   HIPCHECK(hipStreamSynchronize(s_));
 }
@@ -645,7 +645,7 @@ int Solver::kernel_rates(int reps, double* ms, double* bytes) {
 // convergence tests
 // ---------------------------------------------------------------------------
 void Solver::norms(const std::vector<int>& which, bool add, std::vector<double>& out) {
-  const int nb = residual_blocks(nc0_);
+  const int nb = residual_blocks<double>(n_);
   for (size_t i0 = 0; i0 < which.size(); i0 += MG_BATCH_MAX) {
     const int n = (int)std::min<size_t>(MG_BATCH_MAX, which.size() - i0);
     const void* u[MG_BATCH_MAX];
