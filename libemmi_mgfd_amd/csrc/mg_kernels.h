@@ -37,13 +37,11 @@ static_assert(sizeof(SlotVecs<double>) + 2 * sizeof(Level<double>) + sizeof(Tran
               "batched launch arguments exceed 4 KiB");
 
 // Vectors of the convergence test of an FP64 refinement step (k_refine_test):
-// per slot the FP64 iterate u, its pending FP32 correction c, the FP64
-// right-hand side f, the updated iterate unext and the FP32 residual r32.
+// per slot the FP64 iterate u, the FP64 right-hand side f and the FP32
+// residual r32.
 struct RefineVecs {
   const double2* u[MG_BATCH_MAX];
-  const float2* c[MG_BATCH_MAX];
   const double2* f[MG_BATCH_MAX];
-  double2* unext[MG_BATCH_MAX];
   float2* r32[MG_BATCH_MAX];
   double omu[MG_BATCH_MAX];
   const double* geo[MG_BATCH_MAX];
@@ -55,15 +53,11 @@ static_assert(sizeof(RefineVecs) + sizeof(Level<double>) + sizeof(double*) <= 40
 
 template <typename T>
 void launch_residual(const Level<T>& L, const void* u, const void* f, void* r, double* partial, hipStream_t s);
-// For n slots of level-0 shape (Ls[k]: slot k's FP64 level 0): with c (a
-// table of pending FP32 corrections) non-null, unext_k = u_k + c_k and
-// r = f_k - A unext_k, else r = f_k - A u_k; r is written demoted to FP32 into
-// r32_k and the |r|^2 partials of slot k go to partial + k residual_blocks<double>(n).
-// One launch.
-void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* c,
-                        const void* const* f, void* const* unext, void* const* r32, double* partial, hipStream_t s);
-// r_k = f_k - A(omu_k) u_k for n slots of levels of one shape (Ls[k]: slot
-// k's level), one launch
+// For n slots of level-0 shape (Ls[k]: slot k's FP64 level 0): r = f_k - A u_k
+// written demoted to FP32 into r32_k, and the |r|^2 partials of slot k at
+// partial + k residual_blocks<double>(n).  One launch.
+void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* f, void* const* r32,
+                        double* partial, hipStream_t s);
 // (partial, when non-null: slot k's |r|^2 partials at partial + k residual_blocks<T>(n))
 template <typename T>
 void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, const void* const* f, void* const* r,

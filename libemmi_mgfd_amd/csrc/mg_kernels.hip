@@ -302,36 +302,18 @@ __global__ void __launch_bounds__(256) k_residual(Level<T> L, SlotVecs<T> V, dou
 }
 
 // The convergence test that opens a step of FP64 iterative refinement, for
-// the slots of one level-0 shape (grid y = slot).  With ADD the step before
-// left its FP32 correction c: every stencil value is read as the updated
-// iterate u + (double)c, and the point's own edges of that sum are written to
-// unext (another vector: the other points still read u and c).  The residual
-// r = f - A u' is demoted into r32, the next FP32 cycle's right-hand side, and
-// its |r|^2 partial sums go to partial[slot * gridDim.x + block].  The values
-// are those of a residual of the updated FP64 iterate followed by a
-// demotion (k_promote_add, k_residual<double>, k_demote).
-template <bool ADD, int KC, bool NTS>
+// the slots of one level-0 shape (grid y = slot): r = f - A u of the FP64
+// iterate, demoted into r32 (the next FP32 cycle's right-hand side), and its
+// |r|^2 partial sums in partial[slot * gridDim.x + block].  The values are
+// those of k_residual<double> followed by a demotion to FP32.
+template <int KC, bool NTS>
 __device__ __forceinline__ double refine_slot(const Level<double>& L, const double2* __restrict__ u,
-                                              const float2* __restrict__ c, const double2* __restrict__ f,
-                                              double2* __restrict__ un, float2* __restrict__ r32, int q, int k0,
-                                              int k1) {
+                                              const double2* __restrict__ f, float2* __restrict__ r32, int q,
+                                              int k0, int k1) {
   const int nc = (int)L.nc;
   double acc = 0;
-  auto src = [u, c](int x) -> double2 {
-    double2 h = u[x];
-    if constexpr (ADD) {
-      const float2 v = c[x];
-      h.x += v.x;
-      h.y += v.y;
-    }
-    return h;
-  };
-  auto sink = [&](int t, const cxd& v0, const cxd& v1, const cxd& v2, const cxd& ex, const cxd& ey, const cxd& ez) {
-    if constexpr (ADD) {
-      st_stream<double2, NTS>(un, t, ex.re, ex.im);
-      st_stream<double2, NTS>(un, nc + t, ey.re, ey.im);
-      st_stream<double2, NTS>(un, 2 * nc + t, ez.re, ez.im);
-    }
+  auto src = [u](int x) { return u[x]; };
+  auto sink = [&](int t, const cxd& v0, const cxd& v1, const cxd& v2, const cxd&, const cxd&, const cxd&) {
     st_stream<float2, NTS>(r32, t, v0.re, v0.im);
     st_stream<float2, NTS>(r32, nc + t, v1.re, v1.im);
     st_stream<float2, NTS>(r32, 2 * nc + t, v2.re, v2.im);
@@ -341,7 +323,7 @@ __device__ __forceinline__ double refine_slot(const Level<double>& L, const doub
   return acc;
 }
 
-template <bool ADD, int KC, bool NTS>
+template <int KC, bool NTS>
 __global__ void __launch_bounds__(256) k_refine_test(Level<double> L, RefineVecs V, double* partial) {
   const int sl = blockIdx.y;
   slot_geo(L, V, sl);
@@ -349,8 +331,7 @@ __global__ void __launch_bounds__(256) k_refine_test(Level<double> L, RefineVecs
   int q, k0, k1;
   long blk;
   const bool own = march_block(nstrips, KC, L.n[2] + 1, (int)L.es[2], q, k0, k1, blk);
-  const double acc =
-      own ? refine_slot<ADD, KC, NTS>(L, V.u[sl], V.c[sl], V.f[sl], V.unext[sl], V.r32[sl], q, k0, k1) : 0.0;
+  const double acc = own ? refine_slot<KC, NTS>(L, V.u[sl], V.f[sl], V.r32[sl], q, k0, k1) : 0.0;
   block_sum_store(acc, partial + (long)sl * gridDim.x, blk);
 }
 
@@ -818,8 +799,8 @@ void launch_residual_batch(const Level<T>* Ls, int n, const void* const* u, cons
   launch_residual_kernel<T>(L, B, n, partial, s);
 }
 
-void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* c,
-                        const void* const* f, void* const* unext, void* const* r32, double* partial, hipStream_t s) {
+void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, const void* const* f, void* const* r32,
+                        double* partial, hipStream_t s) {
   if (n < 1 || n > MG_BATCH_MAX) throw std::runtime_error("refinement test of 1..MG_BATCH_MAX slots");
   if (!partial) throw std::runtime_error("refinement test without partial sums");
   RefineVecs V = {};
@@ -828,34 +809,21 @@ void launch_refine_test(const Level<double>* Ls, int n, const void* const* u, co
     for (int a = 0; a < 3; a++)
       if (Ls[k].n[a] != Ls[0].n[a]) throw std::runtime_error("batched levels differ in shape");
     if (Ls[k].IW[0] - Ls[k].W[0] != V.gstride) throw std::runtime_error("batched levels differ in layout");
-    if (!u[k] || !f[k] || !r32[k] || (c && (!c[k] || !unext[k])))
-      throw std::runtime_error("refinement test: null vector");
+    if (!u[k] || !f[k] || !r32[k]) throw std::runtime_error("refinement test: null vector");
     V.geo[k] = Ls[k].W[0];
     V.sbar[k] = Ls[k].sbar;
     V.omu[k] = Ls[k].omu;
     V.u[k] = (const double2*)u[k];
     V.f[k] = (const double2*)f[k];
     V.r32[k] = (float2*)r32[k];
-    if (c) {
-      V.c[k] = (const float2*)c[k];
-      V.unext[k] = (double2*)unext[k];
-    }
   }
   const Level<double>& L = Ls[0];
-  // u, f, sbar read and r32 written; with the update also c read and unext
-  // written
-  timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (16.0 + 16.0 + 8.0 + 8.0 + (c ? 8.0 + 16.0 : 0.0)));
+  // u, f, sbar read and r32 written
+  timing::Scope ts(timing::RESIDUAL, s, n * 3.0 * L.nc * (16.0 + 16.0 + 8.0 + 8.0));
   const bool big = march_big(L.nc);
   const dim3 g(march_blocks(L.n, march_kc<double>(big)), n), b(256);
-  if (big) {
-    constexpr int KC = march_kc<double>(true);
-    if (c) hipLaunchKernelGGL((k_refine_test<true, KC, true>), g, b, 0, s, L, V, partial);
-    else hipLaunchKernelGGL((k_refine_test<false, KC, true>), g, b, 0, s, L, V, partial);
-  } else {
-    constexpr int KC = march_kc<double>(false);
-    if (c) hipLaunchKernelGGL((k_refine_test<true, KC, false>), g, b, 0, s, L, V, partial);
-    else hipLaunchKernelGGL((k_refine_test<false, KC, false>), g, b, 0, s, L, V, partial);
-  }
+  if (big) hipLaunchKernelGGL((k_refine_test<march_kc<double>(true), true>), g, b, 0, s, L, V, partial);
+  else hipLaunchKernelGGL((k_refine_test<march_kc<double>(false), false>), g, b, 0, s, L, V, partial);
 }
 
 template <typename T>

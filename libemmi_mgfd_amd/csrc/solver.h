@@ -203,8 +203,7 @@ class Solver {
   // ||f64 - A u64|| of the slots `which`, batched.  FP64 cycles leave the
   // residual in r64 (= r[0]).  FP32-storage cycles leave it demoted in f[0]
   // (the next cycle's right-hand side); with `add` each slot's pending FP32
-  // correction u[0] is added to u64 first (into r64's buffer, which then
-  // becomes u64: u64 and r64 swap).
+  // correction u[0] is added to u64 first.
   void norms(const std::vector<int>& which, bool add, std::vector<double>& out);
 
   int n_[3];

@@ -650,7 +650,6 @@ void Solver::norms(const std::vector<int>& which, bool add, std::vector<double>&
     const int n = (int)std::min<size_t>(MG_BATCH_MAX, which.size() - i0);
     const void* u[MG_BATCH_MAX];
     const void* f[MG_BATCH_MAX];
-    const void* c[MG_BATCH_MAX];
     void* r[MG_BATCH_MAX];
     void* r32[MG_BATCH_MAX];
     double omu[MG_BATCH_MAX];
@@ -661,17 +660,12 @@ void Solver::norms(const std::vector<int>& which, bool add, std::vector<double>&
       u[j] = S.u64;
       f[j] = S.f64;
       r[j] = S.r64;
-      c[j] = S.u[0];
       r32[j] = S.f[0];
       omu[j] = S.omu;
+      if (add) launch_promote_add(S.u64, S.u[0], 3 * nc0_, s_);
     }
-    if (p_.precision == 0) {
-      launch_residual_batch<double>(Ls, n, u, f, r, omu, s_, part_all_);
-    } else {
-      launch_refine_test(Ls, n, u, add ? c : nullptr, f, r, r32, part_all_, s_);
-      if (add)
-        for (int j = 0; j < n; j++) std::swap(slots_[which[i0 + j]].u64, slots_[which[i0 + j]].r64);
-    }
+    if (p_.precision == 0) launch_residual_batch<double>(Ls, n, u, f, r, omu, s_, part_all_);
+    else launch_refine_test(Ls, n, u, f, r32, part_all_, s_);
     launch_sum_partials_batch(part_all_, nb, n, dnorm_ + i0, s_);
   }
   HIPCHECK(hipMemcpyAsync(hnorm_, dnorm_, sizeof(double) * which.size(), hipMemcpyDeviceToHost, s_));
