@@ -102,7 +102,6 @@ void launch_blend_dots(const void* r1, const void* rg, long n, double* part, dou
 int blend_blocks(long n);
 // xg = x1 + beta (xg - x1) over n complex doubles
 void launch_blend(void* xg, const void* x1, double br, double bi, long n, hipStream_t s);
-void launch_demote(const void* hi, void* lo, long n, hipStream_t s);
 void launch_promote_add(void* hi, const void* lo, long n, hipStream_t s);
 void launch_sigma_vol0(int n1, int n2, int n3, const double* W0, const double* W1, const double* W2, const float* sig,
                        double* out, hipStream_t s);

@@ -590,11 +590,6 @@ __global__ void k_curl_h(Level<T> L, const typename cvec<T>::type* __restrict__ 
   }
 }
 
-// lo = (float) hi
-__global__ void k_demote(const double2* __restrict__ hi, float2* __restrict__ lo, long n) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t < n) { const double2 v = hi[t]; lo[t] = make_float2((float)v.x, (float)v.y); }
-}
 // hi += (double) lo
 __global__ void k_promote_add(double2* __restrict__ hi, const float2* __restrict__ lo, long n) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -957,9 +952,6 @@ void launch_curl_h(const Level<T>& L, const void* u, void* h, hipStream_t s) {
   hipLaunchKernelGGL(k_curl_h<T>, dim3(nblk(L.nc, 256)), dim3(256), 0, s, L, (const V*)u, (V*)h);
 }
 
-void launch_demote(const void* hi, void* lo, long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_demote, dim3(nblk(n, 256)), dim3(256), 0, s, (const double2*)hi, (float2*)lo, n);
-}
 void launch_promote_add(void* hi, const void* lo, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_promote_add, dim3(nblk(n, 256)), dim3(256), 0, s, (double2*)hi, (const float2*)lo, n);
 }
