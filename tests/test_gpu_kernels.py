@@ -54,6 +54,34 @@ def test_residual(tiny, precision, tol):
     assert rel(out, ref) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-5)])
+def test_residual_large_level(precision, tol):
+    """A level of more than 2^20 points takes the residual kernel's large-level
+    form (five planes per thread with nontemporal streams in FP64, six in
+    FP32); checked against the oracle on a stretched 128 x 128 x 64 grid with
+    an air layer, a sea layer and a layered, anisotropic earth."""
+    from libemmi_mgfd_amd import native
+    n = (128, 128, 64)
+    rng = np.random.default_rng(5)
+    xs = [np.cumsum(np.r_[0.0, rng.uniform(20.0, 80.0, m)]).astype(np.float32) for m in n]
+    s11 = np.empty((n[2], n[1], n[0]), np.float32)
+    s11[:8] = 1e-8                      # air
+    s11[8:20] = 3.3                     # sea
+    s11[20:] = rng.uniform(0.01, 2.0, (n[2] - 20, 1, 1)).astype(np.float32)
+    s33 = (s11 * np.float32(0.5)).astype(np.float32)
+    L0 = O.Level.level0(n, *xs, s11, s11, s33)
+    freq = 0.75
+    S = native.Solver(n, *xs, s11, s11, s33, O.omega_f32(freq), precision=precision)
+    u = random_field(n, 3)
+    f = random_field(n, 4, interior_only=False)
+    ref = L0.residual(O.omega_mu0(freq), u, f)
+    out = S.check_op(0, 0, 0, 0, 0, u, f)
+    err = rel(out, ref)
+    print(f"precision {precision}: residual of a {L0.nvec // 3}-point level, max rel diff {err:.2e}", flush=True)
+    assert err < tol
+    S.close()
+
+
 def conductive_level(s, g, n):
     """Same grid, uniform 1 S/m medium: well-conditioned line blocks."""
     one = np.ones((n[2], n[1], n[0]), np.float32)
