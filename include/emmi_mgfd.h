@@ -341,7 +341,8 @@ int emmi_x0_conv_fwd(const void *x, int Z, int Y, int X, int Cin, const void *w,
                      float slope, const void *act, void *out_bf16, float *out_f32, int cout_store, void *stream);
 /* weight gradient [Cout][27*Cin] (and bias gradient, db may be NULL) of a
  * convolution with input x [V][Cin] and output gradient dy [V][Cout];
- * part: scratch of ceil(Z/zslab)*27*Cin*Cout floats */
+ * part: scratch of ceil(Z/zslab)*27*Cin*Cout floats; Cin and Cout multiples
+ * of 16, Cout at most 256 */
 int emmi_x0_conv_dw(const void *x, const void *dy, int Z, int Y, int X, int Cin, int Cout, float *part, int zslab,
                     float *dw, float *db, void *stream);
 int emmi_x0_prep_weights(const float *master, int Cin, int Cout, void *w, void *wt, void *stream);

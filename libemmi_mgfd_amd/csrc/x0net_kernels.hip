@@ -188,19 +188,43 @@ __global__ void k_reduce_slabs(const float* __restrict__ part, int nslab, long n
   dw[i] = s;
 }
 
-// bias gradient: sum over voxels of dy[v][co]
-__global__ void k_bias_grad(const bf16* __restrict__ dy, long nv, int C, float* __restrict__ db) {
+// bias gradient db[co] = sum over voxels of dy[v][co] (channels fastest), in
+// two passes of fixed order: block b sums the voxels [b chunk, (b+1) chunk)
+// of every channel, 256 / C voxels at a time with consecutive threads on
+// consecutive (voxel, channel) elements, into part[b][co]; then the blocks'
+// sums are added (a fixed tree per channel).  (C <= 256.)
+__global__ void __launch_bounds__(256) k_bias_part(const bf16* __restrict__ dy, long nv, int C, long chunk,
+                                                   float* __restrict__ part) {
   __shared__ float red[256];
-  const int co = blockIdx.x;
+  const int t = threadIdx.x, G = 256 / C, g = t / C, co = t % C;
   float s = 0.f;
-  for (long v = threadIdx.x; v < nv; v += blockDim.x) s += bf2f(dy[v * C + co]);
-  red[threadIdx.x] = s;
+  if (g < G) {
+    const long v0 = (long)blockIdx.x * chunk, v1 = min(v0 + chunk, nv);
+    for (long v = v0 + g; v < v1; v += G) s += bf2f(dy[v * C + co]);
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < C) {
+    float a = 0.f;
+    for (int k = 0; k < G; k++) a += red[k * C + t];
+    part[(long)blockIdx.x * C + t] = a;
+  }
+}
+
+// block co: the nb block sums of channel co, 256 at a time, then a tree
+__global__ void __launch_bounds__(256) k_bias_sum(const float* __restrict__ part, int nb, int C,
+                                                  float* __restrict__ db) {
+  __shared__ float red[256];
+  const int co = blockIdx.x, t = threadIdx.x;
+  float a = 0.f;
+  for (int b = t; b < nb; b += 256) a += part[(long)b * C + co];
+  red[t] = a;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if (t < o) red[t] += red[t + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[co] = red[0];
+  if (t == 0) db[co] = red[0];
 }
 
 // bf16 weights for the forward (w[co][Kpad]) and input-gradient
@@ -567,7 +591,7 @@ int emmi_x0_conv_fwd(const void* x, int Z, int Y, int X, int Cin, const void* w,
 
 int emmi_x0_conv_dw(const void* x, const void* dy, int Z, int Y, int X, int Cin, int Cout, float* part, int zslab,
                     float* dw, float* db, void* stream) {
-  if (Cin % 16 || Cout % 16 || zslab < 1) return -1;
+  if (Cin % 16 || Cout % 16 || Cout > 256 || zslab < 1 || Z < 1 || Y < 1 || X < 1) return -1;
   if ((long)Z * Y * X >= (1L << 31)) return -1;   // the kernel indexes voxels in 32 bits
   const int nslab = (Z + zslab - 1) / zslab;
   const dim3 g(27 * (Cin / 16) * (Cout / 16), nslab), b(256);
@@ -578,7 +602,16 @@ int emmi_x0_conv_dw(const void* x, const void* dy, int Z, int Y, int X, int Cin,
   else hipLaunchKernelGGL(k_conv_dw, g, b, 0, S(stream), (const bf16*)x, (const bf16*)dy, Z, Y, X, Cin, Cout, zslab, part);
   const long n = 27L * Cin * Cout;
   hipLaunchKernelGGL(k_reduce_slabs, dim3(blocks(n, 256)), dim3(256), 0, S(stream), part, nslab, n, dw);
-  if (db) hipLaunchKernelGGL(k_bias_grad, dim3(Cout), dim3(256), 0, S(stream), (const bf16*)dy, (long)Z * Y * X, Cout, db);
+  if (db) {
+    // the bias partials reuse part once the slab reduction has read it (same
+    // stream): nb Cout floats, nb <= nslab 27 Cin
+    const long nv = (long)Z * Y * X;
+    const long nbl = std::min<long>(std::min<long>(1024, (long)nslab * 27 * Cin), nv);
+    const long chunk = (nv + nbl - 1) / nbl;
+    const int nb = (int)((nv + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_bias_part, dim3(nb), dim3(256), 0, S(stream), (const bf16*)dy, nv, Cout, chunk, part);
+    hipLaunchKernelGGL(k_bias_sum, dim3(Cout), dim3(256), 0, S(stream), (const float*)part, nb, Cout, db);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

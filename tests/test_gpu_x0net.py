@@ -215,7 +215,10 @@ def test_unet_gradients_match_oracle(shape):
 
 def test_unet_training_reduces_loss():
     """Adam steps on one sample lower the loss (the whole training loop runs
-    on the project's kernels)."""
+    on the project's kernels).  The rate is 1e-3: at 3e-3 this single-sample
+    loss oscillates (0.40 -> 0.45 -> 0.38 over 60 steps), and whether step 60
+    lands below 0.8 of the start depends on the rounding order of the
+    gradient sums."""
     from libemmi_mgfd_amd import x0net
     shape = (8, 8, 64)
     net = x0net.UNet(shape, seed=1, levels=(16, 32, 48))
@@ -230,7 +233,7 @@ def test_unet_training_reduces_loss():
         net.forward(feat)
         losses.append(net.loss_and_grad(tgt, mask, m.sum()))
         net.backward()
-        net.adam(lr=3e-3)
+        net.adam(lr=1e-3)
     print("losses", losses[0], losses[-1])
     assert losses[-1] < 0.8 * losses[0]
 
