@@ -361,21 +361,26 @@ def main():
     survey = configs.CONFIGS[a.config]()
     gold = os.path.join(ROOT, "tests", "golden", f"{survey.name}.npz")
     gd = np.load(gold) if os.path.exists(gold) else None
-    res = {}
+    res, errors = {}, {}
     for v in a.variants.split(","):
         if VARIANTS[v][2]:
             from libemmi_mgfd_amd import x0guess
             if not os.path.exists(x0guess.WEIGHTS):
                 log(f"bench.py: {x0guess.WEIGHTS} missing, variant {v} skipped")
                 continue
-        res[v] = time_variant(survey, v, a.steps, a.warmup, graphs=not a.no_graphs, autotune=not a.no_autotune)
+        try:
+            res[v] = time_variant(survey, v, a.steps, a.warmup, graphs=not a.no_graphs, autotune=not a.no_autotune)
+        except Exception as e:   # a failed variant is reported, the others still run
+            errors[v] = f"{type(e).__name__}: {e}"
+            log(f"bench.py: variant {v} failed: {errors[v]}")
+            continue
         log(v, {k: res[v][k] for k in ("solves_per_s", "ms_per_step", "cycles", "status", "rel_res")})
     # receiver data of the signal channels (inline Ex, Hy; inline Ey and Hx
     # vanish by symmetry) against the reference's converged FP64 solve
     # (tol 1e-10); the reference's own solve at the bench tolerance is given
     # for scale
     rec = {}
-    if gd is not None and "dcal_tight" in gd.files:
+    if res and gd is not None and "dcal_tight" in gd.files:
         shp = next(iter(res.values()))["dcal"].shape
         dref = gd["dcal_tight"].reshape(shp)
         rec = {k: receiver_errors(survey, v["dcal"], dref) for k, v in res.items()}
@@ -429,6 +434,8 @@ def main():
                                      for c, t in v["kernels"].items() if t[1]}}
                      for k, v in res.items()},
     }
+    if errors:
+        out["variant_errors"] = errors
     if not a.no_autotune:
         tab = native.line_tuning()
         out["line_autotune"] = {
@@ -449,21 +456,30 @@ def main():
         out["receiver_rel_err_vs_converged_reference"] = rec
         out["reference_at_tol_1e-6_vs_converged_reference"] = receiver_errors(
             survey, gd["dcal"].reshape(shp), gd["dcal_tight"].reshape(shp))
+    def leg(key, fn):
+        """An auxiliary leg; its failure is reported under its key and does
+        not cost the headline line."""
+        try:
+            out[key] = fn()
+        except Exception as e:
+            out[key] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"bench.py: leg {key} failed: {out[key]['error']}")
+
     if not a.no_kernels:
-        out["plain_kernels"] = {"level": "128^3 level 0, complex FP64", **kernel_rates(survey, 0)}
-        out["mixed_kernels"] = {"level": "128^3 level 0, FP32 storage", **kernel_rates(survey, 1)}
+        leg("plain_kernels", lambda: {"level": "128^3 level 0, complex FP64", **kernel_rates(survey, 0)})
+        leg("mixed_kernels", lambda: {"level": "128^3 level 0, FP32 storage", **kernel_rates(survey, 1)})
         # the initial-guess network's BF16 MFMA convolutions, one training
         # step (forward, input and weight gradients) on the 128^3 grid
         from libemmi_mgfd_amd import x0net
-        out["x0net_conv"] = x0net.conv_rates((survey.n3, survey.n2, survey.n1))
+        leg("x0net_conv", lambda: x0net.conv_rates((survey.n3, survey.n2, survey.n1)))
     if not a.no_survey:
-        out["survey"] = survey_leg(autotune=not a.no_autotune)
-        log("survey", out["survey"]["solves_per_s"])
+        leg("survey", lambda: survey_leg(autotune=not a.no_autotune))
+        log("survey", out["survey"].get("solves_per_s"))
     if not a.no_gradient:
-        out["gradient"] = gradient_leg()
-        log("gradient", out["gradient"]["seconds_per_gradient"])
+        leg("gradient", gradient_leg)
+        log("gradient", out["gradient"].get("seconds_per_gradient"))
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(survey, a.cpu_cycles)
+        leg("cpu_baseline", lambda: cpu_baseline(survey, a.cpu_cycles))
     print(json.dumps(out))
 
 
