@@ -91,7 +91,7 @@ struct Slot {
   std::vector<void*> u, f, r;        // per level, working precision
   void* u64 = nullptr;               // level-0 FP64 state (aliases u/f/r[0] in FP64 mode)
   void* f64 = nullptr;
-  void* r64 = nullptr;
+  void* r64 = nullptr;               // FP64 mode: r[0]; FP32-storage cycles: blend_guess scratch (lazy)
   std::vector<LineFactors> facs;     // [(pattern*lmax + level)*3 + axis]; level 0 under pattern 0
   void* lscr = nullptr;              // right-hand side / substitution records of the line solves
   double* partial = nullptr;         // residual-norm partial sums

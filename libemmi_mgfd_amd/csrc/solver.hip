@@ -248,7 +248,7 @@ void Solver::init(int n1, int n2, int n3, int nmed, const float* const* x1, cons
       const size_t b = sizeof(double2) * 3 * (size_t)nc0_;
       S.u64 = dalloc(b, owned_);
       S.f64 = dalloc(b, owned_);
-      S.r64 = dalloc(b, owned_);
+      S.r64 = nullptr;   // scratch of blend_guess only, allocated there on first use
     }
     S.facs.assign((size_t)npat_ * lmax_ * 3, LineFactors());
     S.lscr = dalloc(std::max<size_t>(16, lscr_bytes_), owned_);
@@ -560,6 +560,7 @@ void Solver::blend_guess(int n, const void* const* b, const void* const* x1, voi
   double *part = blend_part_, *dots = blend_dots_;
   for (int k = 0; k < n; k++) {
     Slot& S = slots_[k];
+    if (!S.r64) S.r64 = dalloc(sizeof(double2) * nv, owned_);
     const DevLevel<double>& L0 = meds_[S.med].l0_64;
     launch_scale_rhs<double>(L0.view(S.omu), L0.D(0), L0.D(1), L0.D(2), b[k], S.f64, s_);
     // residuals of the iterate (r64) and of the guess (u64, free before a solve)
