@@ -125,9 +125,20 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const bf16* __restrict__ x, in
 }
 
 // dw partials: block = 4 waves, one (tap, 16-channel input tile, 16-channel
-// output tile) per block, voxels of slab blockIdx.y split over the waves.
+// output tile) per block, the voxels of slab blockIdx.y in tiles of 128.
+// A tile of dy (16 output channels) and of x at the tap's offset (16 input
+// channels, zero outside the grid) is loaded with 16-byte loads (a thread: 8
+// channels of one voxel) and stored channel-major in LDS, so that a lane's
+// 8 consecutive voxels of one channel -- its share of an MFMA operand -- are
+// one aligned 16-byte LDS read.  Rows are padded to 136 values so the 16
+// channel rows start in different banks.  The next tile's loads are in
+// flight while the current one is multiplied.  Wave w takes voxels
+// [32 w, 32 w + 32) of each tile.
+constexpr int DW_TV = 128, DW_ROW = DW_TV + 8;
 __global__ void __launch_bounds__(256) k_conv_dw(const bf16* __restrict__ x, const bf16* __restrict__ dy, int Z, int Y,
                                                  int X, int Cin, int Cout, int zslab, float* __restrict__ part) {
+  __shared__ __align__(16) bf16 As[16 * DW_ROW];
+  __shared__ __align__(16) bf16 Bs[16 * DW_ROW];
   __shared__ float red[4][64][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nci = Cin / 16, nco = Cout / 16;
@@ -138,33 +149,44 @@ __global__ void __launch_bounds__(256) k_conv_dw(const bf16* __restrict__ x, con
   const int z0 = blockIdx.y * zslab, z1 = min(Z, z0 + zslab);
   // voxels of a slab fit 32-bit indices (the launcher checks the volume)
   const int nv = (z1 - z0) * Y * X;
+  const int base = z0 * Y * X;   // flat index of the slab's first voxel
+  // staging role: voxel vi of the tile, channels 8 h .. 8 h + 7
+  const int vi = threadIdx.x >> 1, h = threadIdx.x & 1;
+  auto load = [&](int v0, uint4& ra, uint4& rb) {
+    ra = make_uint4(0, 0, 0, 0);
+    rb = make_uint4(0, 0, 0, 0);
+    const int v = v0 + vi;
+    if (v < nv) {
+      const int g = base + v;
+      ra = *reinterpret_cast<const uint4*>(dy + (long)g * Cout + co0 + 8 * h);
+      const int xx = g % X, q = g / X, yy = q % Y, zz = q / Y;
+      const int sz = zz + dz, sy = yy + dyy, sx = xx + dx;
+      if (sz >= 0 && sz < Z && sy >= 0 && sy < Y && sx >= 0 && sx < X)
+        rb = *reinterpret_cast<const uint4*>(x + ((long)(sz * Y + sy) * X + sx) * Cin + ci0 + 8 * h);
+    }
+  };
+  auto put = [&](const uint4& ra, const uint4& rb) {
+    const bf16* pa = reinterpret_cast<const bf16*>(&ra);
+    const bf16* pb = reinterpret_cast<const bf16*>(&rb);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      As[(8 * h + c) * DW_ROW + vi] = pa[c];
+      Bs[(8 * h + c) * DW_ROW + vi] = pb[c];
+    }
+  };
   const int col = lane & 15, kq = (lane >> 4) * 8;
   f32x4 acc = {0, 0, 0, 0};
-  for (int v0 = wave * 32; v0 < nv; v0 += 128) {
-    bf16x8 a, b;
-    // coordinates of the lane's first voxel, then stepped along x with carry
-    // (one division per 8 elements instead of three 64-bit ones per element)
-    int v = v0 + kq;
-    int xx = v % X, q = v / X;
-    int yy = q % Y, zz = z0 + q / Y;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      float av = 0.f, bv = 0.f;
-      if (v + j < nv) {
-        const int g = (zz * Y + yy) * X + xx;
-        av = bf2f(dy[(long)g * Cout + co0 + col]);
-        const int sz = zz + dz, sy = yy + dyy, sx = xx + dx;
-        if (sz >= 0 && sz < Z && sy >= 0 && sy < Y && sx >= 0 && sx < X)
-          bv = bf2f(x[((long)(sz * Y + sy) * X + sx) * Cin + ci0 + col]);
-      }
-      a[j] = (__bf16)av;
-      b[j] = (__bf16)bv;
-      if (++xx == X) {
-        xx = 0;
-        if (++yy == Y) { yy = 0; ++zz; }
-      }
-    }
+  uint4 ra, rb;
+  load(0, ra, rb);
+  for (int v0 = 0; v0 < nv; v0 += DW_TV) {
+    __syncthreads();   // the previous tile's operands are read
+    put(ra, rb);
+    __syncthreads();
+    if (v0 + DW_TV < nv) load(v0 + DW_TV, ra, rb);
     // A = dy^T (rows co, cols voxels), B = x_tap (rows voxels, cols ci)
+    const int o = col * DW_ROW + 32 * wave + kq;
+    const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&As[o]));
+    const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&Bs[o]));
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   }
 #pragma unroll
