@@ -1291,9 +1291,25 @@ __device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restri
 #ifndef EMMI_QUAD_LDS_KB
 #define EMMI_QUAD_LDS_KB 80
 #endif
+// Lines per block of the fused quad-lane launches: 16, or 8 in a single
+// wave from line length EMMI_LINE_WAVE_MIN_NT on (build-time tunable, off by
+// default).  The 64- and 128-node FP32 instances hold 188 and 314 registers
+// per lane and 40 / 80 KiB of records per 16-line block, so one or two
+// blocks fit a CU and two of its four waves only help with the right-hand
+// sides.  Single-wave blocks of 8 lines fit four to eight to a CU, but the
+// one wave then builds all its records and runs both halves' loops under
+// divergent masks: measured at 128^3 it won on a few shapes (FP64 128-node
+// lines, one right-hand side: 143 -> 125 us; FP32 64-node x lines, three:
+// 77 -> 70 us) and lost on most (FP32 128-node y/z lines, three right-hand
+// sides: 257 -> 357 us; FP32 64-node lines, one: 36 -> 62 us).
+#ifndef EMMI_LINE_WAVE_MIN_NT
+#define EMMI_LINE_WAVE_MIN_NT 1000000
+#endif
+__host__ __device__ constexpr int quad_lpb(int nt) { return nt >= EMMI_LINE_WAVE_MIN_NT ? 8 : 16; }
+
 template <typename T>
-__host__ __device__ constexpr bool quad_lds_records(int nt) {
-  return nt > 0 && 16 * nt * SREC * (int)sizeof(ScrRec<T>) <= EMMI_QUAD_LDS_KB * 1024;
+__host__ __device__ constexpr bool quad_lds_records(int nt, int lpb = 16) {
+  return nt > 0 && lpb * nt * SREC * (int)sizeof(ScrRec<T>) <= EMMI_QUAD_LDS_KB * 1024;
 }
 
 // NT > 0: every job of the launch has lines of NT nodes; up to 64 nodes the
@@ -1302,17 +1318,23 @@ __host__ __device__ constexpr bool quad_lds_records(int nt) {
 // the ring in place (see k_line_solve); the unrolled passes also drop the
 // loop overhead of these short chains.
 //
-// With RHS the block first computes the records of its own 16 lines (the
+// With RHS the block first computes the records of its own LPB lines (the
 // work of k_line_rhs, for launches too small to pay for a launch of their
-// own) with all its BT threads, and then its first two waves substitute;
-// the other waves only help with the right-hand sides.
-template <typename T, int A, int D, bool RHS, int BT, int NT>
+// own) with all its BT threads, and then substitutes.  LPB = 16: a wave per
+// line half side (top halves in wave 0, bottom halves in wave 1), the other
+// waves only help with the right-hand sides, the halves meet through LDS.
+// LPB = 8 (BT = 64): one wave holds both halves of 8 lines (lanes 0-31 the
+// top halves, 32-63 the bottom ones, each side under its own exec mask), so
+// a block needs half the LDS and no second wave, and the halves meet
+// through a cross-lane shuffle.
+template <typename T, int A, int D, bool RHS, int BT, int NT, int LPB = 16>
 __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
+  static_assert(LPB == 16 || (LPB == 8 && BT == 64), "lines per block: 16, or 8 in a single wave");
   using E = typename cvec<T>::type;
   using RS = ScrRec<T>;
   using PS = typename cvec<RS>::type;
   constexpr int FC = fchunks<double>(), SC = schunks<RS>(), VS = Ch<RS>::V;
-  __shared__ double xch[2][2][64];
+  __shared__ double xch[LPB == 16 ? 2 : 1][2][LPB == 16 ? 64 : 1];
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int N = NT > 0 ? NT : L.n[A];
@@ -1322,19 +1344,20 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   // LR: the block's line records live in LDS (lrec[node][line][value]) and
   // never go to memory, which takes a store and a dependent load off each end
   // of the substitution
-  constexpr bool LR = RHS && quad_lds_records<T>(NT);
-  __shared__ RS lrec[LR ? 16 * NT * SREC : 1];
+  constexpr bool LR = RHS && quad_lds_records<T>(NT, LPB);
+  __shared__ RS lrec[LR ? LPB * NT * SREC : 1];
   if (RHS) {
     const long nl = (long)J.nlo * J.nhi;
-    for (int idx = threadIdx.x; idx < 16 * N; idx += blockDim.x) {
-      const int ln = blockIdx.x * 16 + (idx & 15), p = idx >> 4;
+    for (int idx = threadIdx.x; idx < LPB * N; idx += blockDim.x) {
+      const int ll = idx % LPB, p = idx / LPB;
+      const int ln = blockIdx.x * LPB + ll;
       int b2, c2;
       if (line_of<T, A>(J, ln, b2, c2)) {
         if constexpr (LR) {
           Ch<RS> c[SC] = {};
           node_rhs_rec<T, A>(J, b2, c2, p, c);
 #pragma unroll
-          for (int v = 0; v < SREC; v++) lrec[(p * 16 + (idx & 15)) * SREC + v] = (RS)val(c, v);
+          for (int v = 0; v < SREC; v++) lrec[(p * LPB + ll) * SREC + v] = (RS)val(c, v);
         } else {
           node_rhs<T, A>(J, ln, b2, c2, p, nl);
         }
@@ -1342,9 +1365,10 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
     __syncthreads();   // the block's records are complete (workgroup scope)
   }
-  const int lane = threadIdx.x & 63, side = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, side = LPB == 16 ? (int)threadIdx.x >> 6 : lane >> 5;
   const int q = lane & 3;
-  const int line = blockIdx.x * 16 + (lane >> 2);
+  const int lloc = (lane & (4 * LPB - 1)) >> 2;   // the lane's line within the block
+  const int line = blockIdx.x * LPB + lloc;
   int pb = 0, pc = 0;
   const bool live = side < 2 && line_of<T, A>(J, line, pb, pc);
   LineGeo<T, A> G;
@@ -1364,17 +1388,16 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   const int sel = (2 * q) % VS;
   const long off_q = q == 0 ? G.ob0 : q == 1 ? G.ob1 : q == 2 ? G.oc0 : G.oc1;
   // component q of sigma (z once written) and rho of node p of this lane's line
-  const int lloc = lane >> 2;
   auto rec_s = [&](int p) -> PS {
-    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * 16 + lloc) * SREC + 2 * q]);
+    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * LPB + lloc) * SREC + 2 * q]);
     else return *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + p * sstep + soff) + sel);
   };
   auto rec_rho = [&](int p) -> PS {
-    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * 16 + lloc) * SREC + 8]);
+    if constexpr (LR) return *reinterpret_cast<const PS*>(&lrec[(p * LPB + lloc) * SREC + 8]);
     else return *reinterpret_cast<const PS*>(reinterpret_cast<const RS*>(scr + p * sstep + (8 / VS) * stride) + 8 % VS);
   };
   auto put_z = [&](int p, PS z) {
-    if constexpr (LR) *reinterpret_cast<PS*>(&lrec[(p * 16 + lloc) * SREC + 2 * q]) = z;
+    if constexpr (LR) *reinterpret_cast<PS*>(&lrec[(p * LPB + lloc) * SREC + 2 * q]) = z;
     else *reinterpret_cast<PS*>(reinterpret_cast<RS*>(scr + p * sstep + soff) + sel) = z;
   };
   auto qnode = [&](QNode<RS>& n, int p) {
@@ -1509,11 +1532,16 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   }
 
   // ---------------- meeting node ----------------
-  if (side < 2) {
-    xch[side][0][lane] = tm.re;
-    xch[side][1][lane] = tm.im;
+  cxd tpeer(0, 0);   // LPB = 8: the other side's coupling term, same line and component
+  if constexpr (LPB == 16) {
+    if (side < 2) {
+      xch[side][0][lane] = tm.re;
+      xch[side][1][lane] = tm.im;
+    }
+    __syncthreads();
+  } else {
+    tpeer = cxd(__shfl_xor(tm.re, 32), __shfl_xor(tm.im, 32));
   }
-  __syncthreads();
   if (!live) return;
   cxd xm, qmid, rdmid;
   {
@@ -1527,8 +1555,14 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     qmid = cxd((double)nd.rho.x, (double)nd.rho.y) * rdmid;
     const cxd rdm(c.v[0], c.v[1]);
     const cxd dq = cxd((double)r.x, (double)r.y) * rdm - qmid;
-    const cxd ttop(xch[0][0][lane], xch[0][1][lane]);
-    const cxd tbot(xch[1][0][lane], xch[1][1][lane]);
+    cxd ttop, tbot;
+    if constexpr (LPB == 16) {
+      ttop = cxd(xch[0][0][lane], xch[0][1][lane]);
+      tbot = cxd(xch[1][0][lane], xch[1][1][lane]);
+    } else {
+      ttop = side == 0 ? tm : tpeer;
+      tbot = side == 0 ? tpeer : tm;
+    }
     const cxd g = ((cxd((double)nd.s.x, (double)nd.s.y) + dq * vq) - ttop) - tbot;
     xm = qsymv(w, g);
   }
@@ -1760,18 +1794,18 @@ static void launch_timed(K kernel, dim3 g, dim3 b, hipStream_t s, int cls, Bytes
 }
 
 template <typename T, int NT>
-static void launch_quad_fused(const LinePack<T>& P, int axis, dim3 g, hipStream_t s, Bytes by) {
-  // lines of 64 and more nodes come thousands to a colour: blocks of 256
-  // threads let two blocks share a CU (registers, LDS), so the launch needs
-  // fewer rounds of blocks
+static void launch_quad_fused(const LinePack<T>& P, int axis, int maxl, hipStream_t s, Bytes by) {
   // the 128-node lines keep their loop (not unrolled): a deeper ring pays
   // there in FP32 cycles (in FP64 it spills)
   constexpr int QD = NT == 128 && sizeof(T) == 4 ? 5 : EMMI_LINE_QUAD_RING;
-  constexpr int FT = NT >= 64 ? EMMI_LINE_FUSE_THREADS_LONG : EMMI_LINE_FUSE_THREADS;
-  const dim3 b(FT);
-  if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
-  else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
-  else launch_timed(k_line_solve_q<T, 2, QD, true, FT, NT>, g, b, s, timing::LINE, by, P);
+  // lines of 64 and more nodes (thousands to a colour): blocks of one wave
+  // holding both halves of 8 lines; see quad_lpb
+  constexpr int LPB = quad_lpb(NT);
+  constexpr int FT = LPB == 8 ? 64 : (NT >= 64 ? EMMI_LINE_FUSE_THREADS_LONG : EMMI_LINE_FUSE_THREADS);
+  const dim3 g((unsigned)((maxl + LPB - 1) / LPB), P.njobs), b(FT);
+  if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT, LPB>, g, b, s, timing::LINE, by, P);
+  else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT, LPB>, g, b, s, timing::LINE, by, P);
+  else launch_timed(k_line_solve_q<T, 2, QD, true, FT, NT, LPB>, g, b, s, timing::LINE, by, P);
 }
 
 template <typename T>
@@ -1862,15 +1896,15 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
         if (P.job[j].L.n[axis] != nt) nt = 0;
       if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64 && nt != 128) nt = 0;
       // records kept in LDS are neither written to nor read from memory
-      const Bytes by{comp, rhs_bytes + solve_bytes - (quad_lds_records<T>(nt) ? 2.0 * rec_bytes : 0.0)};
+      const Bytes by{comp, rhs_bytes + solve_bytes - (quad_lds_records<T>(nt, quad_lpb(nt)) ? 2.0 * rec_bytes : 0.0)};
       switch (nt) {
-        case 4: launch_quad_fused<T, 4>(P, axis, g, s, by); break;
-        case 8: launch_quad_fused<T, 8>(P, axis, g, s, by); break;
-        case 16: launch_quad_fused<T, 16>(P, axis, g, s, by); break;
-        case 32: launch_quad_fused<T, 32>(P, axis, g, s, by); break;
-        case 64: launch_quad_fused<T, 64>(P, axis, g, s, by); break;
-        case 128: launch_quad_fused<T, 128>(P, axis, g, s, by); break;
-        default: launch_quad_fused<T, 0>(P, axis, g, s, by); break;
+        case 4: launch_quad_fused<T, 4>(P, axis, maxl, s, by); break;
+        case 8: launch_quad_fused<T, 8>(P, axis, maxl, s, by); break;
+        case 16: launch_quad_fused<T, 16>(P, axis, maxl, s, by); break;
+        case 32: launch_quad_fused<T, 32>(P, axis, maxl, s, by); break;
+        case 64: launch_quad_fused<T, 64>(P, axis, maxl, s, by); break;
+        case 128: launch_quad_fused<T, 128>(P, axis, maxl, s, by); break;
+        default: launch_quad_fused<T, 0>(P, axis, maxl, s, by); break;
       }
     } else {
       const dim3 b(128);
