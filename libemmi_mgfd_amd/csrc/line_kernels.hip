@@ -1291,21 +1291,42 @@ __device__ __forceinline__ void qload_f(QNode<RS>& n, const Ch<double>* __restri
 #ifndef EMMI_QUAD_LDS_KB
 #define EMMI_QUAD_LDS_KB 80
 #endif
-// Lines per block of the fused quad-lane launches: 16, or 8 in a single
-// wave from line length EMMI_LINE_WAVE_MIN_NT on (build-time tunable, off by
-// default).  The 64- and 128-node FP32 instances hold 188 and 314 registers
-// per lane and 40 / 80 KiB of records per 16-line block, so one or two
-// blocks fit a CU and two of its four waves only help with the right-hand
-// sides.  Single-wave blocks of 8 lines fit four to eight to a CU, but the
-// one wave then builds all its records and runs both halves' loops under
-// divergent masks: measured at 128^3 it won on a few shapes (FP64 128-node
-// lines, one right-hand side: 143 -> 125 us; FP32 64-node x lines, three:
-// 77 -> 70 us) and lost on most (FP32 128-node y/z lines, three right-hand
-// sides: 257 -> 357 us; FP32 64-node lines, one: 36 -> 62 us).
+// Block shape of the fused quad-lane launches (lines per block, threads;
+// build-time tunables).  The FP32 128-node instance holds 314 registers per
+// lane (one wave per SIMD) and, with 16 lines, 80 KiB of records plus the
+// 2 KiB meeting exchange in LDS: one 256-thread block per CU, whose two
+// further waves only help with the right-hand sides.  15 lines in
+// 128-thread blocks (75 + 2 KiB) let two blocks and four substituting waves
+// share a CU, and measured slower at 128^3 (three right-hand sides:
+// 257 -> 305 us per y/z launch; one: 78 -> 128 us): the right-hand sides,
+// built by two waves instead of four, cost more than the second block gains.
+// Single-wave blocks of 8 lines from line length EMMI_LINE_WAVE_MIN_NT on
+// (off by default) fit four to eight to a CU, but the one wave builds all
+// its records and runs both halves' loops under divergent masks: measured at
+// 128^3 they won on a few shapes (FP64 128-node lines, one right-hand side:
+// 143 -> 125 us; FP32 64-node x lines, three: 77 -> 70 us) and lost on most
+// (FP32 128-node y/z lines, three: 257 -> 357 us; FP32 64-node lines, one:
+// 36 -> 62 us).
 #ifndef EMMI_LINE_WAVE_MIN_NT
 #define EMMI_LINE_WAVE_MIN_NT 1000000
 #endif
-__host__ __device__ constexpr int quad_lpb(int nt) { return nt >= EMMI_LINE_WAVE_MIN_NT ? 8 : 16; }
+#ifndef EMMI_LINE_LPB_F32_128
+#define EMMI_LINE_LPB_F32_128 16
+#endif
+#ifndef EMMI_LINE_BT_F32_128
+#define EMMI_LINE_BT_F32_128 EMMI_LINE_FUSE_THREADS_LONG
+#endif
+template <typename T>
+__host__ __device__ constexpr int quad_lpb(int nt) {
+  return nt >= EMMI_LINE_WAVE_MIN_NT ? 8 : (sizeof(T) == 4 && nt == 128 ? EMMI_LINE_LPB_F32_128 : 16);
+}
+template <typename T>
+__host__ __device__ constexpr int quad_bt(int nt) {
+  return nt >= EMMI_LINE_WAVE_MIN_NT                 ? 64
+         : sizeof(T) == 4 && nt == 128               ? EMMI_LINE_BT_F32_128
+         : nt >= 64                                  ? EMMI_LINE_FUSE_THREADS_LONG
+                                                     : EMMI_LINE_FUSE_THREADS;
+}
 
 template <typename T>
 __host__ __device__ constexpr bool quad_lds_records(int nt, int lpb = 16) {
@@ -1320,21 +1341,22 @@ __host__ __device__ constexpr bool quad_lds_records(int nt, int lpb = 16) {
 //
 // With RHS the block first computes the records of its own LPB lines (the
 // work of k_line_rhs, for launches too small to pay for a launch of their
-// own) with all its BT threads, and then substitutes.  LPB = 16: a wave per
-// line half side (top halves in wave 0, bottom halves in wave 1), the other
-// waves only help with the right-hand sides, the halves meet through LDS.
-// LPB = 8 (BT = 64): one wave holds both halves of 8 lines (lanes 0-31 the
-// top halves, 32-63 the bottom ones, each side under its own exec mask), so
-// a block needs half the LDS and no second wave, and the halves meet
-// through a cross-lane shuffle.
+// own) with all its BT threads, and then substitutes.  BT >= 128: a wave per
+// line half side (top halves of the block's LPB <= 16 lines in wave 0,
+// bottom halves in wave 1; four lanes per line half, lanes beyond 4 LPB
+// idle), further waves only help with the right-hand sides, the halves meet
+// through LDS.  BT = 64 (LPB <= 8): one wave holds both halves (lanes 0-31
+// the top halves, 32-63 the bottom ones, each side under its own exec
+// mask) and the halves meet through a cross-lane shuffle.
 template <typename T, int A, int D, bool RHS, int BT, int NT, int LPB = 16>
 __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
-  static_assert(LPB == 16 || (LPB == 8 && BT == 64), "lines per block: 16, or 8 in a single wave");
+  constexpr bool ONEW = BT == 64;   // both sides in one wave
+  static_assert(ONEW ? LPB <= 8 : (LPB <= 16 && BT >= 128), "lines per block");
   using E = typename cvec<T>::type;
   using RS = ScrRec<T>;
   using PS = typename cvec<RS>::type;
   constexpr int FC = fchunks<double>(), SC = schunks<RS>(), VS = Ch<RS>::V;
-  __shared__ double xch[LPB == 16 ? 2 : 1][2][LPB == 16 ? 64 : 1];
+  __shared__ double xch[ONEW ? 1 : 2][2][ONEW ? 1 : 64];
   const LineJob<T>& J = P.job[blockIdx.y];
   const Level<T>& L = J.L;
   const int N = NT > 0 ? NT : L.n[A];
@@ -1365,12 +1387,12 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     }
     __syncthreads();   // the block's records are complete (workgroup scope)
   }
-  const int lane = threadIdx.x & 63, side = LPB == 16 ? (int)threadIdx.x >> 6 : lane >> 5;
+  const int lane = threadIdx.x & 63, side = ONEW ? lane >> 5 : (int)threadIdx.x >> 6;
   const int q = lane & 3;
-  const int lloc = (lane & (4 * LPB - 1)) >> 2;   // the lane's line within the block
+  const int lloc = ONEW ? (lane & 31) >> 2 : lane >> 2;   // the lane's line within the block
   const int line = blockIdx.x * LPB + lloc;
   int pb = 0, pc = 0;
-  const bool live = side < 2 && line_of<T, A>(J, line, pb, pc);
+  const bool live = side < 2 && lloc < LPB && line_of<T, A>(J, line, pb, pc);
   LineGeo<T, A> G;
   if (live) G.init(L, pb, pc);
   const long stride = (long)J.nlo * J.nhi;
@@ -1532,8 +1554,8 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   }
 
   // ---------------- meeting node ----------------
-  cxd tpeer(0, 0);   // LPB = 8: the other side's coupling term, same line and component
-  if constexpr (LPB == 16) {
+  cxd tpeer(0, 0);   // one wave: the other side's coupling term, same line and component
+  if constexpr (!ONEW) {
     if (side < 2) {
       xch[side][0][lane] = tm.re;
       xch[side][1][lane] = tm.im;
@@ -1556,7 +1578,7 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
     const cxd rdm(c.v[0], c.v[1]);
     const cxd dq = cxd((double)r.x, (double)r.y) * rdm - qmid;
     cxd ttop, tbot;
-    if constexpr (LPB == 16) {
+    if constexpr (!ONEW) {
       ttop = cxd(xch[0][0][lane], xch[0][1][lane]);
       tbot = cxd(xch[1][0][lane], xch[1][1][lane]);
     } else {
@@ -1798,10 +1820,9 @@ static void launch_quad_fused(const LinePack<T>& P, int axis, int maxl, hipStrea
   // the 128-node lines keep their loop (not unrolled): a deeper ring pays
   // there in FP32 cycles (in FP64 it spills)
   constexpr int QD = NT == 128 && sizeof(T) == 4 ? 5 : EMMI_LINE_QUAD_RING;
-  // lines of 64 and more nodes (thousands to a colour): blocks of one wave
-  // holding both halves of 8 lines; see quad_lpb
-  constexpr int LPB = quad_lpb(NT);
-  constexpr int FT = LPB == 8 ? 64 : (NT >= 64 ? EMMI_LINE_FUSE_THREADS_LONG : EMMI_LINE_FUSE_THREADS);
+  // block shape: see quad_lpb / quad_bt
+  constexpr int LPB = quad_lpb<T>(NT);
+  constexpr int FT = quad_bt<T>(NT);
   const dim3 g((unsigned)((maxl + LPB - 1) / LPB), P.njobs), b(FT);
   if (axis == 0) launch_timed(k_line_solve_q<T, 0, QD, true, FT, NT, LPB>, g, b, s, timing::LINE, by, P);
   else if (axis == 1) launch_timed(k_line_solve_q<T, 1, QD, true, FT, NT, LPB>, g, b, s, timing::LINE, by, P);
@@ -1896,7 +1917,7 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
         if (P.job[j].L.n[axis] != nt) nt = 0;
       if (nt != 4 && nt != 8 && nt != 16 && nt != 32 && nt != 64 && nt != 128) nt = 0;
       // records kept in LDS are neither written to nor read from memory
-      const Bytes by{comp, rhs_bytes + solve_bytes - (quad_lds_records<T>(nt, quad_lpb(nt)) ? 2.0 * rec_bytes : 0.0)};
+      const Bytes by{comp, rhs_bytes + solve_bytes - (quad_lds_records<T>(nt, quad_lpb<T>(nt)) ? 2.0 * rec_bytes : 0.0)};
       switch (nt) {
         case 4: launch_quad_fused<T, 4>(P, axis, maxl, s, by); break;
         case 8: launch_quad_fused<T, 8>(P, axis, maxl, s, by); break;
