@@ -1328,6 +1328,11 @@ __host__ __device__ constexpr int quad_bt(int nt) {
                                                      : EMMI_LINE_FUSE_THREADS;
 }
 
+// nodes per thread whose right-hand sides a fused launch builds together
+#ifndef EMMI_LINE_RHS_UNROLL
+#define EMMI_LINE_RHS_UNROLL 4
+#endif
+
 template <typename T>
 __host__ __device__ constexpr bool quad_lds_records(int nt, int lpb = 16) {
   return nt > 0 && lpb * nt * SREC * (int)sizeof(ScrRec<T>) <= EMMI_QUAD_LDS_KB * 1024;
@@ -1370,6 +1375,9 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   __shared__ RS lrec[LR ? LPB * NT * SREC : 1];
   if (RHS) {
     const long nl = (long)J.nlo * J.nhi;
+    // unrolled: several nodes' stencil loads in flight per thread (the
+    // substitution sets the kernel's register count, this loop fits in it)
+#pragma unroll EMMI_LINE_RHS_UNROLL
     for (int idx = threadIdx.x; idx < LPB * N; idx += blockDim.x) {
       const int ll = idx % LPB, p = idx / LPB;
       const int ln = blockIdx.x * LPB + ll;
