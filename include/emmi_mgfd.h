@@ -390,9 +390,11 @@ int emmi_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int emmi_device_sync(void);
 /* Line-substitution kernel of the following launches (no counterpart in the
  * reference; for tests and tuning): 0 by line count (default), 1 one lane per
- * line half, 2 four lanes per line half.  Both solve the same line systems
- * exactly.  Graphs a solver captured earlier keep the kernels they captured.
- * Returns the previous mode, or -1 for an unknown mode. */
+ * line half, 2 four lanes per line half, 3 four lanes with the right-hand
+ * sides built by the substitution launch itself, 4 four lanes with the
+ * right-hand sides built by a launch of their own.  All solve the same line
+ * systems exactly.  Graphs a solver captured earlier keep the kernels they
+ * captured.  Returns the previous mode, or -1 for an unknown mode. */
 int emmi_set_line_kernel(int mode);
 /* kernel timing: accumulated milliseconds, launch counts and compulsory
  * bytes (each input read once, each output written once per launch at its

@@ -66,7 +66,9 @@ inline void line_job_colour(LineJob<T>& J, int a, int colour) {
 
 // substitution kernel of later launches: 0 by line count (or by the
 // autotuner's table where it has the launch's shape), 1 one lane per line
-// half, 2 four lanes per line half; returns the previous mode or -1
+// half, 2 four lanes per line half, 3 four lanes building their own
+// right-hand sides, 4 four lanes after a right-hand-side launch; returns the
+// previous mode or -1
 int set_line_kernel(int mode);
 
 // Autotuning: a launch shape (value size of the cycle precision, axis, line

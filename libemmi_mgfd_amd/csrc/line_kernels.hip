@@ -36,6 +36,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <utility>
 #include "stencil.h"
 #include "timing.h"
 
@@ -667,18 +668,192 @@ __device__ __forceinline__ void node_rhs_rec(const LineJob<T>& J, int pb, int pc
   }
 }
 
-template <typename T, int A>
-__device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, int pc, int p, long nl) {
+// the records of node p of a line into its chunked scratch records
+template <typename T>
+__device__ __forceinline__ void store_rhs_rec(const LineJob<T>& J, int line, int p, long nl,
+                                              const Ch<ScrRec<T>> (&c)[schunks<ScrRec<T>>()]) {
   using R = ScrRec<T>;
   constexpr int SC = schunks<R>(), V = Ch<R>::V;
-  Ch<R> c[SC] = {};
-  node_rhs_rec<T, A>(J, pb, pc, p, c);
   Ch<R>* rec = reinterpret_cast<Ch<R>*>(J.scr) + line + (long)p * SC * nl;
   if (p >= 1) {
 #pragma unroll
     for (int k = 0; k < SC; k++) rec[k * nl] = c[k];
   } else {
     rec[(8 / V) * nl] = c[8 / V];   // node 0: rho_0 only (values 8, 9: one chunk)
+  }
+}
+
+template <typename T, int A>
+__device__ __forceinline__ void node_rhs(const LineJob<T>& J, int line, int pb, int pc, int p, long nl) {
+  using R = ScrRec<T>;
+  constexpr int SC = schunks<R>(), V = Ch<R>::V;
+  Ch<R> c[SC] = {};
+  node_rhs_rec<T, A>(J, pb, pc, p, c);
+  store_rhs_rec<T>(J, line, p, nl, c);
+}
+
+// Right-hand sides of consecutive nodes of one line, marching along it.
+// The five rows of a node read 30 distinct values of u; 13 of them are read
+// again by node p + 1 (the same (b, c) offsets one node further back), so the
+// march carries those and loads 17 per node (with f and sigma-bar of the five
+// rows), against about 40 loads per node when each node is built on its own.
+// A column (k, db, dc) is component (A + k) % 3 at offsets db, dc along the
+// perpendicular axes B, C; the window holds it at da = lo..hi along the line.
+struct MarchCol {
+  int k, db, dc, lo, hi;
+};
+constexpr MarchCol kMarch[] = {
+    {0, -1, 0, -1, 0},  {0, 0, -1, -1, 0}, {0, 0, 0, -1, 0}, {0, 0, 1, -1, 0}, {0, 1, 0, -1, 0},
+    {1, -1, -1, 0, 0},  {1, -1, 0, -1, 1}, {1, -1, 1, 0, 0}, {1, 0, -1, 0, 0}, {1, 0, 0, -1, 1},
+    {1, 0, 1, 0, 0},    {2, -1, -1, 0, 0}, {2, -1, 0, 0, 0}, {2, 0, -1, -1, 1}, {2, 0, 0, -1, 1},
+    {2, 1, -1, 0, 0},   {2, 1, 0, 0, 0}};
+constexpr int kMarchCols = sizeof(kMarch) / sizeof(kMarch[0]);
+constexpr int march_col(int k, int db, int dc) {
+  for (int i = 0; i < kMarchCols; i++)
+    if (kMarch[i].k == k && kMarch[i].db == db && kMarch[i].dc == dc) return i;
+  return -1;
+}
+template <int I>
+struct ColIdx {
+  static constexpr int value = I;
+};
+template <typename F, int... I>
+__device__ __forceinline__ void each_col(const F& f, std::integer_sequence<int, I...>) {
+  (f(ColIdx<I>{}), ...);
+}
+template <typename F>
+__device__ __forceinline__ void each_col(const F& f) {
+  each_col(f, std::make_integer_sequence<int, kMarchCols>{});
+}
+
+struct MarchWin {
+  cxd v[kMarchCols][3];   // [column][da + 1]
+  // the value at (da, db, dc) from the node; a position the table lacks
+  // does not compile
+  template <int K, int DA, int DB, int DC>
+  __device__ __forceinline__ cxd at() const {
+    constexpr int c = march_col(K, DB, DC);
+    static_assert(c >= 0 && DA >= kMarch[c].lo && DA <= kMarch[c].hi, "position outside the march window");
+    return v[c][DA + 1];
+  }
+};
+
+// value KK of row X (0: its own component, 1 and 2 the next ones
+// cyclically) at offsets (dx, dy, dz) along the row's axes X, X+1, X+2, for
+// the row of the edge at offsets (0, RB, RC) from the node; everything in
+// the line frame (0 = A, 1 = B, 2 = C)
+template <int X, int RB, int RC, int KK, int dx, int dy, int dz>
+__device__ __forceinline__ cxd win_val(const MarchWin& w) {
+  constexpr int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  constexpr int da = (X == 0 ? dx : 0) + (Y == 0 ? dy : 0) + (Z == 0 ? dz : 0);
+  constexpr int db = RB + (X == 1 ? dx : 0) + (Y == 1 ? dy : 0) + (Z == 1 ? dz : 0);
+  constexpr int dc = RC + (X == 2 ? dx : 0) + (Y == 2 ? dy : 0) + (Z == 2 ? dz : 0);
+  return w.at<(X + KK) % 3, da, db, dc>();
+}
+// the thirteen values of that row in row_arith's order
+template <int X, int RB, int RC>
+__device__ __forceinline__ void win_row(const MarchWin& w, cxd (&v)[13]) {
+  v[0] = win_val<X, RB, RC, 0, 0, 0, 0>(w);
+  v[1] = win_val<X, RB, RC, 1, 1, 0, 0>(w);
+  v[2] = win_val<X, RB, RC, 1, 0, 0, 0>(w);
+  v[3] = win_val<X, RB, RC, 0, 0, 1, 0>(w);
+  v[4] = win_val<X, RB, RC, 1, 1, -1, 0>(w);
+  v[5] = win_val<X, RB, RC, 1, 0, -1, 0>(w);
+  v[6] = win_val<X, RB, RC, 0, 0, -1, 0>(w);
+  v[7] = win_val<X, RB, RC, 0, 0, 0, 1>(w);
+  v[8] = win_val<X, RB, RC, 2, 1, 0, 0>(w);
+  v[9] = win_val<X, RB, RC, 2, 0, 0, 0>(w);
+  v[10] = win_val<X, RB, RC, 0, 0, 0, -1>(w);
+  v[11] = win_val<X, RB, RC, 2, 1, 0, -1>(w);
+  v[12] = win_val<X, RB, RC, 2, 0, 0, -1>(w);
+}
+
+// the records of nodes p0 .. p1-1 of the line at (pb, pc) of job J, handed
+// to sink(p, c) in order; equal, value for value, to node_rhs_rec's (the
+// rows' arithmetic is row_arith in both).  Every position read lies in the
+// level: nodes -1 .. N along the line are read only where node_rhs_rec reads
+// them (da = -1 from node 1 on).
+template <typename T, int A, typename Sink>
+__device__ __forceinline__ void line_rhs_march(const LineJob<T>& J, int pb, int pc, int p0, int p1,
+                                               const Sink& sink) {
+  using RS = ScrRec<T>;
+  constexpr int B = (A + 1) % 3, C = (A + 2) % 3;
+  const Level<T>& L = J.L;
+  const typename cvec<T>::type* __restrict__ u = J.u;
+  const typename cvec<T>::type* __restrict__ f = J.f;
+  const long sa = L.es[A], sb = L.es[B], sc = L.es[C], nc = L.nc;
+  // geometry of the perpendicular axes: fixed along the line
+  const double iwb = L.IW[B][pb], iwbm = L.IW[B][pb - 1], hb = L.H[B][pb], wb = L.W[B][pb], wbm = L.W[B][pb - 1];
+  const double iwc = L.IW[C][pc], iwcm = L.IW[C][pc - 1], hc = L.H[C][pc], wc = L.W[C][pc], wcm = L.W[C][pc - 1];
+  // column ic (a compile-time index) at the node at flat index o, into the
+  // window slot of da (compile-time too)
+  auto load_col = [&](MarchWin& w, auto ic, auto dai, long o) {
+    constexpr int ci = decltype(ic)::value, da = decltype(dai)::value;
+    constexpr MarchCol m = kMarch[ci];
+    w.v[ci][da + 1] = ld<T>(u, (long)((A + m.k) % 3) * nc + o + m.db * sb + m.dc * sc);
+  };
+  MarchWin w;
+  long o = (long)pb * sb + (long)pc * sc + (long)p0 * sa;
+  // the slots of da = -1 of a run from node 0 are never read (rows B and C
+  // start at node 1, after a shift); they take node 0's values so that every
+  // slot is written unconditionally and the window stays in registers
+  const long om = p0 >= 1 ? o - sa : o;
+  each_col([&](auto ic) {
+    constexpr MarchCol m = kMarch[decltype(ic)::value];
+    if constexpr (m.lo < 0) load_col(w, ic, ColIdx<-1>{}, om);
+    load_col(w, ic, ColIdx<0>{}, o);
+    if constexpr (m.hi > 0) load_col(w, ic, ColIdx<1>{}, o + sa);
+  });
+  double iwa_m = p0 >= 1 ? L.IW[A][p0 - 1] : 0.0;
+  for (int p = p0; p < p1; p++, o += sa) {
+    const double iwa = L.IW[A][p], wa = L.W[A][p], ha = L.H[A][p];
+    Ch<RS> c[schunks<RS>()] = {};
+    cxd v[13];
+    {
+      // rho_p: row A of the node, local indices (p, pb, pc)
+      win_row<0, 0, 0>(w, v);
+      const RowGeo g{iwa, iwb, iwbm, iwc, iwcm, hc, hb, wa, L.omu * (double)L.sbar[(long)A * nc + o]};
+      const cxd rho = ld<T>(f, (long)A * nc + o) - row_arith(v, g);
+      put(c, 8, rho.re);
+      put(c, 9, rho.im);
+    }
+    if (p >= 1) {
+      cxd g;
+      // row B at pb - 1 and at pb: local indices (pb', pc, p)
+      win_row<1, -1, 0>(w, v);
+      g = ld<T>(f, (long)B * nc + o - sb) -
+          row_arith(v, RowGeo{iwbm, iwc, iwcm, iwa, iwa_m, ha, hc, wbm, L.omu * (double)L.sbar[(long)B * nc + o - sb]});
+      put(c, 0, g.re); put(c, 1, g.im);
+      win_row<1, 0, 0>(w, v);
+      g = ld<T>(f, (long)B * nc + o) -
+          row_arith(v, RowGeo{iwb, iwc, iwcm, iwa, iwa_m, ha, hc, wb, L.omu * (double)L.sbar[(long)B * nc + o]});
+      put(c, 2, g.re); put(c, 3, g.im);
+      // row C at pc - 1 and at pc: local indices (pc', p, pb)
+      win_row<2, 0, -1>(w, v);
+      g = ld<T>(f, (long)C * nc + o - sc) -
+          row_arith(v, RowGeo{iwcm, iwa, iwa_m, iwb, iwbm, hb, ha, wcm, L.omu * (double)L.sbar[(long)C * nc + o - sc]});
+      put(c, 4, g.re); put(c, 5, g.im);
+      win_row<2, 0, 0>(w, v);
+      g = ld<T>(f, (long)C * nc + o) -
+          row_arith(v, RowGeo{iwc, iwa, iwa_m, iwb, iwbm, hb, ha, wc, L.omu * (double)L.sbar[(long)C * nc + o]});
+      put(c, 6, g.re); put(c, 7, g.im);
+    }
+    sink(p, c);
+    if (p + 1 < p1) {
+      // node p + 1: shift the window back one node and load its leading values
+      each_col([&](auto ic) {
+        constexpr int ci = decltype(ic)::value;
+        constexpr MarchCol m = kMarch[ci];
+        if constexpr (m.lo < 0) w.v[ci][0] = w.v[ci][1];
+        if constexpr (m.hi > 0) {
+          w.v[ci][1] = w.v[ci][2];
+          load_col(w, ic, ColIdx<1>{}, o + 2 * sa);
+        } else {
+          load_col(w, ic, ColIdx<0>{}, o + sa);
+        }
+      });
+    }
+    iwa_m = iwa;
   }
 }
 
@@ -1332,6 +1507,22 @@ __host__ __device__ constexpr int quad_bt(int nt) {
 #ifndef EMMI_LINE_RHS_UNROLL
 #define EMMI_LINE_RHS_UNROLL 4
 #endif
+// 1: a fused launch of lines of 64 nodes or more builds its right-hand sides
+// by marching along the lines (line_rhs_march; shorter lines leave runs of a
+// node or two, with nothing to carry); 0: node by node.  Lines along x keep
+// the node-by-node loop: there a wave's lanes take runs 8 nodes apart in 16
+// rows, twice the cache lines per load of four neighbouring nodes (and in
+// FP64 the record stores scatter alike).  Measured at 128^3 per launch of a
+// colour with one right-hand side: along y and z the march takes 143 -> 137
+// and 145 -> 139 us (FP64, 128 nodes), along x 156 -> 296 us (FP64) and 100
+// -> 107 us (FP32).
+#ifndef EMMI_LINE_RHS_MARCH
+#define EMMI_LINE_RHS_MARCH 1
+#endif
+template <typename T, int A>
+__host__ __device__ constexpr bool rhs_march(int nt) {
+  return EMMI_LINE_RHS_MARCH != 0 && nt >= 64 && A != 0;
+}
 
 template <typename T>
 __host__ __device__ constexpr bool quad_lds_records(int nt, int lpb = 16) {
@@ -1375,21 +1566,43 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
   __shared__ RS lrec[LR ? LPB * NT * SREC : 1];
   if (RHS) {
     const long nl = (long)J.nlo * J.nhi;
-    // unrolled: several nodes' stencil loads in flight per thread (the
-    // substitution sets the kernel's register count, this loop fits in it)
-#pragma unroll EMMI_LINE_RHS_UNROLL
-    for (int idx = threadIdx.x; idx < LPB * N; idx += blockDim.x) {
-      const int ll = idx % LPB, p = idx / LPB;
+    if constexpr (rhs_march<T, A>(NT)) {
+      // each line's nodes in CH consecutive runs, a run per thread, lines
+      // fastest (a wave's loads cover LPB neighbouring lines), each run
+      // marching along its line
+      constexpr int CH = BT / LPB;
+      const int ll = (int)threadIdx.x % LPB, ch = (int)threadIdx.x / LPB;
+      const int run = (N + CH - 1) / CH;
+      const int p0 = ch * run, p1 = min(N, p0 + run);
       const int ln = blockIdx.x * LPB + ll;
       int b2, c2;
-      if (line_of<T, A>(J, ln, b2, c2)) {
-        if constexpr (LR) {
-          Ch<RS> c[SC] = {};
-          node_rhs_rec<T, A>(J, b2, c2, p, c);
+      if (ch < CH && p0 < p1 && line_of<T, A>(J, ln, b2, c2)) {
+        line_rhs_march<T, A>(J, b2, c2, p0, p1, [&](int p, const Ch<RS>(&c)[SC]) {
+          if constexpr (LR) {
 #pragma unroll
-          for (int v = 0; v < SREC; v++) lrec[(p * LPB + ll) * SREC + v] = (RS)val(c, v);
-        } else {
-          node_rhs<T, A>(J, ln, b2, c2, p, nl);
+            for (int v = 0; v < SREC; v++) lrec[(p * LPB + ll) * SREC + v] = (RS)val(c, v);
+          } else {
+            store_rhs_rec<T>(J, ln, p, nl, c);
+          }
+        });
+      }
+    } else {
+      // unrolled: several nodes' stencil loads in flight per thread (the
+      // substitution sets the kernel's register count, this loop fits in it)
+#pragma unroll EMMI_LINE_RHS_UNROLL
+      for (int idx = threadIdx.x; idx < LPB * N; idx += blockDim.x) {
+        const int ll = idx % LPB, p = idx / LPB;
+        const int ln = blockIdx.x * LPB + ll;
+        int b2, c2;
+        if (line_of<T, A>(J, ln, b2, c2)) {
+          if constexpr (LR) {
+            Ch<RS> c[SC] = {};
+            node_rhs_rec<T, A>(J, b2, c2, p, c);
+#pragma unroll
+            for (int v = 0; v < SREC; v++) lrec[(p * LPB + ll) * SREC + v] = (RS)val(c, v);
+          } else {
+            node_rhs<T, A>(J, ln, b2, c2, p, nl);
+          }
         }
       }
     }
@@ -1725,7 +1938,7 @@ void line_colour_counts(const int* n, int a, int colour, int* nlo, int* nhi) {
 static int g_line_kernel = 0;
 
 int set_line_kernel(int mode) {
-  if (mode < 0 || mode > 2) return -1;
+  if (mode < 0 || mode > 4) return -1;
   const int prev = g_line_kernel;
   g_line_kernel = mode;
   return prev;
@@ -1892,6 +2105,8 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
   // sides keep the one-lane kernel (3 slots at 128^3: 1% faster), a single
   // right-hand side takes the quad-lane one (430 against 567 us per sweep)
   if (choice == LINE_RULE && g_line_kernel == 0) choice = tuned_choice(line_shape(P, axis));
+  if (choice == LINE_RULE && g_line_kernel == 3) choice = LINE_QUAD_FUSED;
+  if (choice == LINE_RULE && g_line_kernel == 4) choice = LINE_QUAD_SEPARATE;
   bool quad, fuse;
   if (choice == LINE_ONE_LANE || choice == LINE_QUAD_FUSED || choice == LINE_QUAD_SEPARATE) {
     quad = choice != LINE_ONE_LANE;

@@ -197,6 +197,40 @@ def test_line_kernels_agree(tiny, axis, precision, tol):
     assert rel(outs[1], outs[0]) < tol
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_line_rhs_march_matches_node_records(precision):
+    """The fused quad-lane launch builds its lines' right-hand sides by
+    marching along them (values shared by neighbouring nodes carried in
+    registers); the separate right-hand-side launch builds each node on its
+    own.  Both evaluate every operator row with the same arithmetic, so a
+    sweep gives the same result to the last bit.  Level of 128 x 128 x 64
+    cells: 128-node lines along x and y (records in LDS in FP32, in memory in
+    FP64), 64-node lines along z, with an air layer and a layered earth."""
+    from libemmi_mgfd_amd import native
+    n = (128, 128, 64)
+    rng = np.random.default_rng(11)
+    xs = [np.cumsum(np.r_[0.0, rng.uniform(20.0, 80.0, m)]).astype(np.float32) for m in n]
+    s11 = np.empty((n[2], n[1], n[0]), np.float32)
+    s11[:8] = 1e-8
+    s11[8:] = rng.uniform(0.01, 2.0, (n[2] - 8, 1, 1)).astype(np.float32)
+    L = native.lib()
+    u, f = random_field(n, 31), random_field(n, 32)
+    S = native.Solver(n, *xs, s11, s11, s11, O.omega_f32(0.5), precision=precision)
+    prev = L.emmi_set_line_kernel(3)
+    try:
+        for axis in range(3):
+            print(f"line sweep precision {precision} axis {axis}: fused march vs separate records", flush=True)
+            outs = []
+            for mode in (3, 4):
+                L.emmi_set_line_kernel(mode)
+                outs.append(S.check_op(1, 0, 0, axis, 0, u, f))
+            assert np.all(np.isfinite(outs[0]))
+            assert np.array_equal(outs[0], outs[1]), f"axis {axis}: max rel diff {rel(outs[0], outs[1]):.2e}"
+    finally:
+        L.emmi_set_line_kernel(prev)
+        S.close()
+
+
 @pytest.mark.parametrize("pattern", [0, 1, 2])
 def test_transfer_fp64(tiny, pattern):
     s, g, n, L0 = tiny
