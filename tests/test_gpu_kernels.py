@@ -197,17 +197,17 @@ def test_line_kernels_agree(tiny, axis, precision, tol):
     assert rel(outs[1], outs[0]) < tol
 
 
+@pytest.mark.parametrize("n", [(128, 128, 64), (64, 96, 48)], ids=["128x128x64", "64x96x48"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_line_rhs_march_matches_node_records(precision):
+def test_line_rhs_march_matches_node_records(precision, n):
     """The fused quad-lane launch builds its lines' right-hand sides by
     marching along them (values shared by neighbouring nodes carried in
     registers); the separate right-hand-side launch builds each node on its
     own.  Both evaluate every operator row with the same arithmetic, so a
-    sweep gives the same result to the last bit.  Level of 128 x 128 x 64
-    cells: 128-node lines along x and y (records in LDS in FP32, in memory in
-    FP64), 64-node lines along z, with an air layer and a layered earth."""
+    sweep gives the same result to the last bit.  Lines of 128, 96, 64 and 48
+    nodes (records in LDS and in memory; the march on the 64- and 128-node
+    lines along y and z), with an air layer and a layered earth."""
     from libemmi_mgfd_amd import native
-    n = (128, 128, 64)
     rng = np.random.default_rng(11)
     xs = [np.cumsum(np.r_[0.0, rng.uniform(20.0, 80.0, m)]).astype(np.float32) for m in n]
     s11 = np.empty((n[2], n[1], n[0]), np.float32)
@@ -219,7 +219,7 @@ def test_line_rhs_march_matches_node_records(precision):
     prev = L.emmi_set_line_kernel(3)
     try:
         for axis in range(3):
-            print(f"line sweep precision {precision} axis {axis}: fused march vs separate records", flush=True)
+            print(f"line sweep {n} precision {precision} axis {axis}: fused vs separate records", flush=True)
             outs = []
             for mode in (3, 4):
                 L.emmi_set_line_kernel(mode)
