@@ -1519,6 +1519,12 @@ __host__ __device__ constexpr int quad_bt(int nt) {
 #ifndef EMMI_LINE_RHS_MARCH
 #define EMMI_LINE_RHS_MARCH 1
 #endif
+// fused launches along x build their node-by-node records with a wave on
+// this many neighbouring nodes of each of 64 / EMMI_LINE_RHS_XGROUP lines
+// (0: 4 nodes of 16 lines, as along y and z)
+#ifndef EMMI_LINE_RHS_XGROUP
+#define EMMI_LINE_RHS_XGROUP 16
+#endif
 template <typename T, int A>
 __host__ __device__ constexpr bool rhs_march(int nt) {
   return EMMI_LINE_RHS_MARCH != 0 && nt >= 64 && A != 0;
@@ -1591,7 +1597,12 @@ __global__ void __launch_bounds__(BT) k_line_solve_q(LinePack<T> P) {
       // substitution sets the kernel's register count, this loop fits in it)
 #pragma unroll EMMI_LINE_RHS_UNROLL
       for (int idx = threadIdx.x; idx < LPB * N; idx += blockDim.x) {
-        const int ll = idx % LPB, p = idx / LPB;
+        // along x (lines in separate rows) a wave takes XN neighbouring
+        // nodes of 64 / XN lines instead of 4 nodes of 16 lines
+        constexpr int XN = EMMI_LINE_RHS_XGROUP;
+        constexpr bool XG = A == 0 && NT > 0 && XN > 0 && NT % XN == 0;
+        const int ll = XG ? (idx / XN) % LPB : idx % LPB;
+        const int p = XG ? idx / (XN * LPB) * XN + idx % XN : idx / LPB;
         const int ln = blockIdx.x * LPB + ll;
         int b2, c2;
         if (line_of<T, A>(J, ln, b2, c2)) {
