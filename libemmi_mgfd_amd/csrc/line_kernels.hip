@@ -872,6 +872,26 @@ __global__ void __launch_bounds__(256, EMMI_LINE_RHS_MIN_WAVES) k_line_rhs(LineP
   node_rhs<T, A>(J, line, pb, pc, p, nl);
 }
 
+// The same records built by marching (line_rhs_march): a thread per run of
+// `run` consecutive nodes of a line, lines fastest (y and z lines, whose
+// neighbours lie side by side along x)
+template <typename T, int A>
+__global__ void __launch_bounds__(256) k_line_rhs_run(LinePack<T> P, int run) {
+  const LineJob<T>& J = P.job[blockIdx.y];
+  const long nl = (long)J.nlo * J.nhi;
+  const int N = J.L.n[A];
+  const long nruns = (N + run - 1) / run;
+  const long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nl * nruns) return;
+  const int r = (int)(t / nl), line = (int)(t - (long)r * nl);
+  const int p0 = r * run, p1 = min(N, p0 + run);
+  int pb, pc;
+  line_of<T, A>(J, line, pb, pc);
+  line_rhs_march<T, A>(J, pb, pc, p0, p1, [&](int p, const Ch<ScrRec<T>>(&c)[schunks<ScrRec<T>>()]) {
+    store_rhs_rec<T>(J, line, p, nl, c);
+  });
+}
+
 // Right-hand sides for lines along x (axis 0): consecutive lines lie in
 // different x rows, so a thread per (node, line) with lines fastest would
 // load one cache line per lane.  A block takes a tile of 16 nodes x 16 lines,
@@ -1519,6 +1539,10 @@ __host__ __device__ constexpr int quad_bt(int nt) {
 #ifndef EMMI_LINE_RHS_MARCH
 #define EMMI_LINE_RHS_MARCH 1
 #endif
+// nodes per thread of the separate right-hand-side launch that marches
+#ifndef EMMI_LINE_RHS_RUN
+#define EMMI_LINE_RHS_RUN 8
+#endif
 // fused launches along x build their node-by-node records with a wave on
 // this many neighbouring nodes of each of 64 / EMMI_LINE_RHS_XGROUP lines
 // (0: 4 nodes of 16 lines, as along y and z)
@@ -2135,6 +2159,13 @@ static void launch_pack(const LinePack<T>& P, int axis, bool factor, hipStream_t
     const int nlb = (maxl + XT - 1) / XT, npb = (maxn + XT - 1) / XT;
     const dim3 g((unsigned)(nlb * npb), P.njobs), b(XT * XT);
     launch_timed(k_line_rhs_x<T>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P, nlb);
+  } else if (EMMI_LINE_RHS_MARCH != 0 && maxna >= 64) {
+    // lines of 64 nodes and more march (as the fused launches do)
+    const int run = EMMI_LINE_RHS_RUN;
+    const long nth = (long)maxl * ((maxna + run - 1) / run);
+    const dim3 g((unsigned)((nth + 255) / 256), P.njobs), b(256);
+    if (axis == 1) launch_timed(k_line_rhs_run<T, 1>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P, run);
+    else launch_timed(k_line_rhs_run<T, 2>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P, run);
   } else {
     const dim3 g((unsigned)((maxnodes + 255) / 256), P.njobs), b(256);
     if (axis == 1) launch_timed(k_line_rhs<T, 1>, g, b, s, timing::LINE_RHS, Bytes{0.0, rhs_bytes}, P);
