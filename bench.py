@@ -64,9 +64,9 @@ HBM_PEAK_GBS = 8000.0
 # with two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) on the
 # mixed+batched 128^3 step, each scaled by the same counter on the run's
 # 103 MB device-to-device copies (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00):
-# 85.20 MB per launch against 8.88 MB compulsory (the stored FP64 factor
+# 87.30 MB per launch against 8.97 MB compulsory (the stored FP64 factor
 # records dominate)
-LINE_SWEEP_TRAFFIC_RATIO = 85.20 / 8.88
+LINE_SWEEP_TRAFFIC_RATIO = 87.30 / 8.97
 
 
 def parse():
